@@ -1,0 +1,193 @@
+// Lexer entry points of the C ABI (lex_string, the device form, and the
+// lexer-only wire record).
+#include <hip/hip_runtime.h>
+
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/alpacc_hip.h"
+#include "lexer.hpp"
+#include "runtime.hpp"
+
+namespace alpacc {
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) return fail(ALPACC_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+// Positions and counts travel in 31-bit fields of the lookback words.
+constexpr uint64_t LEX_MAX_INPUT = (1ull << 31) - 1;
+
+int lex_on_device(Context& c, int mode, const uint8_t* d_in, uint64_t n, hipStream_t st, uint64_t* count) {
+  const GrammarDesc* g = c.g;
+  *count = 0;
+  if (n == 0) return ALPACC_VALID;
+  if (n > LEX_MAX_INPUT) return fail(ALPACC_ERR_LIMIT, "input of %llu bytes exceeds the %llu-byte launch limit",
+                                     (unsigned long long)n, (unsigned long long)LEX_MAX_INPUT);
+  const int lb = g->length_bytes;
+  const uint32_t tile = lex_tile_bytes();
+  const uint32_t ntiles = (uint32_t)((n + tile - 1) / tile);
+  int r;
+  if ((r = c.lx_term.reserve(n)) || (r = c.lx_start.reserve(n * 8)) || (r = c.lx_len.reserve(n * lb)) ||
+      (r = c.lx_status.reserve((size_t)ntiles * 12 + 16)))
+    return r;
+  LexParams p{};
+  p.in = d_in;
+  p.n = n;
+  p.aligned16 = ((uintptr_t)d_in & 15) == 0;
+  p.byte_class = c.d_class;
+  p.step = c.d_step;
+  p.compose = c.d_compose;
+  p.endo_init = c.d_endo_init;
+  const bool lex_ids = mode == ALPACC_MODE_LEXER;
+  p.delta = lex_ids ? c.d_delta_lex : c.d_delta_both;
+  p.nstate = g->nstate;
+  p.nendo = g->nendo;
+  p.nclass = g->nclass;
+  p.ignore = lex_ids ? g->ignore_lex : g->ignore_both;
+  p.dead = lex_ids ? g->dead_lex : g->dead_both;
+  p.max_len = lb >= 8 ? ~0ull : ((1ull << (8 * lb)) - 1);
+  p.length_bytes = lb;
+  p.terminals = c.lx_term.as<uint8_t>();
+  p.starts = c.lx_start.as<int64_t>();
+  p.lengths = c.lx_len.ptr;
+  uint64_t* small = c.small.as<uint64_t>();
+  p.count = small;                                        // [0]
+  p.flags = reinterpret_cast<uint32_t*>(small + 1);       // [8..12)
+  p.ticket = reinterpret_cast<uint32_t*>(small + 1) + 1;  // [12..16)
+  p.count_status = c.lx_status.as<uint64_t>();
+  p.endo_status = reinterpret_cast<uint32_t*>(p.count_status + ntiles);
+  p.ntiles = ntiles;
+  if (!c.lex_grid) HIP_TRY(lex_grid_size(p, &c.lex_grid));
+  if ((r = zero_words(c.small.ptr, 16, st)) || (r = zero_words(c.lx_status.ptr, (size_t)ntiles * 12, st))) return r;
+  int grid = c.lex_grid < (int)ntiles ? c.lex_grid : (int)ntiles;
+  if (c.timing) HIP_TRY(hipEventRecord(c.ev[0], st));
+  HIP_TRY(lex_launch(p, grid, st));
+  if (c.timing) HIP_TRY(hipEventRecord(c.ev[1], st));
+  HIP_TRY(hipMemcpyAsync(c.host_small, c.small.ptr, 16, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (c.timing) {
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, c.ev[0], c.ev[1]));
+    c.lex_ms += ms;
+    c.lex_launches++;
+  }
+  uint32_t flags = reinterpret_cast<uint32_t*>(c.host_small + 1)[0];
+  *count = c.host_small[0];
+  return flags ? ALPACC_REJECT : ALPACC_VALID;
+}
+
+// Stage a host buffer on the device (grow-only staging buffer).
+int stage_input(Context& c, const void* host, uint64_t bytes, hipStream_t st, const uint8_t** d) {
+  int r;
+  if ((r = c.in_copy.reserve(bytes + 16))) return r;
+  if (bytes) HIP_TRY(hipMemcpyAsync(c.in_copy.ptr, host, bytes, hipMemcpyHostToDevice, st));
+  *d = c.in_copy.as<uint8_t>();
+  return 0;
+}
+
+int lexemes_to_host(Context& c, int mode, uint64_t count, hipStream_t st, alpacc_hip_lexemes* out) {
+  const GrammarDesc* g = c.g;
+  const int tb = mode == ALPACC_MODE_LEXER ? g->terminal_bytes_lex : g->terminal_bytes_both;
+  const int lb = g->length_bytes;
+  out->count = count;
+  out->terminals = malloc(count * tb + 1);
+  out->starts = static_cast<int64_t*>(malloc(count * 8 + 1));
+  out->lengths = malloc(count * lb + 1);
+  if (!out->terminals || !out->starts || !out->lengths) {
+    free(out->terminals);
+    free(out->starts);
+    free(out->lengths);
+    out->terminals = out->starts = nullptr;
+    out->lengths = nullptr;
+    return fail(ALPACC_ERR_NOMEM, "host allocation failed");
+  }
+  if (count) {
+    HIP_TRY(hipMemcpyAsync(out->terminals, c.lx_term.ptr, count * tb, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(out->starts, c.lx_start.ptr, count * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(out->lengths, c.lx_len.ptr, count * lb, hipMemcpyDeviceToHost, st));
+  }
+  HIP_TRY(hipStreamSynchronize(st));
+  return 0;
+}
+
+// Lexer-only response record: u8 1, u64 count, then (terminal, start,
+// length) per lexeme with native widths (C/Generator.hs:88-105).
+int lexer_record(Context& c, uint64_t n, const uint8_t* payload, std::vector<uint8_t>& rec) {
+  hipStream_t st = c.stream;
+  const uint8_t* d_in = nullptr;
+  int r;
+  if ((r = stage_input(c, payload, n, st, &d_in))) return r;
+  uint64_t count = 0;
+  r = lex_on_device(c, ALPACC_MODE_LEXER, d_in, n, st, &count);
+  if (r < 0) return r;
+  if (r == ALPACC_REJECT) {
+    rec.assign(1, 0);
+    return ALPACC_REJECT;
+  }
+  alpacc_hip_lexemes lx{};
+  if ((r = lexemes_to_host(c, ALPACC_MODE_LEXER, count, st, &lx))) return r;
+  const GrammarDesc* g = c.g;
+  const int tb = g->terminal_bytes_lex, ib = g->index_bytes, lb = g->length_bytes;
+  const size_t per = tb + ib + lb;
+  rec.resize(1 + 8 + count * per);
+  rec[0] = 1;
+  memcpy(&rec[1], &count, 8);
+  uint8_t* o = rec.data() + 9;
+  const uint8_t* t = static_cast<const uint8_t*>(lx.terminals);
+  const uint8_t* l = static_cast<const uint8_t*>(lx.lengths);
+  for (uint64_t i = 0; i < count; ++i) {
+    memcpy(o, t + i * tb, tb);
+    memcpy(o + tb, reinterpret_cast<const uint8_t*>(lx.starts + i), ib);
+    memcpy(o + tb + ib, l + i * lb, lb);
+    o += per;
+  }
+  free(lx.terminals);
+  free(lx.starts);
+  free(lx.lengths);
+  return ALPACC_VALID;
+}
+
+}  // namespace alpacc
+
+using namespace alpacc;
+
+extern "C" {
+
+int alpacc_hip_lex_device(alpacc_hip_ctx* h, int mode, const uint8_t* d_str, uint64_t n, alpacc_hip_lexemes* d_out,
+                          void* stream) {
+  if (!h || !d_out || (n && !d_str)) return fail(ALPACC_ERR_ARG, "null argument");
+  if (mode != ALPACC_MODE_LEXER && mode != ALPACC_MODE_BOTH) return fail(ALPACC_ERR_ARG, "bad lexer mode %d", mode);
+  Context& c = h->c;
+  HIP_TRY(hipSetDevice(c.device));
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c.stream;
+  uint64_t count = 0;
+  int r = lex_on_device(c, mode, d_str, n, st, &count);
+  if (r < 0) return r;
+  d_out->count = count;
+  d_out->terminals = c.lx_term.ptr;
+  d_out->starts = c.lx_start.as<int64_t>();
+  d_out->lengths = c.lx_len.ptr;
+  return r;
+}
+
+int alpacc_hip_lex_string(alpacc_hip_ctx* h, int mode, const uint8_t* str, uint64_t n, alpacc_hip_lexemes* out) {
+  if (!h || !out || (n && !str)) return fail(ALPACC_ERR_ARG, "null argument");
+  if (mode != ALPACC_MODE_LEXER && mode != ALPACC_MODE_BOTH) return fail(ALPACC_ERR_ARG, "bad lexer mode %d", mode);
+  Context& c = h->c;
+  HIP_TRY(hipSetDevice(c.device));
+  memset(out, 0, sizeof *out);
+  const uint8_t* d_in = nullptr;
+  int r;
+  if ((r = stage_input(c, str, n, c.stream, &d_in))) return r;
+  uint64_t count = 0;
+  r = lex_on_device(c, mode, d_in, n, c.stream, &count);
+  if (r != ALPACC_VALID) return r;
+  if ((r = lexemes_to_host(c, mode, count, c.stream, out))) return r;
+  return ALPACC_VALID;
+}
+
+}  // extern "C"

@@ -1,0 +1,192 @@
+// Parser and combined entry points of the C ABI (parse_test, the combined
+// lex + parse + tree run, their device forms and wire records).
+#include <hip/hip_runtime.h>
+
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/alpacc_hip.h"
+#include "parser.hpp"
+#include "runtime.hpp"
+
+namespace alpacc {
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) return fail(ALPACC_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+int stage_input(Context& c, const void* host, uint64_t bytes, hipStream_t st, const uint8_t** d);
+int lexemes_to_host(Context& c, int mode, uint64_t count, hipStream_t st, alpacc_hip_lexemes* out);
+
+static int lex_parse_device(Context& c, const uint8_t* d_in, uint64_t n, hipStream_t st, alpacc_hip_tree* out) {
+  uint64_t count = 0;
+  int r = lex_on_device(c, ALPACC_MODE_BOTH, d_in, n, st, &count);
+  if (r != ALPACC_VALID) return r;
+  uint64_t np = 0;
+  r = parse_on_device(c, c.lx_term.as<uint8_t>(), count, true, st, &np);
+  if (r != ALPACC_VALID) return r;
+  out->tokens.count = count;
+  out->tokens.terminals = c.lx_term.ptr;
+  out->tokens.starts = c.lx_start.as<int64_t>();
+  out->tokens.lengths = c.lx_len.ptr;
+  out->num_nodes = np - count;
+  out->productions = c.tr_prod.ptr;
+  out->parents = c.tr_parent.as<int64_t>();
+  out->token_parents = c.tr_tokpar.as<int64_t>();
+  return ALPACC_VALID;
+}
+
+static void* host_copy(const void* d, size_t bytes, hipStream_t st, int* err) {
+  void* h = malloc(bytes ? bytes : 1);
+  if (!h) {
+    *err = fail(ALPACC_ERR_NOMEM, "host allocation failed");
+    return nullptr;
+  }
+  if (bytes) {
+    hipError_t e = hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) {
+      free(h);
+      *err = fail(ALPACC_ERR_HIP, "hipMemcpyAsync: %s", hipGetErrorString(e));
+      return nullptr;
+    }
+  }
+  return h;
+}
+
+// Parser-only record: u8 1, u64 count, production_t ids (C/Generator.hs:64-84).
+int parser_record(Context& c, uint64_t n, const uint8_t* payload, std::vector<uint8_t>& rec) {
+  hipStream_t st = c.stream;
+  const uint8_t* d_tok = nullptr;
+  int r;
+  if ((r = stage_input(c, payload, n, st, &d_tok))) return r;
+  uint64_t np = 0;
+  r = parse_on_device(c, d_tok, n, false, st, &np);
+  if (r < 0) return r;
+  if (r == ALPACC_REJECT) {
+    rec.assign(1, 0);
+    return r;
+  }
+  rec.resize(9 + np);
+  rec[0] = 1;
+  memcpy(&rec[1], &np, 8);
+  if (np) HIP_TRY(hipMemcpyAsync(rec.data() + 9, c.ps_b.ptr, np, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return ALPACC_VALID;
+}
+
+// Combined record (C/Generator.hs:114-152): SoA tokens, then the compact tree.
+int both_record(Context& c, uint64_t n, const uint8_t* payload, std::vector<uint8_t>& rec) {
+  hipStream_t st = c.stream;
+  const uint8_t* d_in = nullptr;
+  int r;
+  if ((r = stage_input(c, payload, n, st, &d_in))) return r;
+  alpacc_hip_tree t{};
+  r = lex_parse_device(c, d_in, n, st, &t);
+  if (r < 0) return r;
+  if (r == ALPACC_REJECT) {
+    rec.assign(1, 0);
+    return r;
+  }
+  const GrammarDesc* g = c.g;
+  const uint64_t T = t.tokens.count, N = t.num_nodes;
+  const int tb = g->terminal_bytes_both, ib = g->index_bytes, lb = g->length_bytes, pb = g->production_bytes;
+  rec.resize(1 + 8 + T * (tb + ib + lb) + 8 + N * (pb + ib) + T * ib);
+  uint8_t* o = rec.data();
+  *o++ = 1;
+  memcpy(o, &T, 8);
+  o += 8;
+  auto pull = [&](const void* d, size_t bytes) -> int {
+    if (bytes) HIP_TRY(hipMemcpyAsync(o, d, bytes, hipMemcpyDeviceToHost, st));
+    o += bytes;
+    return 0;
+  };
+  if ((r = pull(t.tokens.terminals, T * tb)) || (r = pull(t.tokens.starts, T * ib)) ||
+      (r = pull(t.tokens.lengths, T * lb)))
+    return r;
+  memcpy(o, &N, 8);
+  o += 8;
+  if ((r = pull(t.productions, N * pb)) || (r = pull(t.parents, N * ib)) || (r = pull(t.token_parents, T * ib)))
+    return r;
+  HIP_TRY(hipStreamSynchronize(st));
+  return ALPACC_VALID;
+}
+
+}  // namespace alpacc
+
+using namespace alpacc;
+
+extern "C" {
+
+int alpacc_hip_parse_device(alpacc_hip_ctx* h, const void* d_tokens, uint64_t n, void** d_prods, uint64_t* num_prods,
+                            void* stream) {
+  if (!h || !d_prods || !num_prods || (n && !d_tokens)) return fail(ALPACC_ERR_ARG, "null argument");
+  Context& c = h->c;
+  HIP_TRY(hipSetDevice(c.device));
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c.stream;
+  int r = parse_on_device(c, static_cast<const uint8_t*>(d_tokens), n, false, st, num_prods);
+  *d_prods = r == ALPACC_VALID ? c.ps_b.ptr : nullptr;
+  return r;
+}
+
+int alpacc_hip_parse_test(alpacc_hip_ctx* h, const void* tokens, uint64_t n, void** prods, uint64_t* num_prods) {
+  if (!h || !prods || !num_prods || (n && !tokens)) return fail(ALPACC_ERR_ARG, "null argument");
+  Context& c = h->c;
+  HIP_TRY(hipSetDevice(c.device));
+  *prods = nullptr;
+  *num_prods = 0;
+  const uint8_t* d_tok = nullptr;
+  int r;
+  if ((r = stage_input(c, tokens, n * c.g->terminal_bytes_both, c.stream, &d_tok))) return r;
+  uint64_t np = 0;
+  r = parse_on_device(c, d_tok, n, false, c.stream, &np);
+  if (r != ALPACC_VALID) return r;
+  int err = 0;
+  void* hp = host_copy(c.ps_b.ptr, np, c.stream, &err);
+  if (!hp) return err;
+  HIP_TRY(hipStreamSynchronize(c.stream));
+  *prods = hp;
+  *num_prods = np;
+  return ALPACC_VALID;
+}
+
+int alpacc_hip_lex_parse_device(alpacc_hip_ctx* h, const uint8_t* d_str, uint64_t n, alpacc_hip_tree* d_out,
+                                void* stream) {
+  if (!h || !d_out || (n && !d_str)) return fail(ALPACC_ERR_ARG, "null argument");
+  Context& c = h->c;
+  HIP_TRY(hipSetDevice(c.device));
+  memset(d_out, 0, sizeof *d_out);
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c.stream;
+  return lex_parse_device(c, d_str, n, st, d_out);
+}
+
+int alpacc_hip_lex_parse(alpacc_hip_ctx* h, const uint8_t* str, uint64_t n, alpacc_hip_tree* out) {
+  if (!h || !out || (n && !str)) return fail(ALPACC_ERR_ARG, "null argument");
+  Context& c = h->c;
+  HIP_TRY(hipSetDevice(c.device));
+  memset(out, 0, sizeof *out);
+  const uint8_t* d_in = nullptr;
+  int r;
+  if ((r = stage_input(c, str, n, c.stream, &d_in))) return r;
+  alpacc_hip_tree t{};
+  r = lex_parse_device(c, d_in, n, c.stream, &t);
+  if (r != ALPACC_VALID) return r;
+  const GrammarDesc* g = c.g;
+  const uint64_t T = t.tokens.count, N = t.num_nodes;
+  int err = 0;
+  out->tokens.count = T;
+  out->num_nodes = N;
+  out->tokens.terminals = host_copy(t.tokens.terminals, T * g->terminal_bytes_both, c.stream, &err);
+  out->tokens.starts = static_cast<int64_t*>(host_copy(t.tokens.starts, T * 8, c.stream, &err));
+  out->tokens.lengths = host_copy(t.tokens.lengths, T * g->length_bytes, c.stream, &err);
+  out->productions = host_copy(t.productions, N * g->production_bytes, c.stream, &err);
+  out->parents = static_cast<int64_t*>(host_copy(t.parents, N * 8, c.stream, &err));
+  out->token_parents = static_cast<int64_t*>(host_copy(t.token_parents, T * 8, c.stream, &err));
+  HIP_TRY(hipStreamSynchronize(c.stream));
+  if (err) return err;
+  return ALPACC_VALID;
+}
+
+}  // extern "C"

@@ -1,0 +1,201 @@
+// alpacc-hip: the generated-program interface of the reference backends
+// (backends/c/cli.c:1-189, docs/wire-protocols.md) over the HIP library.
+//
+//   alpacc-hip -g GRAMMAR [--lexer|--parser] [-i FILE] [-o FILE]
+//              [--timeit] [--server] [--layout] [--device N]
+//
+// Invoked through a link named GRAMMAR or GRAMMAR_MODE (MODE = lexer, parser,
+// both) it needs no -g / mode flag, so it can stand in for a generated
+// program: batch mode reads one counted batch, --server loops over counted
+// batches until EOF and flushes after every response record.  Framing errors
+// go to stderr with exit status 1, as in the reference.
+#include <chrono>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "alpacc_hip.h"
+
+namespace {
+
+bool get_u64(FILE* f, uint64_t* v) { return fread(v, sizeof *v, 1, f) == 1; }
+void put_u64(FILE* f, uint64_t v) { fwrite(&v, sizeof v, 1, f); }
+
+struct Prog {
+  alpacc_hip_ctx* ctx = nullptr;
+  int mode = ALPACC_MODE_BOTH;
+};
+
+int one_frame(Prog& p, FILE* in, FILE* out) {
+  uint64_t flen;
+  if (!get_u64(in, &flen)) {
+    fprintf(stderr, "error: truncated input\n");
+    return -1;
+  }
+  if (flen == UINT64_MAX || flen < 8) {
+    fprintf(stderr, "error: bad frame length\n");
+    return -1;
+  }
+  std::vector<uint8_t> frame(flen);
+  if (fread(frame.data(), 1, flen, in) != flen) {
+    fprintf(stderr, "error: truncated frame\n");
+    return -1;
+  }
+  uint64_t n;
+  memcpy(&n, frame.data(), 8);
+  if (flen != 8 + alpacc_hip_payload_bytes(p.ctx, p.mode, n)) {
+    fprintf(stderr, "error: frame length mismatch\n");
+    return -1;
+  }
+  uint8_t* rec = nullptr;
+  uint64_t len = 0;
+  int r = alpacc_hip_run_case(p.ctx, p.mode, n, frame.data() + 8, &rec, &len);
+  if (r < 0) {
+    fprintf(stderr, "error: %s\n", alpacc_hip_last_error());
+    return -1;
+  }
+  fwrite(rec, 1, len, out);
+  alpacc_hip_free(rec);
+  return 0;
+}
+
+// 0 = batch done, 1 = clean EOF before a count (server), -1 = error
+int one_batch(Prog& p, FILE* in, FILE* out, bool eof_ok, bool flush_each) {
+  uint64_t count;
+  if (!get_u64(in, &count)) {
+    if (eof_ok && feof(in)) return 1;
+    fprintf(stderr, "error: truncated input\n");
+    return -1;
+  }
+  if (count == UINT64_MAX) {
+    fprintf(stderr, "error: truncated input\n");
+    return -1;
+  }
+  put_u64(out, count);
+  for (uint64_t t = 0; t < count; ++t) {
+    if (one_frame(p, in, out) != 0) return -1;
+    if (flush_each) fflush(out);
+  }
+  return 0;
+}
+
+int parse_mode(const std::string& s) {
+  if (s == "lexer") return ALPACC_MODE_LEXER;
+  if (s == "parser") return ALPACC_MODE_PARSER;
+  if (s == "both") return ALPACC_MODE_BOTH;
+  return -1;
+}
+
+void usage(const char* prog) {
+  fprintf(stderr,
+          "Usage: %s -g GRAMMAR [--lexer|--parser] [options]\n"
+          "  -i FILE    input file  (default: stdin)\n"
+          "  -o FILE    output file (default: stdout)\n"
+          "  --timeit   print wall-clock elapsed time to stderr\n"
+          "  --server   counted-batch loop until EOF (flush per response)\n"
+          "  --layout   print native type sizes and exit\n"
+          "  --device N GPU ordinal (default 0)\n"
+          "  -h/--help  show this message\n",
+          prog);
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  Prog p;
+  std::string grammar;
+  {
+    std::string base = argv[0];
+    size_t sl = base.rfind('/');
+    if (sl != std::string::npos) base = base.substr(sl + 1);
+    if (base != "alpacc-hip") {
+      size_t us = base.rfind('_');
+      if (us != std::string::npos && parse_mode(base.substr(us + 1)) >= 0) {
+        p.mode = parse_mode(base.substr(us + 1));
+        grammar = base.substr(0, us);
+      } else {
+        grammar = base;
+      }
+    }
+  }
+  const char *inp = nullptr, *outp = nullptr;
+  bool timeit = false, server = false, layout = false;
+  int device = 0;
+  for (int i = 1; i < argc; ++i) {
+    std::string a = argv[i];
+    if (a == "-h" || a == "--help") {
+      usage(argv[0]);
+      return 0;
+    } else if (a == "--timeit") {
+      timeit = true;
+    } else if (a == "--server") {
+      server = true;
+    } else if (a == "--layout") {
+      layout = true;
+    } else if (a == "--lexer" || a == "-l") {
+      p.mode = ALPACC_MODE_LEXER;
+    } else if (a == "--parser" || a == "-p") {
+      p.mode = ALPACC_MODE_PARSER;
+    } else if (a == "-g" && i + 1 < argc) {
+      grammar = argv[++i];
+    } else if (a == "-i" && i + 1 < argc) {
+      inp = argv[++i];
+    } else if (a == "-o" && i + 1 < argc) {
+      outp = argv[++i];
+    } else if (a == "--device" && i + 1 < argc) {
+      device = atoi(argv[++i]);
+    } else {
+      fprintf(stderr, "error: unknown argument '%s'\n", argv[i]);
+      usage(argv[0]);
+      return 1;
+    }
+  }
+  if (grammar.empty()) {
+    usage(argv[0]);
+    return 1;
+  }
+  if (alpacc_hip_open(grammar.c_str(), device, &p.ctx) < 0) {
+    fprintf(stderr, "error: %s\n", alpacc_hip_last_error());
+    return 1;
+  }
+  if (layout) {
+    char buf[512];
+    if (alpacc_hip_layout(p.ctx, p.mode, buf, sizeof buf) < 0) return 1;
+    fputs(buf, stdout);
+    alpacc_hip_close(p.ctx);
+    return 0;
+  }
+  FILE* in = inp ? fopen(inp, "rb") : stdin;
+  FILE* out = outp ? fopen(outp, "wb") : stdout;
+  if (!in || !out) {
+    perror(inp && !in ? inp : outp);
+    return 1;
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  int rc = 0;
+  if (server) {
+    for (;;) {
+      int r = one_batch(p, in, out, true, true);
+      if (r == 1) break;
+      if (r < 0) {
+        rc = 1;
+        break;
+      }
+      fflush(out);
+    }
+  } else {
+    rc = one_batch(p, in, out, false, false) == 0 ? 0 : 1;
+    fflush(out);
+  }
+  if (timeit) {
+    double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    fprintf(stderr, "Time: %.2fms\n", ms);
+  }
+  if (in != stdin) fclose(in);
+  if (out != stdout) fclose(out);
+  alpacc_hip_close(p.ctx);
+  return rc;
+}

@@ -1,0 +1,44 @@
+// Device-table descriptor of one grammar, filled by the sources that
+// alpacc_amd/gen/emit_hip.py generates (one per grammar).
+#pragma once
+#include <cstdint>
+
+namespace alpacc {
+
+struct GrammarDesc {
+  const char* name;
+  int q, k;
+  // ---- lexer ----
+  int nstate;                    // DFA states including the dead state 0; 1 is initial
+  int nendo;                     // endomorphisms; 0 = identity, 1 = dead
+  int nclass;                    // byte classes
+  const uint8_t* byte_class;     // [256]
+  const uint16_t* step;          // [nendo * nclass]  e then class c
+  const uint16_t* compose;       // [nendo * nendo]   a then b
+  const uint8_t* endo_init;      // [nendo]           state the initial state maps to
+  const uint16_t* delta_lex;     // [nstate * 256]    next | produce<<8 | token(s)<<9
+  const uint16_t* delta_both;    // same, combined-mode terminal ids
+  int ignore_lex, ignore_both;   // -1 when the grammar has no ignore terminal
+  int dead_lex, dead_both;       // terminal id of "no token"
+  int terminal_bytes_lex, terminal_bytes_both, index_bytes, length_bytes;
+  // ---- LLP parser ----
+  int empty_t, start_t, end_t;
+  int window_base;               // terminal symbols (window digits are base-window_base)
+  int nwindow;
+  const uint16_t* window;        // [nwindow] pair id or 0xffff
+  int npairs;
+  const int32_t* spans;          // [npairs * 4] stack start, stack len, prod start, prod len
+  int nstacks;
+  const uint16_t* stacks;        // bracket: symbol | left << 15
+  int nprods_flat;
+  const uint8_t* productions;    // production strings
+  int nproductions;
+  int production_bytes;
+  const uint8_t* arity;          // [nproductions]
+  const uint8_t* prod_is_terminal;  // [nproductions]
+  int max_brackets, max_prods;   // per window
+};
+
+const GrammarDesc* find_grammar(const char* name);
+
+}  // namespace alpacc

@@ -1,0 +1,56 @@
+// Parameters and entry points of the tile lexer (lexer.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace alpacc {
+
+constexpr int LEX_WG = 256;   // lanes per workgroup (4 waves)
+constexpr int LEX_IPT = 32;   // input bytes per lane; a tile is 8 KiB
+
+constexpr uint32_t LEX_REJECT_FINAL = 1u;     // last state not accepting
+constexpr uint32_t LEX_REJECT_OVERFLOW = 2u;  // a kept token is longer than length_t
+
+struct LexParams {
+  const uint8_t* in;
+  uint64_t n;
+  int aligned16;
+  // tables
+  const uint8_t* byte_class;
+  const uint16_t* step;
+  const uint16_t* compose;
+  const uint8_t* endo_init;
+  const uint16_t* delta;
+  int nstate, nendo, nclass;
+  int ignore, dead;
+  uint64_t max_len;
+  int length_bytes;
+  // outputs (capacity n tokens)
+  uint8_t* terminals;
+  int64_t* starts;
+  void* lengths;
+  uint64_t* count;    // [1] number of kept tokens
+  uint32_t* flags;    // [1] LEX_REJECT_* bits
+  // lookback state (zeroed before every launch)
+  uint32_t* endo_status;   // [ntiles]
+  uint64_t* count_status;  // [ntiles]
+  uint32_t* ticket;        // [1]
+  uint32_t ntiles;
+};
+
+__device__ __forceinline__ void store_length(const LexParams& p, uint64_t slot, uint64_t len) {
+  switch (p.length_bytes) {
+    case 1: static_cast<uint8_t*>(p.lengths)[slot] = (uint8_t)len; break;
+    case 2: static_cast<uint16_t*>(p.lengths)[slot] = (uint16_t)len; break;
+    case 4: static_cast<uint32_t*>(p.lengths)[slot] = (uint32_t)len; break;
+    default: static_cast<uint64_t*>(p.lengths)[slot] = len; break;
+  }
+}
+
+size_t lex_lds_bytes(const LexParams& p);
+uint32_t lex_tile_bytes();
+hipError_t lex_grid_size(const LexParams& p, int* grid);
+hipError_t lex_launch(const LexParams& p, int grid, hipStream_t st);
+
+}  // namespace alpacc

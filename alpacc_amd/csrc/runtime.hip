@@ -1,0 +1,216 @@
+// Host runtime of the HIP backend: grammar contexts with device-resident
+// tables, grow-only device workspaces, and the C ABI of include/alpacc_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/alpacc_hip.h"
+#include "grammar.hpp"
+#include "lexer.hpp"
+#include "runtime.hpp"
+
+namespace alpacc {
+
+static thread_local std::string g_error;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_error = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) return fail(ALPACC_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+int DevBuf::reserve(size_t want) {
+  if (want <= cap) return 0;
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  cap = 0;
+  size_t sz = want < 256 ? 256 : want;
+  sz = (sz + 255) & ~size_t(255);
+  hipError_t e = hipMalloc(&ptr, sz);
+  if (e != hipSuccess) {
+    ptr = nullptr;
+    return fail(ALPACC_ERR_NOMEM, "hipMalloc(%zu): %s", sz, hipGetErrorString(e));
+  }
+  cap = sz;
+  return 0;
+}
+
+void DevBuf::release() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  cap = 0;
+}
+
+template <typename T>
+static int upload(T** dst, const T* src, size_t count) {
+  size_t bytes = (count ? count : 1) * sizeof(T);
+  HIP_TRY(hipMalloc(reinterpret_cast<void**>(dst), bytes));
+  HIP_TRY(hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// Zero-fill of lookback state by a kernel (no memset nodes, so the launch
+// sequence stays safe to capture in a graph).
+__global__ void fill_zero_u32(uint32_t* p, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = 0;
+}
+
+int zero_words(void* p, size_t bytes, hipStream_t st) {
+  size_t n = (bytes + 3) / 4;
+  int blocks = (int)((n + 255) / 256);
+  if (blocks > 4096) blocks = 4096;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(fill_zero_u32, dim3(blocks), dim3(256), 0, st, static_cast<uint32_t*>(p), n);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+}  // namespace alpacc
+
+using namespace alpacc;
+
+static int check_mode(int mode) {
+  if (mode < ALPACC_MODE_LEXER || mode > ALPACC_MODE_BOTH) return fail(ALPACC_ERR_ARG, "bad mode %d", mode);
+  return 0;
+}
+
+extern "C" {
+
+const char* alpacc_hip_last_error(void) { return g_error.c_str(); }
+
+int alpacc_hip_open(const char* grammar, int device, alpacc_hip_ctx** out) {
+  if (!grammar || !out) return fail(ALPACC_ERR_ARG, "null argument");
+  *out = nullptr;
+  const GrammarDesc* g = find_grammar(grammar);
+  if (!g) return fail(ALPACC_ERR_ARG, "unknown grammar '%s'", grammar);
+  HIP_TRY(hipSetDevice(device));
+  auto* h = new alpacc_hip_ctx();
+  Context& c = h->c;
+  c.g = g;
+  c.device = device;
+  int r = 0;
+  hipError_t e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete h;
+    return fail(ALPACC_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+  }
+  if ((r = upload(&c.d_class, g->byte_class, 256)) || (r = upload(&c.d_step, g->step, (size_t)g->nendo * g->nclass)) ||
+      (r = upload(&c.d_compose, g->compose, (size_t)g->nendo * g->nendo)) ||
+      (r = upload(&c.d_endo_init, g->endo_init, (size_t)g->nendo)) ||
+      (r = upload(&c.d_delta_lex, g->delta_lex, (size_t)g->nstate * 256)) ||
+      (r = upload(&c.d_delta_both, g->delta_both, (size_t)g->nstate * 256)) ||
+      (r = upload(&c.d_window, g->window, (size_t)g->nwindow)) ||
+      (r = upload(&c.d_spans, g->spans, (size_t)g->npairs * 4)) ||
+      (r = upload(&c.d_stacks, g->stacks, (size_t)g->nstacks)) ||
+      (r = upload(&c.d_productions, g->productions, (size_t)g->nprods_flat)) ||
+      (r = upload(&c.d_arity, g->arity, (size_t)g->nproductions)) ||
+      (r = upload(&c.d_isterm, g->prod_is_terminal, (size_t)g->nproductions)) || (r = c.small.reserve(4096))) {
+    alpacc_hip_close(h);
+    return r;
+  }
+  e = hipHostMalloc(reinterpret_cast<void**>(&c.host_small), 4096, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    c.host_small = nullptr;
+    alpacc_hip_close(h);
+    return fail(ALPACC_ERR_NOMEM, "hipHostMalloc: %s", hipGetErrorString(e));
+  }
+  *out = h;
+  return 0;
+}
+
+void alpacc_hip_close(alpacc_hip_ctx* h) {
+  if (!h) return;
+  Context& c = h->c;
+  (void)hipSetDevice(c.device);
+  void* ptrs[] = {c.d_class, c.d_step, c.d_compose, c.d_endo_init, c.d_delta_lex, c.d_delta_both, c.d_window,
+                  c.d_spans, c.d_stacks, c.d_productions, c.d_arity, c.d_isterm};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  for (DevBuf* b : c.all_buffers()) b->release();
+  if (c.host_small) (void)hipHostFree(c.host_small);
+  for (hipEvent_t ev : c.ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (c.stream) (void)hipStreamDestroy(c.stream);
+  delete h;
+}
+
+int alpacc_hip_layout(const alpacc_hip_ctx* h, int mode, char* buf, size_t cap) {
+  if (!h || !buf) return fail(ALPACC_ERR_ARG, "null argument");
+  if (int r = check_mode(mode)) return r;
+  const GrammarDesc* g = h->c.g;
+  std::string s;
+  char line[64];
+  int tb = mode == ALPACC_MODE_LEXER ? g->terminal_bytes_lex : g->terminal_bytes_both;
+  snprintf(line, sizeof line, "terminal_t=%d\n", tb);
+  s += line;
+  if (mode != ALPACC_MODE_LEXER) {
+    snprintf(line, sizeof line, "production_t=%d\n", g->production_bytes);
+    s += line;
+  }
+  snprintf(line, sizeof line, "index_t=%d\n", g->index_bytes);
+  s += line;
+  if (mode != ALPACC_MODE_PARSER) {
+    snprintf(line, sizeof line, "length_t=%d\n", g->length_bytes);
+    s += line;
+  }
+  snprintf(line, sizeof line, "workgroup_size=%d\nitems_per_thread=%d\n", LEX_WG, LEX_IPT);
+  s += line;
+  if (s.size() + 1 > cap) return fail(ALPACC_ERR_ARG, "layout buffer too small");
+  memcpy(buf, s.c_str(), s.size() + 1);
+  return (int)s.size();
+}
+
+uint64_t alpacc_hip_payload_bytes(const alpacc_hip_ctx* h, int mode, uint64_t n) {
+  if (!h) return 0;
+  return mode == ALPACC_MODE_PARSER ? n * (uint64_t)h->c.g->terminal_bytes_both : n;
+}
+
+void alpacc_hip_free(void* p) { free(p); }
+
+int alpacc_hip_set_timing(alpacc_hip_ctx* h, int on) {
+  if (!h) return fail(ALPACC_ERR_ARG, "null argument");
+  Context& c = h->c;
+  HIP_TRY(hipSetDevice(c.device));
+  for (hipEvent_t& ev : c.ev)
+    if (!ev) HIP_TRY(hipEventCreate(&ev));
+  c.timing = on != 0;
+  c.lex_ms = c.parse_ms = 0;
+  c.lex_launches = c.parse_launches = 0;
+  return 0;
+}
+
+int alpacc_hip_timing(const alpacc_hip_ctx* h, const char* stage, double* total_ms, uint64_t* runs) {
+  if (!h || !stage || !total_ms || !runs) return fail(ALPACC_ERR_ARG, "null argument");
+  const Context& c = h->c;
+  if (!strcmp(stage, "lex")) {
+    *total_ms = c.lex_ms;
+    *runs = c.lex_launches;
+  } else if (!strcmp(stage, "parse")) {
+    *total_ms = c.parse_ms;
+    *runs = c.parse_launches;
+  } else {
+    return fail(ALPACC_ERR_ARG, "unknown stage '%s'", stage);
+  }
+  return 0;
+}
+
+const char* const* alpacc_hip_grammars(void) { return grammar_names(); }
+
+}  // extern "C"

@@ -1,0 +1,146 @@
+"""Emit the device tables of one grammar as a C++ source for the HIP library.
+
+The GPU path does not use the oracle's packed-endomorphism format.  It needs:
+
+* ``BYTE_CLASS[256]`` -- bytes with equal endomorphisms share a class;
+* ``STEP[e * NCLASS + c]`` -- endomorphism e followed by class c (u16), the
+  per-byte fold of the first lexer pass (staged in LDS);
+* ``COMPOSE[a * NENDO + b]`` -- "a then b" for the scans (u16, L2-resident);
+* ``ENDO_INIT[e]`` -- the DFA state e maps the initial state to;
+* ``DELTA_*[s * 256 + b]`` -- the token DFA itself, u16 packed as
+  ``next (8 bits) | produce << 8 | token(s) << 9`` for the two terminal
+  numberings (lexer-only, combined), staged in LDS for the second pass;
+* the LLP pair table as a dense window index (``WINDOW[key] -> pair id``,
+  0xffff = no pair) over the padded q+k window, pair spans into the bracket
+  and production strings (brackets as u16: symbol | left << 15), arities and
+  the terminal-production flags.
+"""
+
+from __future__ import annotations
+
+from typing import List
+
+from .tables import Grammar
+
+MAX_WINDOW_ENTRIES = 1 << 24
+
+
+class EmitError(ValueError):
+    pass
+
+
+def _arr(ctype: str, name: str, vals: List[int], per_line: int = 24) -> str:
+    vals = [str(int(v)) for v in vals] or ["0"]
+    body = ",\n".join("  " + ", ".join(vals[i:i + per_line]) for i in range(0, len(vals), per_line))
+    return f"static const {ctype} {name}[{len(vals)}] = {{\n{body}\n}};\n"
+
+
+def device_tables(g: Grammar) -> dict:
+    """Plain-int device tables (shared by the emitter and the tests)."""
+    lb, lo = g.lex_both, g.lex_only
+    endos = lb.endos
+    dfa = lb.dfa
+    nstate = dfa.n + 1
+    if nstate > 256:
+        raise EmitError(f"{g.name}: {nstate} DFA states exceed the u8 state field")
+    classes: List[int] = []
+    class_of_endo = {}
+    for b in range(256):
+        e = endos.byte_endo[b]
+        if e not in class_of_endo:
+            class_of_endo[e] = len(class_of_endo)
+        classes.append(class_of_endo[e])
+    class_endo = sorted(class_of_endo, key=lambda e: class_of_endo[e])
+    ne, nc = endos.size, len(class_endo)
+    if ne >= 0xFFFF:
+        raise EmitError(f"{g.name}: {ne} endomorphisms exceed u16")
+    step = [endos.compose[e][ce] for e in range(ne) for ce in class_endo]
+    compose = [endos.compose[a][b] for a in range(ne) for b in range(ne)]
+    endo_init = [endos.funcs[e][1] for e in range(ne)]
+
+    def delta(lt) -> List[int]:
+        out = []
+        for s in range(nstate):
+            tok = dfa.token[s] if s else None
+            t = lt.term_id[tok] if tok is not None else lt.dead_id
+            if t > 127:
+                raise EmitError(f"{g.name}: terminal id {t} exceeds the 7-bit token field")
+            for b in range(256):
+                nxt = dfa.delta[s][b] if s else 0
+                p = 1 if (s and dfa.produce[s][b]) else 0
+                out.append(nxt | (p << 8) | (t << 9))
+        return out
+
+    p = g.parser
+    if p.production_bytes != 1:
+        raise EmitError(f"{g.name}: {p.n_productions} productions exceed the u8 production field")
+    base = p.end_t + 1
+    w = p.q + p.k
+    nwin = base ** w
+    if nwin > MAX_WINDOW_ENTRIES:
+        raise EmitError(f"{g.name}: window index of {nwin} entries is too large")
+    window = [0xFFFF] * nwin
+    for i, key in enumerate(p.keys):
+        code = 0
+        for t in key:
+            code = code * base + t
+        window[code] = i
+    top = 1 << (8 * p.bracket_bytes - 1)
+    stacks16 = [(v & (top - 1)) | (0x8000 if v & top else 0) for v in p.stacks]
+    if p.n_symbols >= 0x8000:
+        raise EmitError(f"{g.name}: {p.n_symbols} grammar symbols exceed the 15-bit bracket field")
+    spans = []
+    for (s0, s1, p0, p1) in p.spans:
+        spans += [s0, s1 - s0, p0, p1 - p0]
+    return dict(
+        nstate=nstate, nendo=ne, nclass=nc, byte_class=classes, step=step, compose=compose,
+        endo_init=endo_init, delta_lex=delta(lo), delta_both=delta(lb),
+        ignore_lex=-1 if lo.ignore_id is None else lo.ignore_id,
+        ignore_both=-1 if lb.ignore_id is None else lb.ignore_id,
+        dead_lex=lo.dead_id, dead_both=lb.dead_id,
+        window=window, base=base, spans=spans, stacks=stacks16, productions=p.productions,
+        arity=p.arity, prod_is_term=[1 if t is not None else 0 for t in p.prod_terminal],
+    )
+
+
+def emit(g: Grammar) -> str:
+    d = device_tables(g)
+    p = g.parser
+    ns = f"grammar_{g.name}"
+    src = [
+        f"// generated by alpacc_amd.gen.emit_hip for grammar '{g.name}'; do not edit",
+        '#include "../grammar.hpp"',
+        "namespace alpacc {",
+        f"namespace {ns} {{",
+        _arr("uint8_t", "BYTE_CLASS", d["byte_class"]),
+        _arr("uint16_t", "STEP", d["step"]),
+        _arr("uint16_t", "COMPOSE", d["compose"]),
+        _arr("uint8_t", "ENDO_INIT", d["endo_init"]),
+        _arr("uint16_t", "DELTA_LEX", d["delta_lex"]),
+        _arr("uint16_t", "DELTA_BOTH", d["delta_both"]),
+        _arr("uint16_t", "WINDOW", d["window"]),
+        _arr("int32_t", "SPANS", d["spans"]),
+        _arr("uint16_t", "STACKS", d["stacks"]),
+        _arr("uint8_t", "PRODUCTIONS", d["productions"]),
+        _arr("uint8_t", "ARITY", d["arity"]),
+        _arr("uint8_t", "PROD_IS_TERMINAL", d["prod_is_term"]),
+        "}  // namespace",
+        "",
+        f"extern const GrammarDesc GRAMMAR_{g.name.upper()} = {{",
+        f'  "{g.name}", {g.q}, {g.k},',
+        f"  {d['nstate']}, {d['nendo']}, {d['nclass']},",
+        f"  {ns}::BYTE_CLASS, {ns}::STEP, {ns}::COMPOSE, {ns}::ENDO_INIT,",
+        f"  {ns}::DELTA_LEX, {ns}::DELTA_BOTH,",
+        f"  {d['ignore_lex']}, {d['ignore_both']}, {d['dead_lex']}, {d['dead_both']},",
+        f"  {g.terminal_bytes_lex}, {g.terminal_bytes_both}, {g.index_bytes}, {g.length_bytes},",
+        f"  {p.empty_t}, {p.start_t}, {p.end_t}, {d['base']},",
+        f"  {len(d['window'])}, {ns}::WINDOW, {len(p.keys)}, {ns}::SPANS,",
+        f"  {max(1, len(p.stacks))}, {ns}::STACKS,",
+        f"  {max(1, len(p.productions))}, {ns}::PRODUCTIONS,",
+        f"  {p.n_productions}, {p.production_bytes}, {ns}::ARITY, {ns}::PROD_IS_TERMINAL,",
+        f"  {max(1, p.max_brackets)}, {max(1, p.max_prods)},",
+        "};",
+        "}  // namespace alpacc",
+        "",
+    ]
+    return "\n".join(src)

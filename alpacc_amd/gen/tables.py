@@ -118,6 +118,8 @@ class Grammar:
     parser: ParserTables
     terminal_bytes_both: int
     terminal_bytes_lex: int
+    productions: List[Tuple[int, Tuple[int, ...]]] = dataclasses.field(default_factory=list)
+    llp_grammar: Optional[llp.LLPGrammar] = None
 
     @property
     def index_bytes(self) -> int:
@@ -203,10 +205,16 @@ def _lexer_tables(dfa, endos, term_id, ignore_id, dead_id, n_terminals) -> Lexer
                        state_bytes, to_state, compose, accept, packed[lexgen.IDENTITY_E])
 
 
-def analyse(name: str, text: str, index32: bool = False) -> Grammar:
+def analyse(name: str, text: str, index32: bool = False, q: Optional[int] = None,
+            k: Optional[int] = None) -> Grammar:
+    """Analyse grammar text; ``q`` / ``k`` override the grammar's params."""
     cfg = G.parse_cfg(text)
     if not cfg.nt_rules:
         raise G.GrammarError("CFG has no production rules.")
+    if q is not None:
+        cfg.params.lookback = q
+    if k is not None:
+        cfg.params.lookahead = k
     q, k = cfg.params.lookback, cfg.params.lookahead
     terminals = _grammar_terminals(cfg)
     ignore = ("T", "ignore")
@@ -345,6 +353,7 @@ def analyse(name: str, text: str, index32: bool = False) -> Grammar:
         name=name, cfg=cfg, terminals=terminals, q=q, k=k, length_bits=cfg.params.length,
         index32=index32, lex_both=lex_both, lex_only=lex_only, parser=parser,
         terminal_bytes_both=uint_bytes(end_t), terminal_bytes_lex=uint_bytes(dead_lex),
+        productions=prods, llp_grammar=g,
     )
 
 

@@ -1,0 +1,224 @@
+"""Python mirror of the generated program's entry points over the HIP library.
+
+``Backend(grammar)`` keeps one grammar resident on a GPU and exposes the
+reference's operations (backends/c/lexer.c ``lex_string``, backends/c/parser.c
+``parse_test`` / ``compute_parents`` / ``compact_tree``, and the generated
+``run_case`` / ``print_layout``) with the reference's return convention:
+``True`` = accepted, ``False`` = rejected, an exception for errors.
+
+The library is alpacc_amd/lib/libalpacc_hip.so, built in-tree by
+``python -m alpacc_amd.build``; there is no fallback, so a missing library is
+an ImportError.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import wire
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libalpacc_hip.so")
+
+LEXER, PARSER, BOTH = 0, 1, 2
+MODES = {"lexer": LEXER, "parser": PARSER, "both": BOTH}
+
+
+class Lexemes(C.Structure):
+    _fields_ = [("count", C.c_uint64), ("terminals", C.c_void_p), ("starts", C.c_void_p), ("lengths", C.c_void_p)]
+
+
+class Tree(C.Structure):
+    _fields_ = [("tokens", Lexemes), ("num_nodes", C.c_uint64), ("productions", C.c_void_p),
+                ("parents", C.c_void_p), ("token_parents", C.c_void_p)]
+
+
+class HipError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: run `python -m alpacc_amd.build` first")
+    lib = C.CDLL(path)
+    vp, u64, i32 = C.c_void_p, C.c_uint64, C.c_int
+    sig = {
+        "alpacc_hip_open": (i32, [C.c_char_p, i32, C.POINTER(vp)]),
+        "alpacc_hip_close": (None, [vp]),
+        "alpacc_hip_last_error": (C.c_char_p, []),
+        "alpacc_hip_grammars": (C.POINTER(C.c_char_p), []),
+        "alpacc_hip_layout": (i32, [vp, i32, C.c_char_p, C.c_size_t]),
+        "alpacc_hip_payload_bytes": (u64, [vp, i32, u64]),
+        "alpacc_hip_lex_string": (i32, [vp, i32, vp, u64, C.POINTER(Lexemes)]),
+        "alpacc_hip_parse_test": (i32, [vp, vp, u64, C.POINTER(vp), C.POINTER(u64)]),
+        "alpacc_hip_lex_parse": (i32, [vp, vp, u64, C.POINTER(Tree)]),
+        "alpacc_hip_run_case": (i32, [vp, i32, u64, vp, C.POINTER(vp), C.POINTER(u64)]),
+        "alpacc_hip_free": (None, [vp]),
+        "alpacc_hip_lex_device": (i32, [vp, i32, vp, u64, C.POINTER(Lexemes), vp]),
+        "alpacc_hip_parse_device": (i32, [vp, vp, u64, C.POINTER(vp), C.POINTER(u64), vp]),
+        "alpacc_hip_lex_parse_device": (i32, [vp, vp, u64, C.POINTER(Tree), vp]),
+        "alpacc_hip_set_timing": (i32, [vp, i32]),
+        "alpacc_hip_timing": (i32, [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(u64)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    lib = load_library()
+    return [n for n in dir(lib) if n.startswith("alpacc_hip_")]
+
+
+def grammars():
+    lib = load_library()
+    arr = lib.alpacc_hip_grammars()
+    out = []
+    i = 0
+    while arr[i]:
+        out.append(arr[i].decode())
+        i += 1
+    return out
+
+
+def _check(rc: int) -> bool:
+    if rc < 0:
+        raise HipError(f"alpacc_hip error {rc}: {load_library().alpacc_hip_last_error().decode()}")
+    return rc == 1
+
+
+def _take(ptr, nbytes: int, dtype) -> np.ndarray:
+    if nbytes == 0:
+        return np.zeros(0, dtype=dtype)
+    buf = (C.c_uint8 * nbytes).from_address(ptr)
+    arr = np.frombuffer(bytes(buf), dtype=dtype).copy()
+    load_library().alpacc_hip_free(ptr)
+    return arr
+
+
+class Backend:
+    """One grammar resident on one device."""
+
+    def __init__(self, grammar: str, device: int = 0):
+        self.lib = load_library()
+        self.grammar = grammar
+        h = C.c_void_p()
+        _check(self.lib.alpacc_hip_open(grammar.encode(), device, C.byref(h)))
+        self.h = h
+        self.layouts = {m: self._layout(m) for m in MODES}
+
+    def close(self):
+        if self.h:
+            self.lib.alpacc_hip_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- layout ----------------------------------------------------------------
+    def layout_text(self, mode: str) -> str:
+        buf = C.create_string_buffer(512)
+        _check(self.lib.alpacc_hip_layout(self.h, MODES[mode], buf, 512) >= 0 and 1 or -1)
+        return buf.value.decode()
+
+    def _layout(self, mode: str) -> wire.Layout:
+        kv = dict(l.split("=") for l in self.layout_text(mode).strip().splitlines())
+        return wire.Layout(mode, int(kv["terminal_t"]), int(kv["index_t"]), int(kv.get("length_t", 0)),
+                           int(kv.get("production_t", 0)))
+
+    # -- reference entry points --------------------------------------------------
+    def run_case(self, mode: str, payload: bytes, n: Optional[int] = None) -> bytes:
+        """run_case: one request -> one wire response record."""
+        lay = self.layouts[mode]
+        if n is None:
+            n = len(payload) // (lay.terminal if mode == "parser" else 1)
+        rec = C.c_void_p()
+        ln = C.c_uint64()
+        src = C.c_char_p(payload)
+        _check(self.lib.alpacc_hip_run_case(self.h, MODES[mode], n, src, C.byref(rec), C.byref(ln)))
+        return _take(rec.value, ln.value, np.uint8).tobytes()
+
+    def lex_string(self, data: bytes, mode: str = "lexer"):
+        """lex_string: (accepted, terminals, starts, lengths)."""
+        out = Lexemes()
+        ok = _check(self.lib.alpacc_hip_lex_string(self.h, MODES[mode], C.c_char_p(data), len(data), C.byref(out)))
+        if not ok:
+            return False, None, None, None
+        lay = self.layouts[mode]
+        n = out.count
+        return (True, _take(out.terminals, n * lay.terminal, f"<u{lay.terminal}"),
+                _take(out.starts, n * 8, "<i8"), _take(out.lengths, n * lay.length, f"<u{lay.length}"))
+
+    def parse_test(self, tokens: np.ndarray):
+        """parse_test: (accepted, productions)."""
+        tok = np.ascontiguousarray(tokens, dtype=np.uint8)
+        prods = C.c_void_p()
+        npr = C.c_uint64()
+        ok = _check(self.lib.alpacc_hip_parse_test(self.h, tok.ctypes.data, len(tok), C.byref(prods), C.byref(npr)))
+        if not ok:
+            return False, None
+        return True, _take(prods.value, npr.value, np.uint8)
+
+    def lex_parse(self, data: bytes):
+        """lex_string + parse_test + compute_parents + compact_tree."""
+        out = Tree()
+        ok = _check(self.lib.alpacc_hip_lex_parse(self.h, C.c_char_p(data), len(data), C.byref(out)))
+        if not ok:
+            return None
+        lay = self.layouts["both"]
+        T, N = out.tokens.count, out.num_nodes
+        return dict(
+            terminals=_take(out.tokens.terminals, T * lay.terminal, np.uint8),
+            starts=_take(out.tokens.starts, T * 8, "<i8"),
+            lengths=_take(out.tokens.lengths, T * lay.length, f"<u{lay.length}"),
+            productions=_take(out.productions, N * lay.production, np.uint8),
+            parents=_take(out.parents, N * 8, "<i8"),
+            token_parents=_take(out.token_parents, T * 8, "<i8"),
+        )
+
+    # -- stage timing (HIP events on the launch stream) ------------------------------
+    def set_timing(self, on: bool = True):
+        _check(self.lib.alpacc_hip_set_timing(self.h, int(on)) >= 0 and 1 or -1)
+
+    def timing(self, stage: str):
+        """(total milliseconds, runs) of a stage since set_timing."""
+        ms = C.c_double()
+        runs = C.c_uint64()
+        if self.lib.alpacc_hip_timing(self.h, stage.encode(), C.byref(ms), C.byref(runs)) < 0:
+            _check(-1)
+        return ms.value, runs.value
+
+    # -- device-resident forms ---------------------------------------------------
+    def lex_device(self, d_ptr: int, n: int, mode: str = "lexer", stream: int = 0):
+        out = Lexemes()
+        ok = _check(self.lib.alpacc_hip_lex_device(self.h, MODES[mode], C.c_void_p(d_ptr), n, C.byref(out),
+                                                   C.c_void_p(stream or None)))
+        return ok, out
+
+    def lex_parse_device(self, d_ptr: int, n: int, stream: int = 0):
+        out = Tree()
+        ok = _check(self.lib.alpacc_hip_lex_parse_device(self.h, C.c_void_p(d_ptr), n, C.byref(out),
+                                                         C.c_void_p(stream or None)))
+        return ok, out

@@ -1,0 +1,130 @@
+/*
+ * alpacc_hip.h -- C ABI of the MI355X (gfx950) backend for alpacc lexers and
+ * LLP parsers.
+ *
+ * The reference has no shared library: its boundary is a set of file-static
+ * functions in the generated C program plus a byte-level wire protocol.  Each
+ * entry point below names the reference function it replaces.  All pointers
+ * are plain host or device pointers; no HIP or torch types appear here (a
+ * stream is passed as an opaque pointer, NULL = the context's own stream).
+ *
+ * Return codes: 1 = input accepted, 0 = input rejected (the wire protocol's
+ * 0x00 record), negative = error (alpacc_hip_last_error() has the message).
+ *
+ * Build modes (lexer-only, parser-only, combined) are a runtime argument; the
+ * terminal numbering of each mode is that of the reference's build of the
+ * same mode (lexer-only numbers terminals in sorted order).
+ */
+#ifndef ALPACC_HIP_H
+#define ALPACC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct alpacc_hip_ctx alpacc_hip_ctx;
+
+enum alpacc_hip_mode { ALPACC_MODE_LEXER = 0, ALPACC_MODE_PARSER = 1, ALPACC_MODE_BOTH = 2 };
+
+enum alpacc_hip_status {
+  ALPACC_VALID = 1,
+  ALPACC_REJECT = 0,
+  ALPACC_ERR_ARG = -1,     /* bad argument or unknown grammar */
+  ALPACC_ERR_HIP = -2,     /* HIP runtime failure */
+  ALPACC_ERR_NOMEM = -3,   /* device or host allocation failed */
+  ALPACC_ERR_LIMIT = -4    /* input beyond a documented size limit */
+};
+
+/* Tokens as structure-of-arrays: terminal_t, index_t (int64) and length_t
+ * arrays of `count` entries each.  Same content as the reference's lexeme_t
+ * array (backends/c/lexer.c:30-34). */
+typedef struct {
+  uint64_t count;
+  void *terminals;
+  int64_t *starts;
+  void *lengths;
+} alpacc_hip_lexemes;
+
+/* Combined-mode result: the tokens plus the compact productions-only tree of
+ * docs/wire-protocols.md:108-125 (preorder production ids, parent indices,
+ * and each token's parent node). */
+typedef struct {
+  alpacc_hip_lexemes tokens;
+  uint64_t num_nodes;
+  void *productions;       /* production_t[num_nodes] */
+  int64_t *parents;        /* index_t[num_nodes] */
+  int64_t *token_parents;  /* index_t[tokens.count] */
+} alpacc_hip_tree;
+
+/* ---- context ------------------------------------------------------------ */
+
+/* Make one grammar's tables resident on `device` (replaces the generated
+ * program's static const tables, src/Alpacc/Generator/C/Lexer.hs:16-58 and
+ * C/Parser.hs:21-95). */
+int alpacc_hip_open(const char *grammar, int device, alpacc_hip_ctx **out);
+void alpacc_hip_close(alpacc_hip_ctx *ctx);
+const char *alpacc_hip_last_error(void);
+
+/* Names of the compiled-in grammars, NULL-terminated. */
+const char *const *alpacc_hip_grammars(void);
+
+/* `--layout` (print_layout, src/Alpacc/Generator/C/Generator.hs:64-110):
+ * key=value lines into buf; returns the length or a negative error. */
+int alpacc_hip_layout(const alpacc_hip_ctx *ctx, int mode, char *buf, size_t cap);
+
+/* INPUT_BYTES(n) of the mode (C/Generator.hs:64,87,117). */
+uint64_t alpacc_hip_payload_bytes(const alpacc_hip_ctx *ctx, int mode, uint64_t n);
+
+/* ---- host-memory entry points ------------------------------------------ */
+
+/* lex_string (backends/c/lexer.c:39-97).  mode is LEXER or BOTH (terminal
+ * numbering).  Arrays are malloc'd; release with alpacc_hip_free. */
+int alpacc_hip_lex_string(alpacc_hip_ctx *ctx, int mode, const uint8_t *str, uint64_t n,
+                          alpacc_hip_lexemes *out);
+
+/* parse_test (backends/c/parser.c:49-98): n terminal ids (terminal_t) to the
+ * leftmost derivation (production_t[*num_prods], malloc'd). */
+int alpacc_hip_parse_test(alpacc_hip_ctx *ctx, const void *tokens, uint64_t n, void **prods,
+                          uint64_t *num_prods);
+
+/* Combined run: lex_string + parse_test + compute_parents + compact_tree
+ * (backends/c/parser.c:103-153) -> tree; arrays malloc'd. */
+int alpacc_hip_lex_parse(alpacc_hip_ctx *ctx, const uint8_t *str, uint64_t n, alpacc_hip_tree *out);
+
+/* run_case (src/Alpacc/Generator/C/Generator.hs:64-152): one request payload
+ * to one wire response record (u8 valid + fields), malloc'd. */
+int alpacc_hip_run_case(alpacc_hip_ctx *ctx, int mode, uint64_t n, const uint8_t *payload,
+                        uint8_t **record, uint64_t *record_len);
+
+void alpacc_hip_free(void *p);
+
+/* Stage timing with HIP events recorded on the launch stream (no reference
+ * counterpart; the reference's CUDA CLI times with events in --benchmark,
+ * backends/cuda/cli.cu:459-493).  Stages: "lex" = the lexer kernel, "parse" =
+ * the parser's kernels from the pair lookup to the tree compaction.
+ * set_timing resets the totals. */
+int alpacc_hip_set_timing(alpacc_hip_ctx *ctx, int on);
+int alpacc_hip_timing(const alpacc_hip_ctx *ctx, const char *stage, double *total_ms, uint64_t *runs);
+
+/* ---- device-memory entry points ---------------------------------------- */
+/* Input already in device memory; results stay in device buffers owned by the
+ * context, valid until the next call on it.  Only the count and verdict cross
+ * to the host.  A context serves one call at a time. */
+
+int alpacc_hip_lex_device(alpacc_hip_ctx *ctx, int mode, const uint8_t *d_str, uint64_t n,
+                          alpacc_hip_lexemes *d_out, void *stream);
+
+int alpacc_hip_parse_device(alpacc_hip_ctx *ctx, const void *d_tokens, uint64_t n, void **d_prods,
+                            uint64_t *num_prods, void *stream);
+
+int alpacc_hip_lex_parse_device(alpacc_hip_ctx *ctx, const uint8_t *d_str, uint64_t n,
+                                alpacc_hip_tree *d_out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ALPACC_HIP_H */
