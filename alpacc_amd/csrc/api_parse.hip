@@ -72,7 +72,7 @@ int parser_record(Context& c, uint64_t n, const uint8_t* payload, std::vector<ui
   rec.resize(9 + np);
   rec[0] = 1;
   memcpy(&rec[1], &np, 8);
-  if (np) HIP_TRY(hipMemcpyAsync(rec.data() + 9, c.ps_b.ptr, np, hipMemcpyDeviceToHost, st));
+  if (np) HIP_TRY(hipMemcpyAsync(rec.data() + 9, c.ps_d.ptr, np, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   return ALPACC_VALID;
 }
@@ -127,7 +127,7 @@ int alpacc_hip_parse_device(alpacc_hip_ctx* h, const void* d_tokens, uint64_t n,
   HIP_TRY(hipSetDevice(c.device));
   hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c.stream;
   int r = parse_on_device(c, static_cast<const uint8_t*>(d_tokens), n, false, st, num_prods);
-  *d_prods = r == ALPACC_VALID ? c.ps_b.ptr : nullptr;
+  *d_prods = r == ALPACC_VALID ? c.ps_d.ptr : nullptr;
   return r;
 }
 
@@ -144,7 +144,7 @@ int alpacc_hip_parse_test(alpacc_hip_ctx* h, const void* tokens, uint64_t n, voi
   r = parse_on_device(c, d_tok, n, false, c.stream, &np);
   if (r != ALPACC_VALID) return r;
   int err = 0;
-  void* hp = host_copy(c.ps_b.ptr, np, c.stream, &err);
+  void* hp = host_copy(c.ps_d.ptr, np, c.stream, &err);
   if (!hp) return err;
   HIP_TRY(hipStreamSynchronize(c.stream));
   *prods = hp;

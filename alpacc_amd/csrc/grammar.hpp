@@ -28,6 +28,7 @@ struct GrammarDesc {
   const uint16_t* window;        // [nwindow] pair id or 0xffff
   int npairs;
   const int32_t* spans;          // [npairs * 4] stack start, stack len, prod start, prod len
+  const uint8_t* profile;        // [npairs * 8] i8 ddepth, i8 dmin, i8 dslot, i8 smin, u8 terminals, pad
   int nstacks;
   const uint16_t* stacks;        // bracket: symbol | left << 15
   int nprods_flat;

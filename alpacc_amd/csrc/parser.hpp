@@ -8,10 +8,10 @@
 namespace alpacc {
 
 // Parse n terminal ids (u8, device) with the combined-mode numbering.
-// On success: returns 1 (valid) / 0 (rejected); the derivation (u8
-// productions) is in c.ps_b with *num_prods entries.  When `tree` is set it
-// also computes the parents and the compacted productions-only tree into
-// c.tr_* (num_nodes = *num_prods - n).
+// Returns 1 (valid) / 0 (rejected) / < 0 (error) and the derivation length in
+// *num_prods.  With `tree` the parents and the compact productions-only tree
+// land in c.tr_* (num_nodes = *num_prods - n); without it the flat
+// derivation (u8 productions) lands in c.ps_d.
 int parse_on_device(Context& c, const uint8_t* d_tok, uint64_t n, bool tree, hipStream_t st, uint64_t* num_prods);
 
 }  // namespace alpacc

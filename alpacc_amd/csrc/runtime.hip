@@ -118,6 +118,7 @@ int alpacc_hip_open(const char* grammar, int device, alpacc_hip_ctx** out) {
       (r = upload(&c.d_delta_both, g->delta_both, (size_t)g->nstate * 256)) ||
       (r = upload(&c.d_window, g->window, (size_t)g->nwindow)) ||
       (r = upload(&c.d_spans, g->spans, (size_t)g->npairs * 4)) ||
+      (r = upload(&c.d_profile, g->profile, (size_t)g->npairs * 8)) ||
       (r = upload(&c.d_stacks, g->stacks, (size_t)g->nstacks)) ||
       (r = upload(&c.d_productions, g->productions, (size_t)g->nprods_flat)) ||
       (r = upload(&c.d_arity, g->arity, (size_t)g->nproductions)) ||
@@ -140,7 +141,7 @@ void alpacc_hip_close(alpacc_hip_ctx* h) {
   Context& c = h->c;
   (void)hipSetDevice(c.device);
   void* ptrs[] = {c.d_class, c.d_step, c.d_compose, c.d_endo_init, c.d_delta_lex, c.d_delta_both, c.d_window,
-                  c.d_spans, c.d_stacks, c.d_productions, c.d_arity, c.d_isterm};
+                  c.d_spans, c.d_profile, c.d_stacks, c.d_productions, c.d_arity, c.d_isterm};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (DevBuf* b : c.all_buffers()) b->release();

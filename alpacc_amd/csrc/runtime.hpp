@@ -36,6 +36,7 @@ struct Context {
   uint16_t* d_delta_both = nullptr;
   uint16_t* d_window = nullptr;
   int32_t* d_spans = nullptr;
+  uint8_t* d_profile = nullptr;
   uint16_t* d_stacks = nullptr;
   uint8_t* d_productions = nullptr;
   uint8_t* d_arity = nullptr;

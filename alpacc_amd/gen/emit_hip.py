@@ -92,6 +92,26 @@ def device_tables(g: Grammar) -> dict:
     spans = []
     for (s0, s1, p0, p1) in p.spans:
         spans += [s0, s1 - s0, p0, p1 - p0]
+    # Per-pair depth profiles for the tiled parser: net bracket depth, the
+    # minimum of {0, depth after each bracket}; net open-child-slot depth
+    # (sum of arity - 1), the minimum of {0, depth after each slot's pop};
+    # the number of terminal productions.
+    profile = []
+    for (s0, s1, p0, p1) in p.spans:
+        d = dmin = 0
+        for v in stacks16[s0:s1]:
+            d += 1 if v & 0x8000 else -1
+            dmin = min(dmin, d)
+        e = emin = 0
+        nt = 0
+        for prod in p.productions[p0:p1]:
+            emin = min(emin, e - 1)
+            e += p.arity[prod] - 1
+            nt += 1 if p.prod_terminal[prod] is not None else 0
+        for v in (d, dmin, e, emin):
+            if not -128 <= v <= 127:
+                raise EmitError(f"{g.name}: pair depth profile {v} exceeds i8")
+        profile += [d & 0xFF, dmin & 0xFF, e & 0xFF, emin & 0xFF, nt, 0, 0, 0]
     return dict(
         nstate=nstate, nendo=ne, nclass=nc, byte_class=classes, step=step, compose=compose,
         endo_init=endo_init, delta_lex=delta(lo), delta_both=delta(lb),
@@ -100,6 +120,7 @@ def device_tables(g: Grammar) -> dict:
         dead_lex=lo.dead_id, dead_both=lb.dead_id,
         window=window, base=base, spans=spans, stacks=stacks16, productions=p.productions,
         arity=p.arity, prod_is_term=[1 if t is not None else 0 for t in p.prod_terminal],
+        profile=profile,
     )
 
 
@@ -120,6 +141,7 @@ def emit(g: Grammar) -> str:
         _arr("uint16_t", "DELTA_BOTH", d["delta_both"]),
         _arr("uint16_t", "WINDOW", d["window"]),
         _arr("int32_t", "SPANS", d["spans"]),
+        _arr("uint8_t", "PROFILE", d["profile"]),
         _arr("uint16_t", "STACKS", d["stacks"]),
         _arr("uint8_t", "PRODUCTIONS", d["productions"]),
         _arr("uint8_t", "ARITY", d["arity"]),
@@ -134,7 +156,7 @@ def emit(g: Grammar) -> str:
         f"  {d['ignore_lex']}, {d['ignore_both']}, {d['dead_lex']}, {d['dead_both']},",
         f"  {g.terminal_bytes_lex}, {g.terminal_bytes_both}, {g.index_bytes}, {g.length_bytes},",
         f"  {p.empty_t}, {p.start_t}, {p.end_t}, {d['base']},",
-        f"  {len(d['window'])}, {ns}::WINDOW, {len(p.keys)}, {ns}::SPANS,",
+        f"  {len(d['window'])}, {ns}::WINDOW, {len(p.keys)}, {ns}::SPANS, {ns}::PROFILE,",
         f"  {max(1, len(p.stacks))}, {ns}::STACKS,",
         f"  {max(1, len(p.productions))}, {ns}::PRODUCTIONS,",
         f"  {p.n_productions}, {p.production_bytes}, {ns}::ARITY, {ns}::PROD_IS_TERMINAL,",
