@@ -31,8 +31,11 @@ int lex_on_device(Context& c, int mode, const uint8_t* d_in, uint64_t n, hipStre
   const uint32_t tile = lex_tile_bytes();
   const uint32_t ntiles = (uint32_t)((n + tile - 1) / tile);
   int r;
+  // scan state: count_status u64[ntiles] | tile_endo u32[ntiles] | tile_prefix u16[ntiles] | lane_prefix
+  const size_t st_bytes = (size_t)ntiles * 8, te_off = st_bytes, tp_off = te_off + (size_t)ntiles * 4,
+               lp_off = (tp_off + (size_t)ntiles * 2 + 255) & ~size_t(255);
   if ((r = c.lx_term.reserve(n)) || (r = c.lx_start.reserve(n * 8)) || (r = c.lx_len.reserve(n * lb)) ||
-      (r = c.lx_status.reserve((size_t)ntiles * 12 + 16)))
+      (r = c.lx_status.reserve(lp_off + (size_t)ntiles * LEX_WG * 2 + 64)))
     return r;
   LexParams p{};
   p.in = d_in;
@@ -58,14 +61,16 @@ int lex_on_device(Context& c, int mode, const uint8_t* d_in, uint64_t n, hipStre
   p.count = small;                                        // [0]
   p.flags = reinterpret_cast<uint32_t*>(small + 1);       // [8..12)
   p.ticket = reinterpret_cast<uint32_t*>(small + 1) + 1;  // [12..16)
-  p.count_status = c.lx_status.as<uint64_t>();
-  p.endo_status = reinterpret_cast<uint32_t*>(p.count_status + ntiles);
+  uint8_t* sb = c.lx_status.as<uint8_t>();
+  p.count_status = reinterpret_cast<uint64_t*>(sb);
+  p.tile_endo = reinterpret_cast<uint32_t*>(sb + te_off);
+  p.tile_prefix = reinterpret_cast<uint16_t*>(sb + tp_off);
+  p.lane_prefix = reinterpret_cast<uint16_t*>(sb + lp_off);
   p.ntiles = ntiles;
-  if (!c.lex_grid) HIP_TRY(lex_grid_size(p, &c.lex_grid));
-  if ((r = zero_words(c.small.ptr, 16, st)) || (r = zero_words(c.lx_status.ptr, (size_t)ntiles * 12, st))) return r;
-  int grid = c.lex_grid < (int)ntiles ? c.lex_grid : (int)ntiles;
+  if (!c.lex_grid) HIP_TRY(lex_grid_size(p, &c.lex_grid, &c.lex_grid2));
+  if ((r = zero_words(c.small.ptr, 16, st)) || (r = zero_words(c.lx_status.ptr, st_bytes, st))) return r;
   if (c.timing) HIP_TRY(hipEventRecord(c.ev[0], st));
-  HIP_TRY(lex_launch(p, grid, st));
+  HIP_TRY(lex_launch(p, c.lex_grid, c.lex_grid2, st));
   if (c.timing) HIP_TRY(hipEventRecord(c.ev[1], st));
   HIP_TRY(hipMemcpyAsync(c.host_small, c.small.ptr, 16, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));

@@ -25,8 +25,10 @@ bool get_u64(FILE* f, uint64_t* v) { return fread(v, sizeof *v, 1, f) == 1; }
 void put_u64(FILE* f, uint64_t v) { fwrite(&v, sizeof v, 1, f); }
 
 struct Prog {
-  alpacc_hip_ctx* ctx = nullptr;
+  alpacc_hip_ctx* ctx = nullptr;  // opened at the first request
+  std::string grammar;
   int mode = ALPACC_MODE_BOTH;
+  int device = 0;
 };
 
 int one_frame(Prog& p, FILE* in, FILE* out) {
@@ -46,8 +48,12 @@ int one_frame(Prog& p, FILE* in, FILE* out) {
   }
   uint64_t n;
   memcpy(&n, frame.data(), 8);
-  if (flen != 8 + alpacc_hip_payload_bytes(p.ctx, p.mode, n)) {
+  if (flen != 8 + alpacc_hip_grammar_payload_bytes(p.grammar.c_str(), p.mode, n)) {
     fprintf(stderr, "error: frame length mismatch\n");
+    return -1;
+  }
+  if (!p.ctx && alpacc_hip_open(p.grammar.c_str(), p.device, &p.ctx) < 0) {
+    fprintf(stderr, "error: %s\n", alpacc_hip_last_error());
     return -1;
   }
   uint8_t* rec = nullptr;
@@ -157,16 +163,18 @@ int main(int argc, char** argv) {
     usage(argv[0]);
     return 1;
   }
-  if (alpacc_hip_open(grammar.c_str(), device, &p.ctx) < 0) {
-    fprintf(stderr, "error: %s\n", alpacc_hip_last_error());
-    return 1;
-  }
-  if (layout) {
+  p.grammar = grammar;
+  p.device = device;
+  {
     char buf[512];
-    if (alpacc_hip_layout(p.ctx, p.mode, buf, sizeof buf) < 0) return 1;
-    fputs(buf, stdout);
-    alpacc_hip_close(p.ctx);
-    return 0;
+    if (alpacc_hip_grammar_layout(grammar.c_str(), p.mode, buf, sizeof buf) < 0) {
+      fprintf(stderr, "error: %s\n", alpacc_hip_last_error());
+      return 1;
+    }
+    if (layout) {
+      fputs(buf, stdout);
+      return 0;
+    }
   }
   FILE* in = inp ? fopen(inp, "rb") : stdin;
   FILE* out = outp ? fopen(outp, "wb") : stdout;
@@ -196,6 +204,6 @@ int main(int argc, char** argv) {
   }
   if (in != stdin) fclose(in);
   if (out != stdout) fclose(out);
-  alpacc_hip_close(p.ctx);
+  if (p.ctx) alpacc_hip_close(p.ctx);
   return rc;
 }

@@ -38,6 +38,9 @@ struct GrammarDesc {
   const uint8_t* arity;          // [nproductions]
   const uint8_t* prod_is_terminal;  // [nproductions]
   int max_brackets, max_prods;   // per window
+  int max_slot_items;            // per window: productions + child slots they open
+  const uint8_t* pair_rec;       // [npairs * 32] compiled pair records (see emit_hip.py)
+  const uint8_t* prod_info;      // [nproductions] arity | terminal << 7
 };
 
 const GrammarDesc* find_grammar(const char* name);

@@ -1,4 +1,4 @@
-// Parameters and entry points of the tile lexer (lexer.hip).
+// Parameters and entry points of the DFA lexer (lexer.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstddef>
@@ -7,7 +7,7 @@
 namespace alpacc {
 
 constexpr int LEX_WG = 256;   // lanes per workgroup (4 waves)
-constexpr int LEX_IPT = 32;   // input bytes per lane; a tile is 8 KiB
+constexpr int LEX_IPT = 128;  // input bytes per lane; a tile is 32 KiB
 
 constexpr uint32_t LEX_REJECT_FINAL = 1u;     // last state not accepting
 constexpr uint32_t LEX_REJECT_OVERFLOW = 2u;  // a kept token is longer than length_t
@@ -32,10 +32,12 @@ struct LexParams {
   void* lengths;
   uint64_t* count;    // [1] number of kept tokens
   uint32_t* flags;    // [1] LEX_REJECT_* bits
-  // lookback state (zeroed before every launch)
-  uint32_t* endo_status;   // [ntiles]
-  uint64_t* count_status;  // [ntiles]
-  uint32_t* ticket;        // [1]
+  // scan state
+  uint16_t* lane_prefix;   // [ntiles * LEX_WG] exclusive endomorphism of each lane inside its tile
+  uint32_t* tile_endo;     // [ntiles] tile aggregates
+  uint16_t* tile_prefix;   // [ntiles] exclusive tile prefixes
+  uint64_t* count_status;  // [ntiles] lookback words, zeroed before every launch
+  uint32_t* ticket;        // [1] zeroed before every launch
   uint32_t ntiles;
 };
 
@@ -48,9 +50,10 @@ __device__ __forceinline__ void store_length(const LexParams& p, uint64_t slot, 
   }
 }
 
-size_t lex_lds_bytes(const LexParams& p);
+size_t lex_fold_lds(const LexParams& p);
+size_t lex_emit_lds(const LexParams& p);
 uint32_t lex_tile_bytes();
-hipError_t lex_grid_size(const LexParams& p, int* grid);
-hipError_t lex_launch(const LexParams& p, int grid, hipStream_t st);
+hipError_t lex_grid_size(const LexParams& p, int* fold_grid, int* emit_grid);
+hipError_t lex_launch(const LexParams& p, int fold_grid, int emit_grid, hipStream_t st);
 
 }  // namespace alpacc

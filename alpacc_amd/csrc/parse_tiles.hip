@@ -44,10 +44,9 @@ namespace alpacc {
 
 namespace {
 
-constexpr int TWG = 256;         // lanes per workgroup
-constexpr int TPL = 4;           // positions per lane
+constexpr int TWG = 256;         // lanes per workgroup of pass A
+constexpr int TPL = 4;           // positions per lane of pass A
 constexpr int TPOS = TWG * TPL;  // positions per tile
-constexpr int MAXW = 5;          // brackets / productions per position handled in LDS
 
 constexpr uint32_t BAD_PAIR = 1u, BAD_DEPTH = 2u, BAD_MATCH = 4u, BAD_COUNT = 8u, BAD_ROOT = 16u;
 constexpr int64_t TOKEN_TARGET = 1ll << 62;
@@ -180,89 +179,133 @@ __global__ __launch_bounds__(TWG) void pair_profile(Pairs a, uint16_t* pid, Tile
 // ---------------------------------------------------------------- pass S
 
 struct ScanOut {
-  int64_t nb, np, nt;          // totals
-  int64_t bend, send;          // final depths
-  int32_t bmin, pad;           // global minimum bracket depth
+  int64_t nb, np, nt;  // totals
+  int64_t bend, send;  // final depths
 };
 
-__global__ __launch_bounds__(1024) void tile_scan(const TileRec* rec, uint32_t ntiles, TileInfo* info, int32_t* mb,
-                                                   int32_t* ms, ScanOut* out, uint64_t ntokens, uint32_t* flags) {
-  // per lane: a contiguous run of tiles; 9 running sums
-  constexpr int NV = 9;
-  __shared__ int64_t wsum[NV][16];
-  const uint32_t tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const uint32_t per = (ntiles + 1023) / 1024;
-  const uint32_t lo = tid * per, hi = lo + per < ntiles ? lo + per : ntiles;
-  int64_t v[NV] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (uint32_t t = lo; t < hi; ++t) {
-    TileRec r = rec[t];
-    v[0] += r.nb;
-    v[1] += r.np;
-    v[2] += r.nt;
-    v[3] += r.bs;
-    v[4] += r.ss;
-    v[5] += r.bs - r.bm;  // open pushes at the tile end
-    v[6] += -r.bm;        // unmatched pops
-    v[7] += r.ss - r.sm;
-    v[8] += -r.sm;
-  }
-  int64_t inc[NV];
+// The nine running sums over tiles: brackets, slots, terminals, bracket
+// depth, slot depth, open pushes / unmatched pops of both streams.
+constexpr int NV = 9;
+constexpr int SB = 1024;  // tiles per scan block
+
+__device__ __forceinline__ void tile_terms(const TileRec& r, int64_t* v) {
+  v[0] = r.nb;
+  v[1] = r.np;
+  v[2] = r.nt;
+  v[3] = r.bs;
+  v[4] = r.ss;
+  v[5] = r.bs - r.bm;
+  v[6] = -r.bm;
+  v[7] = r.ss - r.sm;
+  v[8] = -r.sm;
+}
+
+// Block-wide exclusive scan of NV int64 values (256 lanes); totals out.
+__device__ __forceinline__ void block_scan_nv(int64_t* v, int64_t* tot) {
+  __shared__ int64_t ws[NV][4];
+  const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
-    inc[k] = (int64_t)wave_scan_add64((uint64_t)v[k]);
-    if (lane == 63) wsum[k][wid] = inc[k];
+    int64_t inc = (int64_t)wave_scan_add64((uint64_t)v[k]);
+    if (lane == 63) ws[k][wid] = inc;
+    v[k] = inc - v[k];
   }
   __syncthreads();
-  int64_t pre[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
-    int64_t o = 0;
-    for (uint32_t w = 0; w < wid; ++w) o += wsum[k][w];
-    pre[k] = o + inc[k] - v[k];
+    int64_t o = 0, t = 0;
+    for (uint32_t w = 0; w < 4; ++w) {
+      if (w < wid) o += ws[k][w];
+      t += ws[k][w];
+    }
+    v[k] += o;
+    tot[k] = t;
   }
-  int32_t gmin = 0x7FFFFFFF;
-  for (uint32_t t = lo; t < hi; ++t) {
-    TileRec r = rec[t];
+  __syncthreads();
+}
+
+// S1: per block of SB tiles, the sums.
+__global__ __launch_bounds__(256) void tile_block_sums(const TileRec* rec, uint32_t ntiles, int64_t* bsum) {
+  int64_t v[NV] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const uint32_t t0 = blockIdx.x * SB + threadIdx.x * (SB / 256);
+  for (int i = 0; i < SB / 256; ++i) {
+    if (t0 + i >= ntiles) break;
+    int64_t x[NV];
+    tile_terms(rec[t0 + i], x);
+    for (int k = 0; k < NV; ++k) v[k] += x[k];
+  }
+  int64_t tot[NV];
+  block_scan_nv(v, tot);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < NV; ++k) bsum[(uint64_t)blockIdx.x * NV + k] = tot[k];
+}
+
+// S2: exclusive scan of the block sums (one workgroup), totals and checks.
+__global__ __launch_bounds__(256) void tile_block_scan(int64_t* bsum, uint32_t nblk, ScanOut* out, uint64_t ntokens,
+                                                       uint32_t* flags) {
+  __shared__ int64_t carry[NV];
+  if (threadIdx.x < NV) carry[threadIdx.x] = 0;
+  __syncthreads();
+  for (uint32_t b0 = 0; b0 < nblk; b0 += 256) {
+    const uint32_t b = b0 + threadIdx.x;
+    int64_t v[NV];
+    for (int k = 0; k < NV; ++k) v[k] = b < nblk ? bsum[(uint64_t)b * NV + k] : 0;
+    int64_t tot[NV];
+    block_scan_nv(v, tot);
+    if (b < nblk)
+      for (int k = 0; k < NV; ++k) bsum[(uint64_t)b * NV + k] = carry[k] + v[k];
+    __syncthreads();
+    if (threadIdx.x < NV) carry[threadIdx.x] += tot[threadIdx.x];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    ScanOut o{carry[0], carry[1], carry[2], carry[3], carry[4]};
+    *out = o;
+    if (carry[3] != 0) atomicOr(flags, BAD_DEPTH);
+    if (ntokens != ~0ull && (uint64_t)carry[2] != ntokens) atomicOr(flags, BAD_COUNT);
+  }
+}
+
+// S3: every tile's prefixes, minima and list offsets.
+__global__ __launch_bounds__(256) void tile_downsweep(const TileRec* rec, uint32_t ntiles, const int64_t* bsum,
+                                                      TileInfo* info, int32_t* mb, int32_t* ms, uint32_t* flags) {
+  int64_t v[NV] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const uint32_t t0 = blockIdx.x * SB + threadIdx.x * (SB / 256);
+  for (int i = 0; i < SB / 256; ++i) {
+    if (t0 + i >= ntiles) break;
+    int64_t x[NV];
+    tile_terms(rec[t0 + i], x);
+    for (int k = 0; k < NV; ++k) v[k] += x[k];
+  }
+  int64_t tot[NV];
+  block_scan_nv(v, tot);
+  for (int k = 0; k < NV; ++k) v[k] += bsum[(uint64_t)blockIdx.x * NV + k];
+  bool neg = false;
+  for (int i = 0; i < SB / 256; ++i) {
+    const uint32_t t = t0 + i;
+    if (t >= ntiles) break;
+    const TileRec r = rec[t];
     TileInfo ti;
-    ti.b_off = pre[0];
-    ti.p_off = pre[1];
-    ti.t_off = pre[2];
-    ti.bstart = (int32_t)pre[3];
-    ti.sstart = (int32_t)(pre[4] + 1);  // the virtual root slot
+    ti.b_off = v[0];
+    ti.p_off = v[1];
+    ti.t_off = v[2];
+    ti.bstart = (int32_t)v[3];
+    ti.sstart = (int32_t)(v[4] + 1);  // the virtual root slot
     ti.bmin = r.bm;
     ti.smin = r.sm;
-    ti.ob_off = (int32_t)pre[5];
-    ti.ub_off = (int32_t)pre[6];
-    ti.os_off = (int32_t)pre[7];
-    ti.us_off = (int32_t)pre[8];
+    ti.ob_off = (int32_t)v[5];
+    ti.ub_off = (int32_t)v[6];
+    ti.os_off = (int32_t)v[7];
+    ti.us_off = (int32_t)v[8];
     info[t] = ti;
     mb[t] = ti.bstart + r.bm;
     ms[t] = ti.sstart + r.sm;
-    gmin = gmin < ti.bstart + r.bm ? gmin : ti.bstart + r.bm;
-    pre[0] += r.nb;
-    pre[1] += r.np;
-    pre[2] += r.nt;
-    pre[3] += r.bs;
-    pre[4] += r.ss;
-    pre[5] += r.bs - r.bm;
-    pre[6] += -r.bm;
-    pre[7] += r.ss - r.sm;
-    pre[8] += -r.sm;
+    neg |= ti.bstart + r.bm < 0;
+    int64_t x[NV];
+    tile_terms(r, x);
+    for (int k = 0; k < NV; ++k) v[k] += x[k];
   }
-  if (gmin < 0) atomicOr(flags, BAD_DEPTH);
-  if (tid == 1023) {
-    ScanOut o;
-    o.nb = pre[0];
-    o.np = pre[1];
-    o.nt = pre[2];
-    o.bend = pre[3];
-    o.send = pre[4];
-    o.bmin = 0;
-    o.pad = 0;
-    *out = o;
-    if (pre[3] != 0) atomicOr(flags, BAD_DEPTH);
-    if (ntokens != ~0ull && (uint64_t)pre[2] != ntokens) atomicOr(flags, BAD_COUNT);
-  }
+  if (neg) atomicOr(flags, BAD_DEPTH);
 }
 
 // ---------------------------------------------------------------- pass B
@@ -272,10 +315,12 @@ struct Resolve {
   uint64_t m;
   const TileInfo* info;
   const int32_t* spans;
+  const int8_t* prof;
   const uint16_t* stacks;
   const uint8_t* prods;
   const uint8_t* arity;
   const uint8_t* isterm;
+  int bcap, scap;     // per-lane LDS stack room (brackets, slots)
   uint16_t* ob_list;  // open pushes (bracket symbols)
   uint16_t* ub_list;  // unmatched pops (bracket symbols)
   int64_t* os_list;   // open child slots (owner's compact index)
@@ -288,199 +333,178 @@ struct Resolve {
   uint32_t* flags;
 };
 
-constexpr int CAP = TPOS * MAXW;  // items per tile in LDS (per stream)
-constexpr int CH = 32;            // chunk width of the LDS min index
+constexpr int RWG = 128;         // lanes per tile in pass B
+constexpr int RPL = TPOS / RWG;  // positions per lane
+constexpr uint16_t TERM_TARGET = 0x8000u;
 
-// Last k < j with a[k] < v (STRICT) or a[k] <= v; -1 if none.  `cm` holds
-// the minimum of every CH-wide chunk.
-template <bool STRICT>
-__device__ __forceinline__ int lds_prev(const int16_t* a, const int16_t* cm, int j, int v) {
-  auto ok = [&](int x) { return STRICT ? x < v : x <= v; };
-  int lo = j & ~(CH - 1);
-  for (int k = j - 1; k >= lo; --k)
-    if (ok(a[k])) return k;
-  for (int c = (j / CH) - 1; c >= 0; --c) {
-    if (ok(cm[c])) {
-      for (int k = c * CH + CH - 1; k >= c * CH; --k)
-        if (ok(a[k])) return k;
-    }
-  }
-  return -1;
-}
-
-__global__ __launch_bounds__(TWG) void tile_resolve(Resolve r) {
-  __shared__ uint16_t bsym[CAP];
-  __shared__ int16_t bdep[CAP];
-  __shared__ uint8_t bhit[CAP];
-  __shared__ uint8_t sprod[CAP];
-  __shared__ int16_t se[CAP];
-  __shared__ uint16_t stinc[CAP];
-  __shared__ __align__(4) uint8_t scons[CAP];
-  __shared__ int16_t bcm[CAP / CH], scm[CAP / CH];
-  __shared__ int32_t wsum[5][TWG / WAVE];
-
-  const uint32_t tile = blockIdx.x;
+// Every lane runs the reference's stack algorithms over its RPL positions:
+// brackets on a symbol stack, derivation slots on a stack of open child slots
+// (one entry per slot still owed, holding the owner's tile-local compact
+// index).  Both stacks live in LDS, growing up from the lane's base; pops
+// that find the lane's stack empty are recorded growing down from the top.
+// A lane's open entries sit at consecutive depth levels just above the
+// lane's minimum, so a pop left over by a later lane at tile-relative level D
+// reads entry D - min - 1 of the last earlier lane whose minimum is below D.
+__global__ __launch_bounds__(RWG) void tile_resolve(Resolve r) {
+  extern __shared__ __align__(16) uint16_t stk[];  // [RWG * bcap] brackets, then [RWG * scap] slots
+  __shared__ int32_t l_bs[RWG], l_bm[RWG], l_bsp[RWG], l_bun[RWG], l_bsuf[RWG];
+  __shared__ int32_t l_ss[RWG], l_sm[RWG], l_ssp[RWG], l_sun[RWG], l_ssuf[RWG];
+  __shared__ int32_t ws[4][RWG / WAVE];
+  const uint32_t tile = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const TileInfo ti = r.info[tile];
-  const uint64_t i0 = (uint64_t)tile * TPOS + (uint64_t)threadIdx.x * TPL;
-  const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const uint64_t i0 = (uint64_t)tile * TPOS + (uint64_t)tid * RPL;
+  uint16_t* bst = stk + tid * r.bcap;
+  uint16_t* sst = stk + RWG * r.bcap + tid * r.scap;
 
-  uint32_t ids[TPL];
-  int32_t nb = 0, np = 0, bd = 0, sd = 0, nt = 0;
+  // phase 0: lane offsets (slots, terminals) and starting depths
+  uint32_t ids[RPL];
+  int32_t np = 0, nt = 0, bd = 0, sd = 0;
 #pragma unroll
-  for (int k = 0; k < TPL; ++k) {
+  for (int k = 0; k < RPL; ++k) {
     uint64_t i = i0 + k;
     ids[k] = i < r.m ? r.pid[i] : 0xFFFFu;
     if (ids[k] != 0xFFFFu) {
-      nb += r.spans[4 * ids[k] + 1];
+      const int8_t* pf = r.prof + 8 * ids[k];
       np += r.spans[4 * ids[k] + 3];
+      nt += (uint8_t)pf[4];
+      bd += pf[0];
+      sd += pf[2];
     }
   }
-  // exclusive offsets of this lane's brackets / slots inside the tile
-  uint32_t ib = wave_scan_add32((uint32_t)nb), ip = wave_scan_add32((uint32_t)np);
+  int32_t x0 = (int32_t)wave_scan_add32((uint32_t)np), x1 = (int32_t)wave_scan_add32((uint32_t)nt),
+          x2 = (int32_t)wave_scan_add32((uint32_t)bd), x3 = (int32_t)wave_scan_add32((uint32_t)sd);
   if (lane == 63) {
-    wsum[0][wid] = (int32_t)ib;
-    wsum[1][wid] = (int32_t)ip;
+    ws[0][wid] = x0;
+    ws[1][wid] = x1;
+    ws[2][wid] = x2;
+    ws[3][wid] = x3;
   }
   __syncthreads();
-  int32_t ob = (int32_t)ib - nb, op = (int32_t)ip - np;
-  int32_t tb = 0, tp = 0;
-  for (int w = 0; w < TWG / WAVE; ++w) {
-    if ((uint32_t)w < wid) {
-      ob += wsum[0][w];
-      op += wsum[1][w];
-    }
-    tb += wsum[0][w];
-    tp += wsum[1][w];
+  int32_t slot = x0 - np, tinc = x1 - nt, bS = x2 - bd, sS = x3 - sd;
+  for (uint32_t w = 0; w < wid; ++w) {
+    slot += ws[0][w];
+    tinc += ws[1][w];
+    bS += ws[2][w];
+    sS += ws[3][w];
   }
-  __syncthreads();
-  // expand: brackets with relative depth-before; slots with the relative
-  // depth after their pop and the inclusive terminal count
-  {
-    // lane depth prefixes need the per-lane sums first
-    int32_t lbd = 0, lsd = 0, lnt = 0;
-#pragma unroll
-    for (int k = 0; k < TPL; ++k) {
-      if (ids[k] == 0xFFFFu) continue;
-      int32_t s0 = r.spans[4 * ids[k]], sl = r.spans[4 * ids[k] + 1];
-      int32_t p0 = r.spans[4 * ids[k] + 2], pl = r.spans[4 * ids[k] + 3];
-      for (int j = 0; j < sl; ++j) lbd += (r.stacks[s0 + j] & 0x8000u) ? 1 : -1;
-      for (int j = 0; j < pl; ++j) {
-        uint8_t pr = r.prods[p0 + j];
-        lsd += (int32_t)r.arity[pr] - 1;
-        lnt += r.isterm[pr];
-      }
-    }
-    uint32_t xb = wave_scan_add32((uint32_t)lbd), xs = wave_scan_add32((uint32_t)lsd),
-             xt = wave_scan_add32((uint32_t)lnt);
-    if (lane == 63) {
-      wsum[2][wid] = (int32_t)xb;
-      wsum[3][wid] = (int32_t)xs;
-      wsum[4][wid] = (int32_t)xt;
-    }
-    __syncthreads();
-    bd = (int32_t)xb - lbd;
-    sd = (int32_t)xs - lsd;
-    nt = (int32_t)xt - lnt;
-    for (uint32_t w = 0; w < wid; ++w) {
-      bd += wsum[2][w];
-      sd += wsum[3][w];
-      nt += wsum[4][w];
-    }
-    int32_t bo = ob, po = op;
-#pragma unroll
-    for (int k = 0; k < TPL; ++k) {
-      if (ids[k] == 0xFFFFu) continue;
-      int32_t s0 = r.spans[4 * ids[k]], sl = r.spans[4 * ids[k] + 1];
-      int32_t p0 = r.spans[4 * ids[k] + 2], pl = r.spans[4 * ids[k] + 3];
-      for (int j = 0; j < sl; ++j) {
-        uint16_t b = r.stacks[s0 + j];
-        bsym[bo] = b;
-        bdep[bo] = (int16_t)bd;
-        bhit[bo] = 0;
-        bd += (b & 0x8000u) ? 1 : -1;
-        ++bo;
-      }
-      for (int j = 0; j < pl; ++j) {
-        uint8_t pr = r.prods[p0 + j];
-        sprod[po] = pr;
-        se[po] = (int16_t)(sd - 1);
-        nt += r.isterm[pr];
-        stinc[po] = (uint16_t)nt;
-        scons[po] = 0;
-        sd += (int32_t)r.arity[pr] - 1;
-        ++po;
-      }
-    }
-  }
-  __syncthreads();
-  for (int c = threadIdx.x; c * CH < tb; c += TWG) {
-    int lo = c * CH, hi = lo + CH < tb ? lo + CH : tb;
-    int mn = 0x7FFF;
-    for (int k = lo; k < hi; ++k) mn = bdep[k] < mn ? bdep[k] : mn;
-    bcm[c] = (int16_t)mn;
-  }
-  for (int c = threadIdx.x; c * CH < tp; c += TWG) {
-    int lo = c * CH, hi = lo + CH < tp ? lo + CH : tp;
-    int mn = 0x7FFF;
-    for (int k = lo; k < hi; ++k) mn = se[k] < mn ? se[k] : mn;
-    scm[c] = (int16_t)mn;
-  }
-  __syncthreads();
 
-  // brackets: pops match the last earlier bracket with a smaller depth-before
+  // phase 1: stacks
+  const int64_t p_off = ti.p_off, t_off = ti.t_off, c_off = p_off - t_off;
+  int bsp = 0, bun = 0, ssp = 0, sun = 0;
+  int32_t bdep = 0, bmin = 0, sdep = 0, smin = 0;
   bool bad = false;
-  for (int j = threadIdx.x; j < tb; j += TWG) {
-    uint16_t b = bsym[j];
-    if (b & 0x8000u) continue;
-    int d = bdep[j];
-    int k = lds_prev<true>(bdep, bcm, j, d);
-    if (k >= 0) {
-      bad |= bsym[k] != (uint16_t)(b | 0x8000u);
-      bhit[k] = 1;
-    } else {
-      r.ub_list[ti.ub_off + (-d)] = b;
+  for (int k = 0; k < RPL; ++k) {
+    const uint32_t id = ids[k];
+    if (id == 0xFFFFu) continue;
+    const int32_t s0 = r.spans[4 * id], sl = r.spans[4 * id + 1];
+    const int32_t p0 = r.spans[4 * id + 2], pl = r.spans[4 * id + 3];
+    for (int j = 0; j < sl; ++j) {
+      const uint16_t b = r.stacks[s0 + j];
+      if (b & 0x8000u) {
+        bst[bsp++] = b;
+        ++bdep;
+      } else {
+        if (bsp > 0) {
+          bad |= bst[--bsp] != (uint16_t)(b | 0x8000u);
+        } else {
+          bst[r.bcap - 1 - bun] = b;
+          ++bun;
+        }
+        --bdep;
+        bmin = bdep < bmin ? bdep : bmin;
+      }
     }
-  }
-  // slots: the parent is the owner of the last earlier slot with e <= e_j
-  const int64_t p_off = ti.p_off, t_off = ti.t_off;
-  if (r.deriv)
-    for (int j = threadIdx.x; j < tp; j += TWG) r.deriv[p_off + j] = sprod[j];
-  for (int j = threadIdx.x; r.tree && j < tp; j += TWG) {
-    int v = se[j];
-    int k = lds_prev<false>(se, scm, j, v);
-    uint8_t pr = sprod[j];
-    int64_t J = p_off + j, tinc = t_off + stinc[j];
-    bool term = r.isterm[pr];
-    int64_t target = term ? (TOKEN_TARGET | (tinc - 1)) : (J - tinc);
-    if (!term) r.tree_prod[J - tinc] = pr;
-    if (k >= 0) {
-      int64_t K = p_off + k;
-      int64_t pc = K - (t_off + stinc[k]);
-      atomicOr(reinterpret_cast<uint32_t*>(scons + (k & ~3)), (1u << (v - se[k])) << (8 * (k & 3)));
-      if (term)
-        r.tok_par[tinc - 1] = pc;
-      else
-        r.tree_par[J - tinc] = pc;
-    } else {
-      r.us_list[ti.us_off + (-(v + 1))] = target;
+    for (int j = 0; j < pl; ++j) {
+      const uint8_t pr = r.prods[p0 + j];
+      const int64_t J = p_off + slot;
+      if (r.deriv) r.deriv[J] = pr;
+      if (r.tree) {
+        const bool term = r.isterm[pr];
+        tinc += term;
+        const int32_t self_c = slot - tinc;  // tile-local compact index (productions)
+        if (!term) r.tree_prod[c_off + self_c] = pr;
+        if (ssp > 0) {
+          const int64_t pc = c_off + sst[--ssp];
+          if (term)
+            r.tok_par[t_off + tinc - 1] = pc;
+          else
+            r.tree_par[c_off + self_c] = pc;
+        } else {
+          sst[r.scap - 1 - sun] = term ? (uint16_t)(TERM_TARGET | (tinc - 1)) : (uint16_t)self_c;
+          ++sun;
+        }
+        --sdep;
+        smin = sdep < smin ? sdep : smin;
+        const int a = r.arity[pr];
+        for (int q = 0; q < a; ++q) sst[ssp++] = (uint16_t)self_c;
+        sdep += a;
+      }
+      ++slot;
     }
   }
   if (bad) atomicOr(r.flags, BAD_MATCH);
+  l_bs[tid] = bS;
+  l_bm[tid] = bS + bmin;
+  l_bsp[tid] = bsp;
+  l_bun[tid] = bun;
+  l_ss[tid] = sS;
+  l_sm[tid] = sS + smin;
+  l_ssp[tid] = ssp;
+  l_sun[tid] = sun;
   __syncthreads();
-  // pushes still open at the tile end, indexed by level above the tile minimum
-  for (int j = threadIdx.x; j < tb; j += TWG) {
-    uint16_t b = bsym[j];
-    if ((b & 0x8000u) && !bhit[j]) r.ob_list[ti.ob_off + (bdep[j] + 1 - ti.bmin - 1)] = b;
+  // suffix minima over later lanes (a lane's open entries above it are consumed)
+  if (tid == 0) {
+    int32_t fb = 0x7FFFFFFF, fs = 0x7FFFFFFF;
+    for (int l = RWG - 1; l >= 0; --l) {
+      l_bsuf[l] = fb;
+      l_ssuf[l] = fs;
+      fb = l_bm[l] < fb ? l_bm[l] : fb;
+      fs = l_sm[l] < fs ? l_sm[l] : fs;
+    }
   }
-  for (int j = threadIdx.x; r.tree && j < tp; j += TWG) {
-    uint8_t pr = sprod[j];
-    int a = r.arity[pr];
-    if (!a) continue;
-    int64_t pc = p_off + j - (t_off + stinc[j]);
-    uint32_t cons = scons[j];
-    for (int l = 0; l < a; ++l)
-      if (!((cons >> l) & 1u)) r.os_list[ti.os_off + (se[j] + 1 + l - ti.smin - 1)] = pc;
+  __syncthreads();
+
+  // phase 2: pops left over by this lane; entries left open at the tile end
+  for (int u = 0; u < bun; ++u) {
+    const int32_t D = bS - u;
+    const uint16_t b = bst[r.bcap - 1 - u];
+    int l = (int)tid - 1;
+    while (l >= 0 && l_bm[l] >= D) --l;
+    if (l >= 0) {
+      const uint16_t push = stk[l * r.bcap + (D - l_bm[l] - 1)];
+      bad |= push != (uint16_t)(b | 0x8000u);
+    } else {
+      r.ub_list[ti.ub_off + (-D)] = b;
+    }
   }
+  for (int e = 0; e < bsp; ++e) {
+    const int32_t v = bS + bmin + 1 + e;
+    if (v <= l_bsuf[tid]) r.ob_list[ti.ob_off + (v - ti.bmin - 1)] = bst[e];
+  }
+  if (r.tree) {
+    for (int u = 0; u < sun; ++u) {
+      const int32_t D = sS - u;
+      const uint16_t tg = sst[r.scap - 1 - u];
+      int l = (int)tid - 1;
+      while (l >= 0 && l_sm[l] >= D) --l;
+      const bool term = tg & TERM_TARGET;
+      const int64_t gtarget = term ? (TOKEN_TARGET | (t_off + (tg & 0x7FFF))) : (c_off + tg);
+      if (l >= 0) {
+        const int64_t pc = c_off + stk[RWG * r.bcap + l * r.scap + (D - l_sm[l] - 1)];
+        if (term)
+          r.tok_par[t_off + (tg & 0x7FFF)] = pc;
+        else
+          r.tree_par[c_off + tg] = pc;
+      } else {
+        r.us_list[ti.us_off + (-D)] = gtarget;
+      }
+    }
+    for (int e = 0; e < ssp; ++e) {
+      const int32_t v = sS + smin + 1 + e;
+      if (v <= l_ssuf[tid]) r.os_list[ti.os_off + (v - ti.smin - 1)] = c_off + sst[e];
+    }
+  }
+  if (bad) atomicOr(r.flags, BAD_MATCH);
 }
 
 // ---------------------------------------------------------------- pass C
@@ -613,14 +637,17 @@ uint64_t tree_scratch(uint64_t n) {
 int parse_on_device(Context& c, const uint8_t* d_tok, uint64_t n, bool tree, hipStream_t st, uint64_t* num_prods) {
   const GrammarDesc* g = c.g;
   *num_prods = 0;
-  if (g->max_brackets > MAXW || g->max_prods > MAXW)
-    return fail(ALPACC_ERR_LIMIT, "grammar has windows with more than %d brackets or productions", MAXW);
+  const int bcap = RPL * g->max_brackets, scap = RPL * g->max_slot_items;
+  const size_t lds_b = (size_t)RWG * (bcap + scap) * 2;
+  if (bcap > 0x7FFF || scap > 0x7FFF || lds_b > 96 * 1024)
+    return fail(ALPACC_ERR_LIMIT, "grammar windows too long for the LDS stacks");
   const uint64_t m = n + 2;
   const uint64_t ntiles = (m + TPOS - 1) / TPOS;
   if (ntiles > 0x7FFFFFFFull) return fail(ALPACC_ERR_LIMIT, "too many tiles");
   int r;
-  // workspace: pid | TileRec | TileInfo | mb | ms | trees | ScanOut
+  // workspace: pid | TileRec | TileInfo | mb | ms | trees | block sums | ScanOut
   const uint64_t tt = tree_scratch(ntiles);
+  const uint32_t nsb = (uint32_t)((ntiles + SB - 1) / SB);
   size_t off_pid = 0;
   size_t off_rec = (off_pid + m * 2 + 255) & ~size_t(255);
   size_t off_info = (off_rec + ntiles * sizeof(TileRec) + 255) & ~size_t(255);
@@ -628,7 +655,8 @@ int parse_on_device(Context& c, const uint8_t* d_tok, uint64_t n, bool tree, hip
   size_t off_ms = (off_mb + ntiles * 4 + 255) & ~size_t(255);
   size_t off_tb = (off_ms + ntiles * 4 + 255) & ~size_t(255);
   size_t off_ts = (off_tb + tt * 4 + 255) & ~size_t(255);
-  size_t off_out = (off_ts + tt * 4 + 255) & ~size_t(255);
+  size_t off_bs = (off_ts + tt * 4 + 255) & ~size_t(255);
+  size_t off_out = (off_bs + (size_t)nsb * NV * 8 + 255) & ~size_t(255);
   size_t total = off_out + sizeof(ScanOut) + 256;
   if ((r = c.ps_a.reserve(total))) return r;
   uint8_t* base = c.ps_a.as<uint8_t>();
@@ -638,6 +666,7 @@ int parse_on_device(Context& c, const uint8_t* d_tok, uint64_t n, bool tree, hip
   int32_t* mb = reinterpret_cast<int32_t*>(base + off_mb);
   int32_t* ms = reinterpret_cast<int32_t*>(base + off_ms);
   ScanOut* sout = reinterpret_cast<ScanOut*>(base + off_out);
+  int64_t* bsum = reinterpret_cast<int64_t*>(base + off_bs);
   uint32_t* flags = reinterpret_cast<uint32_t*>(c.small.as<uint64_t>() + 2);
   if ((r = zero_words(flags, 4, st))) return r;
 
@@ -645,8 +674,9 @@ int parse_on_device(Context& c, const uint8_t* d_tok, uint64_t n, bool tree, hip
   Pairs a{d_tok, m, g->q, g->k, g->window_base, g->empty_t, g->start_t, g->end_t, c.d_window, c.d_spans,
           reinterpret_cast<const int8_t*>(c.d_profile)};
   hipLaunchKernelGGL(pair_profile, dim3((uint32_t)ntiles), dim3(TWG), 0, st, a, pid, rec, flags);
-  hipLaunchKernelGGL(tile_scan, dim3(1), dim3(1024), 0, st, rec, (uint32_t)ntiles, info, mb, ms, sout,
-                     tree ? n : ~0ull, flags);
+  hipLaunchKernelGGL(tile_block_sums, dim3(nsb), dim3(256), 0, st, rec, (uint32_t)ntiles, bsum);
+  hipLaunchKernelGGL(tile_block_scan, dim3(1), dim3(256), 0, st, bsum, nsb, sout, tree ? n : ~0ull, flags);
+  hipLaunchKernelGGL(tile_downsweep, dim3(nsb), dim3(256), 0, st, rec, (uint32_t)ntiles, bsum, info, mb, ms, flags);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(c.host_small + 4, sout, sizeof(ScanOut), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(c.host_small + 2, flags, 4, hipMemcpyDeviceToHost, st));
@@ -670,10 +700,16 @@ int parse_on_device(Context& c, const uint8_t* d_tok, uint64_t n, bool tree, hip
   if ((r = build_tile_tree(mb, ntiles, reinterpret_cast<int32_t*>(base + off_tb), st, &tb)) ||
       (r = build_tile_tree(ms, ntiles, reinterpret_cast<int32_t*>(base + off_ts), st, &ts)))
     return r;
-  Resolve rs{pid, m, info, c.d_spans, c.d_stacks, c.d_productions, c.d_arity, c.d_isterm, ob_list, ub_list,
-             os_list, us_list, c.tr_prod.as<uint8_t>(), c.tr_parent.as<int64_t>(), c.tr_tokpar.as<int64_t>(),
-             tree ? nullptr : c.ps_d.as<uint8_t>(), tree ? 1 : 0, flags};
-  hipLaunchKernelGGL(tile_resolve, dim3((uint32_t)ntiles), dim3(TWG), 0, st, rs);
+  Resolve rs{pid, m, info, c.d_spans, reinterpret_cast<const int8_t*>(c.d_profile), c.d_stacks, c.d_productions,
+             c.d_arity, c.d_isterm, bcap, scap, ob_list, ub_list, os_list, us_list, c.tr_prod.as<uint8_t>(),
+             c.tr_parent.as<int64_t>(), c.tr_tokpar.as<int64_t>(), tree ? nullptr : c.ps_d.as<uint8_t>(),
+             tree ? 1 : 0, flags};
+  if (!c.resolve_lds_set || c.resolve_lds_set < lds_b) {
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(tile_resolve),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b));
+    c.resolve_lds_set = lds_b;
+  }
+  hipLaunchKernelGGL(tile_resolve, dim3((uint32_t)ntiles), dim3(RWG), lds_b, st, rs);
   Far fr{info, tb, ts, ob_list, ub_list, os_list, us_list, c.tr_parent.as<int64_t>(), c.tr_tokpar.as<int64_t>(),
          tree ? 1 : 0, flags};
   hipLaunchKernelGGL(far_resolve, dim3((uint32_t)ntiles), dim3(TWG), 0, st, fr);

@@ -119,6 +119,8 @@ int alpacc_hip_open(const char* grammar, int device, alpacc_hip_ctx** out) {
       (r = upload(&c.d_window, g->window, (size_t)g->nwindow)) ||
       (r = upload(&c.d_spans, g->spans, (size_t)g->npairs * 4)) ||
       (r = upload(&c.d_profile, g->profile, (size_t)g->npairs * 8)) ||
+      (r = upload(&c.d_pair_rec, g->pair_rec, (size_t)g->npairs * 32)) ||
+      (r = upload(&c.d_prod_info, g->prod_info, (size_t)g->nproductions)) ||
       (r = upload(&c.d_stacks, g->stacks, (size_t)g->nstacks)) ||
       (r = upload(&c.d_productions, g->productions, (size_t)g->nprods_flat)) ||
       (r = upload(&c.d_arity, g->arity, (size_t)g->nproductions)) ||
@@ -141,7 +143,7 @@ void alpacc_hip_close(alpacc_hip_ctx* h) {
   Context& c = h->c;
   (void)hipSetDevice(c.device);
   void* ptrs[] = {c.d_class, c.d_step, c.d_compose, c.d_endo_init, c.d_delta_lex, c.d_delta_both, c.d_window,
-                  c.d_spans, c.d_profile, c.d_stacks, c.d_productions, c.d_arity, c.d_isterm};
+                  c.d_spans, c.d_profile, c.d_pair_rec, c.d_prod_info, c.d_stacks, c.d_productions, c.d_arity, c.d_isterm};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (DevBuf* b : c.all_buffers()) b->release();
@@ -152,10 +154,11 @@ void alpacc_hip_close(alpacc_hip_ctx* h) {
   delete h;
 }
 
-int alpacc_hip_layout(const alpacc_hip_ctx* h, int mode, char* buf, size_t cap) {
-  if (!h || !buf) return fail(ALPACC_ERR_ARG, "null argument");
+int alpacc_hip_grammar_layout(const char* grammar, int mode, char* buf, size_t cap) {
+  if (!grammar || !buf) return fail(ALPACC_ERR_ARG, "null argument");
   if (int r = check_mode(mode)) return r;
-  const GrammarDesc* g = h->c.g;
+  const GrammarDesc* g = find_grammar(grammar);
+  if (!g) return fail(ALPACC_ERR_ARG, "unknown grammar '%s'", grammar);
   std::string s;
   char line[64];
   int tb = mode == ALPACC_MODE_LEXER ? g->terminal_bytes_lex : g->terminal_bytes_both;
@@ -178,9 +181,20 @@ int alpacc_hip_layout(const alpacc_hip_ctx* h, int mode, char* buf, size_t cap) 
   return (int)s.size();
 }
 
+int alpacc_hip_layout(const alpacc_hip_ctx* h, int mode, char* buf, size_t cap) {
+  if (!h) return fail(ALPACC_ERR_ARG, "null argument");
+  return alpacc_hip_grammar_layout(h->c.g->name, mode, buf, cap);
+}
+
+uint64_t alpacc_hip_grammar_payload_bytes(const char* grammar, int mode, uint64_t n) {
+  const GrammarDesc* g = grammar ? find_grammar(grammar) : nullptr;
+  if (!g) return 0;
+  return mode == ALPACC_MODE_PARSER ? n * (uint64_t)g->terminal_bytes_both : n;
+}
+
 uint64_t alpacc_hip_payload_bytes(const alpacc_hip_ctx* h, int mode, uint64_t n) {
   if (!h) return 0;
-  return mode == ALPACC_MODE_PARSER ? n * (uint64_t)h->c.g->terminal_bytes_both : n;
+  return alpacc_hip_grammar_payload_bytes(h->c.g->name, mode, n);
 }
 
 void alpacc_hip_free(void* p) { free(p); }

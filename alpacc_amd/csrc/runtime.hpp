@@ -26,7 +26,8 @@ struct Context {
   const GrammarDesc* g = nullptr;
   int device = 0;
   hipStream_t stream = nullptr;
-  int lex_grid = 0;
+  int lex_grid = 0, lex_grid2 = 0;  // persistent grids of the fold and emit kernels
+  size_t resolve_lds_set = 0;        // dynamic LDS granted to the parser's resolve kernel
   // resident tables
   uint8_t* d_class = nullptr;
   uint16_t* d_step = nullptr;
@@ -37,6 +38,8 @@ struct Context {
   uint16_t* d_window = nullptr;
   int32_t* d_spans = nullptr;
   uint8_t* d_profile = nullptr;
+  uint8_t* d_pair_rec = nullptr;
+  uint8_t* d_prod_info = nullptr;
   uint16_t* d_stacks = nullptr;
   uint8_t* d_productions = nullptr;
   uint8_t* d_arity = nullptr;

@@ -112,6 +112,28 @@ def device_tables(g: Grammar) -> dict:
             if not -128 <= v <= 127:
                 raise EmitError(f"{g.name}: pair depth profile {v} exceeds i8")
         profile += [d & 0xFF, dmin & 0xFF, e & 0xFF, emin & 0xFF, nt, 0, 0, 0]
+    # Compiled pair records, 32 bytes each: counts and depth profile (bytes
+    # 0-7), up to 8 brackets (u16, bytes 8-23), up to 8 productions (u8,
+    # bytes 24-31).  One load per window position gives the parser all it
+    # needs; PROD_INFO packs arity | terminal << 7 per production.
+    rec = []
+    for i, (s0, s1, p0, p1) in enumerate(p.spans):
+        if s1 - s0 > 8 or p1 - p0 > 8:
+            raise EmitError(f"{g.name}: a window has more than 8 brackets or productions")
+        pf = profile[8 * i:8 * i + 8]
+        head = [s1 - s0, p1 - p0, pf[4], pf[0], pf[1], pf[2], pf[3], 0]
+        br = stacks16[s0:s1] + [0] * (8 - (s1 - s0))
+        pr = list(p.productions[p0:p1]) + [0] * (8 - (p1 - p0))
+        rec += head + [b for v in br for b in (v & 0xFF, v >> 8)] + pr
+    prod_info = []
+    for prod in range(p.n_productions):
+        if p.arity[prod] > 127:
+            raise EmitError(f"{g.name}: arity above 127")
+        prod_info.append(p.arity[prod] | (0x80 if p.prod_terminal[prod] is not None else 0))
+    # LDS stack room one position can need: its brackets, and its slots plus
+    # the child slots they open
+    max_slot_items = max([(p1 - p0) + sum(p.arity[x] for x in p.productions[p0:p1])
+                          for (_, _, p0, p1) in p.spans] + [1])
     return dict(
         nstate=nstate, nendo=ne, nclass=nc, byte_class=classes, step=step, compose=compose,
         endo_init=endo_init, delta_lex=delta(lo), delta_both=delta(lb),
@@ -120,7 +142,7 @@ def device_tables(g: Grammar) -> dict:
         dead_lex=lo.dead_id, dead_both=lb.dead_id,
         window=window, base=base, spans=spans, stacks=stacks16, productions=p.productions,
         arity=p.arity, prod_is_term=[1 if t is not None else 0 for t in p.prod_terminal],
-        profile=profile,
+        profile=profile, max_slot_items=max_slot_items, pair_rec=rec, prod_info=prod_info,
     )
 
 
@@ -142,6 +164,8 @@ def emit(g: Grammar) -> str:
         _arr("uint16_t", "WINDOW", d["window"]),
         _arr("int32_t", "SPANS", d["spans"]),
         _arr("uint8_t", "PROFILE", d["profile"]),
+        "__attribute__((aligned(32)))\n" + _arr("uint8_t", "PAIR_REC", d["pair_rec"]),
+        _arr("uint8_t", "PROD_INFO", d["prod_info"]),
         _arr("uint16_t", "STACKS", d["stacks"]),
         _arr("uint8_t", "PRODUCTIONS", d["productions"]),
         _arr("uint8_t", "ARITY", d["arity"]),
@@ -160,7 +184,8 @@ def emit(g: Grammar) -> str:
         f"  {max(1, len(p.stacks))}, {ns}::STACKS,",
         f"  {max(1, len(p.productions))}, {ns}::PRODUCTIONS,",
         f"  {p.n_productions}, {p.production_bytes}, {ns}::ARITY, {ns}::PROD_IS_TERMINAL,",
-        f"  {max(1, p.max_brackets)}, {max(1, p.max_prods)},",
+        f"  {max(1, p.max_brackets)}, {max(1, p.max_prods)}, {d['max_slot_items']},",
+        f"  {ns}::PAIR_REC, {ns}::PROD_INFO,",
         "};",
         "}  // namespace alpacc",
         "",

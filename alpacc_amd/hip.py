@@ -67,6 +67,8 @@ def load_library(path: str = LIB_PATH):
         "alpacc_hip_lex_device": (i32, [vp, i32, vp, u64, C.POINTER(Lexemes), vp]),
         "alpacc_hip_parse_device": (i32, [vp, vp, u64, C.POINTER(vp), C.POINTER(u64), vp]),
         "alpacc_hip_lex_parse_device": (i32, [vp, vp, u64, C.POINTER(Tree), vp]),
+        "alpacc_hip_grammar_layout": (i32, [C.c_char_p, i32, C.c_char_p, C.c_size_t]),
+        "alpacc_hip_grammar_payload_bytes": (u64, [C.c_char_p, i32, u64]),
         "alpacc_hip_set_timing": (i32, [vp, i32]),
         "alpacc_hip_timing": (i32, [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(u64)]),
     }

@@ -78,6 +78,11 @@ int alpacc_hip_layout(const alpacc_hip_ctx *ctx, int mode, char *buf, size_t cap
 /* INPUT_BYTES(n) of the mode (C/Generator.hs:64,87,117). */
 uint64_t alpacc_hip_payload_bytes(const alpacc_hip_ctx *ctx, int mode, uint64_t n);
 
+/* The same two answers by grammar name, without a device (a client can
+ * size frames and print --layout before any GPU work). */
+int alpacc_hip_grammar_layout(const char *grammar, int mode, char *buf, size_t cap);
+uint64_t alpacc_hip_grammar_payload_bytes(const char *grammar, int mode, uint64_t n);
+
 /* ---- host-memory entry points ------------------------------------------ */
 
 /* lex_string (backends/c/lexer.c:39-97).  mode is LEXER or BOTH (terminal

@@ -157,3 +157,24 @@ def test_large_conforming(backends, grammar, n):
     data = datagen.generate(grammar, n, seed=3, conforming=True).tobytes()
     check(backends, grammar, "both", data, "large")
     check(backends, grammar, "lexer", data, "large")
+
+
+def test_cli_batch_and_server_replay():
+    """The wire program over the HIP library answers a mixed batch exactly as
+    the oracle program does, and --server fed the batch twice answers twice
+    (tests/testc.sh:102-112)."""
+    import subprocess
+    from alpacc_amd import wire
+    cli = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "alpacc_amd", "lib", "alpacc-hip")
+    for grammar in ("arithmetic", "json", "sexp"):
+        payloads = [b"", b" "] + [datagen.generate(grammar, n, seed=n, conforming=n % 3 != 0).tobytes()
+                                  for n in (1, 17, 300, 5000, 70000)]
+        for mode in ("lexer", "both"):
+            batch = wire.encode_batch(payloads, wire.Layout(mode, 1, 8, 2, 1))
+            flag = ["--lexer"] if mode == "lexer" else []
+            ours = subprocess.run([cli, "-g", grammar] + flag, input=batch, capture_output=True, timeout=120)
+            ref = subprocess.run([pyoracle.program(grammar, mode)], input=batch, capture_output=True, timeout=120)
+            assert ours.returncode == 0 and ours.stdout == ref.stdout, (grammar, mode, ours.stderr)
+            srv = subprocess.run([cli, "-g", grammar, "--server"] + flag, input=batch + batch, capture_output=True,
+                                 timeout=120)
+            assert srv.returncode == 0 and srv.stdout == ref.stdout + ref.stdout, (grammar, mode)
