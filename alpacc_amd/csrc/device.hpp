@@ -8,7 +8,11 @@ namespace alpacc {
 constexpr int WAVE = 64;
 
 // Status words of the decoupled lookback: the flag lives in the top two bits
-// of the same word as the payload so one agent-scope atomic publishes both.
+// of the same word as the payload, so one atomic publishes both and no other
+// memory has to be ordered around it.  The accesses are therefore relaxed
+// (monotonic) at agent scope: coherent across the XCDs' L2s (sc1), but with
+// no release/acquire fences -- on gfx950 those are a writeback and an
+// invalidate of the whole L2, paid on every publish and every probe.
 constexpr uint32_t ST32_AGG = 1u << 30;
 constexpr uint32_t ST32_PRE = 2u << 30;
 constexpr uint32_t ST32_MASK = 3u << 30;
@@ -17,16 +21,16 @@ constexpr uint64_t ST64_PRE = 2ull << 62;
 constexpr uint64_t ST64_MASK = 3ull << 62;
 
 __device__ __forceinline__ void publish32(uint32_t* p, uint32_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ uint32_t observe32(const uint32_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void publish64(uint64_t* p, uint64_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ uint64_t observe64(const uint64_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }

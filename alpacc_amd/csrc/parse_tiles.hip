@@ -56,8 +56,7 @@ struct Pairs {
   uint64_t m;
   int q, k, base, empty_t, start_t, end_t;
   const uint16_t* window;
-  const int32_t* spans;
-  const int8_t* prof;  // 8 bytes per pair
+  const uint2* head;  // first 8 bytes of each 32-byte pair record
 };
 
 __device__ __forceinline__ uint32_t pair_of(const Pairs& a, uint64_t i) {
@@ -124,12 +123,13 @@ __global__ __launch_bounds__(TWG) void pair_profile(Pairs a, uint16_t* pid, Tile
       if (id == 0xFFFFu) {
         bad = true;
       } else {
-        const int8_t* pf = a.prof + 8 * id;
-        nb += a.spans[4 * id + 1];
-        np += a.spans[4 * id + 3];
-        nt += (uint8_t)pf[4];
-        b = sm_cat(b, SM{pf[0], pf[1]});
-        s = sm_cat(s, SM{pf[2], pf[3]});
+        // [brackets, productions, terminals, bracket sum, bracket min, slot sum, slot min, 0]
+        const uint2 h = a.head[4 * id];
+        nb += h.x & 0xFFu;
+        np += (h.x >> 8) & 0xFFu;
+        nt += (h.x >> 16) & 0xFFu;
+        b = sm_cat(b, SM{(int8_t)(h.x >> 24), (int8_t)h.y});
+        s = sm_cat(s, SM{(int8_t)(h.y >> 8), (int8_t)(h.y >> 16)});
       }
     }
     ids[r] = (uint16_t)id;
@@ -314,12 +314,9 @@ struct Resolve {
   const uint16_t* pid;
   uint64_t m;
   const TileInfo* info;
-  const int32_t* spans;
-  const int8_t* prof;
-  const uint16_t* stacks;
-  const uint8_t* prods;
-  const uint8_t* arity;
-  const uint8_t* isterm;
+  const uint4* rec;          // 32-byte pair records
+  const uint8_t* prod_info;  // arity | terminal << 7
+  int nprod;
   int bcap, scap;     // per-lane LDS stack room (brackets, slots)
   uint16_t* ob_list;  // open pushes (bracket symbols)
   uint16_t* ub_list;  // unmatched pops (bracket symbols)
@@ -333,43 +330,100 @@ struct Resolve {
   uint32_t* flags;
 };
 
-constexpr int RWG = 128;         // lanes per tile in pass B
+constexpr int RWG = 256;         // lanes per tile in pass B
 constexpr int RPL = TPOS / RWG;  // positions per lane
+constexpr int RLV = 8;           // sparse-table levels: blocks of 1..128 lanes
+constexpr int RNW = RWG / WAVE;
 constexpr uint16_t TERM_TARGET = 0x8000u;
+
+// Bracket j (0..7) and production j of a pair record held as two 16-byte words.
+__device__ __forceinline__ uint32_t rec_bracket(const uint4& h, const uint4& t, int j) {
+  const uint64_t w = j < 4 ? (((uint64_t)h.w << 32) | h.z) : (((uint64_t)t.y << 32) | t.x);
+  return (uint32_t)(w >> (16 * (j & 3))) & 0xFFFFu;
+}
+__device__ __forceinline__ uint32_t rec_prod(const uint4& t, int j) {
+  const uint64_t w = ((uint64_t)t.w << 32) | t.z;
+  return (uint32_t)(w >> (8 * j)) & 0xFFu;
+}
+
+// Last lane l <= from whose minimum (tb[0][l]) is below D, or a negative
+// value: tb[k][l] is the minimum over lanes [l - 2^k + 1, l] (clipped at 0),
+// and maximal blocks with minimum >= D are skipped, largest first.
+__device__ __forceinline__ int prev_below(const int16_t (*tb)[RWG], int from, int32_t D) {
+  int l = from;
+  if (l < 0 || tb[0][l] < D) return l;
+#pragma unroll
+  for (int k = RLV - 1; k >= 0; --k)
+    if (l >= 0 && tb[k][l] >= D) l -= 1 << k;
+  return l;
+}
+
+// Exclusive minimum over the later lanes of the workgroup.
+__device__ __forceinline__ int32_t suffix_min_after(int32_t v, int32_t* wtot) {
+  const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int32_t inc = v;
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) {
+    const int32_t o = __shfl_down(inc, d, WAVE);
+    if (lane + d < (uint32_t)WAVE) inc = o < inc ? o : inc;
+  }
+  if (lane == 0) wtot[wid] = inc;
+  __syncthreads();
+  int32_t ex = __shfl_down(inc, 1, WAVE);
+  if (lane == WAVE - 1) ex = 0x7FFFFFFF;
+  for (uint32_t w = wid + 1; w < (uint32_t)RNW; ++w) ex = wtot[w] < ex ? wtot[w] : ex;
+  return ex;
+}
 
 // Every lane runs the reference's stack algorithms over its RPL positions:
 // brackets on a symbol stack, derivation slots on a stack of open child slots
 // (one entry per slot still owed, holding the owner's tile-local compact
-// index).  Both stacks live in LDS, growing up from the lane's base; pops
-// that find the lane's stack empty are recorded growing down from the top.
+// index).  The stacks live in LDS interleaved by lane (entry e of lane t at
+// e * RWG + t, so the lanes of a wave touch distinct banks); pops that find
+// the lane's stack empty are recorded from the top entry down.
 // A lane's open entries sit at consecutive depth levels just above the
 // lane's minimum, so a pop left over by a later lane at tile-relative level D
 // reads entry D - min - 1 of the last earlier lane whose minimum is below D.
+// Depths are tile-relative and fit int16 (the host checks the bound).
 __global__ __launch_bounds__(RWG) void tile_resolve(Resolve r) {
-  extern __shared__ __align__(16) uint16_t stk[];  // [RWG * bcap] brackets, then [RWG * scap] slots
-  __shared__ int32_t l_bs[RWG], l_bm[RWG], l_bsp[RWG], l_bun[RWG], l_bsuf[RWG];
-  __shared__ int32_t l_ss[RWG], l_sm[RWG], l_ssp[RWG], l_sun[RWG], l_ssuf[RWG];
-  __shared__ int32_t ws[4][RWG / WAVE];
+  extern __shared__ __align__(16) uint16_t stk[];  // [bcap][RWG] brackets, then [scap][RWG] slots
+  __shared__ int16_t tbb[RLV][RWG], tbs[RLV][RWG];
+  __shared__ uint8_t s_pinfo[256];
+  __shared__ int32_t ws[4][RNW];
+  __shared__ int32_t wmin[2][RNW];
   const uint32_t tile = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  for (int i = tid; i < r.nprod; i += RWG) s_pinfo[i] = r.prod_info[i];
   const TileInfo ti = r.info[tile];
   const uint64_t i0 = (uint64_t)tile * TPOS + (uint64_t)tid * RPL;
-  uint16_t* bst = stk + tid * r.bcap;
-  uint16_t* sst = stk + RWG * r.bcap + tid * r.scap;
+  const int bcap = r.bcap, scap = r.scap;
 
-  // phase 0: lane offsets (slots, terminals) and starting depths
+  // phase 0: pair records, lane offsets (slots, terminals) and starting depths
   uint32_t ids[RPL];
+  if (i0 + RPL <= r.m) {
+    const uint2 v = *reinterpret_cast<const uint2*>(r.pid + i0);
+    ids[0] = v.x & 0xFFFFu;
+    ids[1] = v.x >> 16;
+    ids[2] = v.y & 0xFFFFu;
+    ids[3] = v.y >> 16;
+  } else {
+#pragma unroll
+    for (int k = 0; k < RPL; ++k) ids[k] = i0 + k < r.m ? r.pid[i0 + k] : 0xFFFFu;
+  }
+  uint4 rh[RPL], rt[RPL];
+#pragma unroll
+  for (int k = 0; k < RPL; ++k) {
+    const uint32_t id = ids[k] == 0xFFFFu ? 0u : ids[k];
+    rh[k] = r.rec[2 * id];
+    rt[k] = r.rec[2 * id + 1];
+  }
   int32_t np = 0, nt = 0, bd = 0, sd = 0;
 #pragma unroll
   for (int k = 0; k < RPL; ++k) {
-    uint64_t i = i0 + k;
-    ids[k] = i < r.m ? r.pid[i] : 0xFFFFu;
-    if (ids[k] != 0xFFFFu) {
-      const int8_t* pf = r.prof + 8 * ids[k];
-      np += r.spans[4 * ids[k] + 3];
-      nt += (uint8_t)pf[4];
-      bd += pf[0];
-      sd += pf[2];
-    }
+    if (ids[k] == 0xFFFFu) continue;
+    np += (rh[k].x >> 8) & 0xFFu;
+    nt += (rh[k].x >> 16) & 0xFFu;
+    bd += (int8_t)(rh[k].x >> 24);
+    sd += (int8_t)(rh[k].y >> 8);
   }
   int32_t x0 = (int32_t)wave_scan_add32((uint32_t)np), x1 = (int32_t)wave_scan_add32((uint32_t)nt),
           x2 = (int32_t)wave_scan_add32((uint32_t)bd), x3 = (int32_t)wave_scan_add32((uint32_t)sd);
@@ -390,24 +444,26 @@ __global__ __launch_bounds__(RWG) void tile_resolve(Resolve r) {
 
   // phase 1: stacks
   const int64_t p_off = ti.p_off, t_off = ti.t_off, c_off = p_off - t_off;
+  uint16_t* const bst = stk + tid;
+  uint16_t* const sst = stk + RWG * bcap + tid;
   int bsp = 0, bun = 0, ssp = 0, sun = 0;
   int32_t bdep = 0, bmin = 0, sdep = 0, smin = 0;
   bool bad = false;
+#pragma unroll
   for (int k = 0; k < RPL; ++k) {
-    const uint32_t id = ids[k];
-    if (id == 0xFFFFu) continue;
-    const int32_t s0 = r.spans[4 * id], sl = r.spans[4 * id + 1];
-    const int32_t p0 = r.spans[4 * id + 2], pl = r.spans[4 * id + 3];
+    if (ids[k] == 0xFFFFu) continue;
+    const uint4 h = rh[k], t = rt[k];
+    const int sl = h.x & 0xFFu, pl = (h.x >> 8) & 0xFFu;
     for (int j = 0; j < sl; ++j) {
-      const uint16_t b = r.stacks[s0 + j];
+      const uint32_t b = rec_bracket(h, t, j);
       if (b & 0x8000u) {
-        bst[bsp++] = b;
+        bst[RWG * bsp++] = (uint16_t)b;
         ++bdep;
       } else {
         if (bsp > 0) {
-          bad |= bst[--bsp] != (uint16_t)(b | 0x8000u);
+          bad |= bst[RWG * --bsp] != (uint16_t)(b | 0x8000u);
         } else {
-          bst[r.bcap - 1 - bun] = b;
+          bst[RWG * (bcap - 1 - bun)] = (uint16_t)b;
           ++bun;
         }
         --bdep;
@@ -415,93 +471,93 @@ __global__ __launch_bounds__(RWG) void tile_resolve(Resolve r) {
       }
     }
     for (int j = 0; j < pl; ++j) {
-      const uint8_t pr = r.prods[p0 + j];
+      const uint32_t pr = rec_prod(t, j);
       const int64_t J = p_off + slot;
-      if (r.deriv) r.deriv[J] = pr;
+      if (r.deriv) r.deriv[J] = (uint8_t)pr;
       if (r.tree) {
-        const bool term = r.isterm[pr];
+        const uint32_t info = s_pinfo[pr];
+        const bool term = info & 0x80u;
         tinc += term;
         const int32_t self_c = slot - tinc;  // tile-local compact index (productions)
-        if (!term) r.tree_prod[c_off + self_c] = pr;
+        if (!term) r.tree_prod[c_off + self_c] = (uint8_t)pr;
         if (ssp > 0) {
-          const int64_t pc = c_off + sst[--ssp];
+          const int64_t pc = c_off + sst[RWG * --ssp];
           if (term)
             r.tok_par[t_off + tinc - 1] = pc;
           else
             r.tree_par[c_off + self_c] = pc;
         } else {
-          sst[r.scap - 1 - sun] = term ? (uint16_t)(TERM_TARGET | (tinc - 1)) : (uint16_t)self_c;
+          sst[RWG * (scap - 1 - sun)] = term ? (uint16_t)(TERM_TARGET | (tinc - 1)) : (uint16_t)self_c;
           ++sun;
         }
         --sdep;
         smin = sdep < smin ? sdep : smin;
-        const int a = r.arity[pr];
-        for (int q = 0; q < a; ++q) sst[ssp++] = (uint16_t)self_c;
+        const int a = info & 0x7Fu;
+        for (int q = 0; q < a; ++q) sst[RWG * ssp++] = (uint16_t)self_c;
         sdep += a;
       }
       ++slot;
     }
   }
   if (bad) atomicOr(r.flags, BAD_MATCH);
-  l_bs[tid] = bS;
-  l_bm[tid] = bS + bmin;
-  l_bsp[tid] = bsp;
-  l_bun[tid] = bun;
-  l_ss[tid] = sS;
-  l_sm[tid] = sS + smin;
-  l_ssp[tid] = ssp;
-  l_sun[tid] = sun;
-  __syncthreads();
-  // suffix minima over later lanes (a lane's open entries above it are consumed)
-  if (tid == 0) {
-    int32_t fb = 0x7FFFFFFF, fs = 0x7FFFFFFF;
-    for (int l = RWG - 1; l >= 0; --l) {
-      l_bsuf[l] = fb;
-      l_ssuf[l] = fs;
-      fb = l_bm[l] < fb ? l_bm[l] : fb;
-      fs = l_sm[l] < fs ? l_sm[l] : fs;
+  const int32_t lbm = bS + bmin, lsm = sS + smin;
+  tbb[0][tid] = (int16_t)lbm;
+  tbs[0][tid] = (int16_t)lsm;
+  const int32_t bsuf = suffix_min_after(lbm, wmin[0]);
+  const int32_t ssuf = suffix_min_after(lsm, wmin[1]);
+  // level 0 is visible: suffix_min_after synchronised after it was written
+#pragma unroll
+  for (int k = 1; k < RLV; ++k) {
+    const int h = 1 << (k - 1);
+    int16_t a = tbb[k - 1][tid], c = tbs[k - 1][tid];
+    if ((int)tid >= h) {
+      const int16_t a2 = tbb[k - 1][tid - h], c2 = tbs[k - 1][tid - h];
+      a = a2 < a ? a2 : a;
+      c = c2 < c ? c2 : c;
     }
+    tbb[k][tid] = a;
+    tbs[k][tid] = c;
+    __syncthreads();
   }
-  __syncthreads();
 
-  // phase 2: pops left over by this lane; entries left open at the tile end
+  // phase 2: pops left over by this lane (depth falling, so the matching
+  // lane only moves back); entries left open at the tile end
+  int l = (int)tid - 1;
   for (int u = 0; u < bun; ++u) {
     const int32_t D = bS - u;
-    const uint16_t b = bst[r.bcap - 1 - u];
-    int l = (int)tid - 1;
-    while (l >= 0 && l_bm[l] >= D) --l;
+    const uint32_t b = bst[RWG * (bcap - 1 - u)];
+    l = prev_below(tbb, l, D);
     if (l >= 0) {
-      const uint16_t push = stk[l * r.bcap + (D - l_bm[l] - 1)];
+      const uint16_t push = stk[RWG * (D - tbb[0][l] - 1) + l];
       bad |= push != (uint16_t)(b | 0x8000u);
     } else {
-      r.ub_list[ti.ub_off + (-D)] = b;
+      r.ub_list[ti.ub_off + (-D)] = (uint16_t)b;
     }
   }
   for (int e = 0; e < bsp; ++e) {
-    const int32_t v = bS + bmin + 1 + e;
-    if (v <= l_bsuf[tid]) r.ob_list[ti.ob_off + (v - ti.bmin - 1)] = bst[e];
+    const int32_t v = lbm + 1 + e;
+    if (v <= bsuf) r.ob_list[ti.ob_off + (v - ti.bmin - 1)] = bst[RWG * e];
   }
   if (r.tree) {
+    l = (int)tid - 1;
     for (int u = 0; u < sun; ++u) {
       const int32_t D = sS - u;
-      const uint16_t tg = sst[r.scap - 1 - u];
-      int l = (int)tid - 1;
-      while (l >= 0 && l_sm[l] >= D) --l;
+      const uint32_t tg = sst[RWG * (scap - 1 - u)];
+      l = prev_below(tbs, l, D);
       const bool term = tg & TERM_TARGET;
-      const int64_t gtarget = term ? (TOKEN_TARGET | (t_off + (tg & 0x7FFF))) : (c_off + tg);
       if (l >= 0) {
-        const int64_t pc = c_off + stk[RWG * r.bcap + l * r.scap + (D - l_sm[l] - 1)];
+        const int64_t pc = c_off + stk[RWG * (bcap + D - tbs[0][l] - 1) + l];
         if (term)
-          r.tok_par[t_off + (tg & 0x7FFF)] = pc;
+          r.tok_par[t_off + (tg & 0x7FFFu)] = pc;
         else
           r.tree_par[c_off + tg] = pc;
       } else {
-        r.us_list[ti.us_off + (-D)] = gtarget;
+        r.us_list[ti.us_off + (-D)] = term ? (TOKEN_TARGET | (t_off + (tg & 0x7FFFu))) : (c_off + tg);
       }
     }
     for (int e = 0; e < ssp; ++e) {
-      const int32_t v = sS + smin + 1 + e;
-      if (v <= l_ssuf[tid]) r.os_list[ti.os_off + (v - ti.smin - 1)] = c_off + sst[e];
+      const int32_t v = lsm + 1 + e;
+      if (v <= ssuf) r.os_list[ti.os_off + (v - ti.smin - 1)] = c_off + sst[RWG * e];
     }
   }
   if (bad) atomicOr(r.flags, BAD_MATCH);
@@ -639,8 +695,9 @@ int parse_on_device(Context& c, const uint8_t* d_tok, uint64_t n, bool tree, hip
   *num_prods = 0;
   const int bcap = RPL * g->max_brackets, scap = RPL * g->max_slot_items;
   const size_t lds_b = (size_t)RWG * (bcap + scap) * 2;
-  if (bcap > 0x7FFF || scap > 0x7FFF || lds_b > 96 * 1024)
-    return fail(ALPACC_ERR_LIMIT, "grammar windows too long for the LDS stacks");
+  // tile-relative depths are int16 in pass B
+  if ((int64_t)TPOS * g->max_brackets > 0x7FFF || (int64_t)TPOS * g->max_slot_items > 0x7FFF || lds_b > 96 * 1024)
+    return fail(ALPACC_ERR_LIMIT, "grammar windows too long for the tile stacks");
   const uint64_t m = n + 2;
   const uint64_t ntiles = (m + TPOS - 1) / TPOS;
   if (ntiles > 0x7FFFFFFFull) return fail(ALPACC_ERR_LIMIT, "too many tiles");
@@ -671,8 +728,8 @@ int parse_on_device(Context& c, const uint8_t* d_tok, uint64_t n, bool tree, hip
   if ((r = zero_words(flags, 4, st))) return r;
 
   if (c.timing) HIP_TRY(hipEventRecord(c.ev[2], st));
-  Pairs a{d_tok, m, g->q, g->k, g->window_base, g->empty_t, g->start_t, g->end_t, c.d_window, c.d_spans,
-          reinterpret_cast<const int8_t*>(c.d_profile)};
+  Pairs a{d_tok, m, g->q, g->k, g->window_base, g->empty_t, g->start_t, g->end_t, c.d_window,
+          reinterpret_cast<const uint2*>(c.d_pair_rec)};
   hipLaunchKernelGGL(pair_profile, dim3((uint32_t)ntiles), dim3(TWG), 0, st, a, pid, rec, flags);
   hipLaunchKernelGGL(tile_block_sums, dim3(nsb), dim3(256), 0, st, rec, (uint32_t)ntiles, bsum);
   hipLaunchKernelGGL(tile_block_scan, dim3(1), dim3(256), 0, st, bsum, nsb, sout, tree ? n : ~0ull, flags);
@@ -700,8 +757,7 @@ int parse_on_device(Context& c, const uint8_t* d_tok, uint64_t n, bool tree, hip
   if ((r = build_tile_tree(mb, ntiles, reinterpret_cast<int32_t*>(base + off_tb), st, &tb)) ||
       (r = build_tile_tree(ms, ntiles, reinterpret_cast<int32_t*>(base + off_ts), st, &ts)))
     return r;
-  Resolve rs{pid, m, info, c.d_spans, reinterpret_cast<const int8_t*>(c.d_profile), c.d_stacks, c.d_productions,
-             c.d_arity, c.d_isterm, bcap, scap, ob_list, ub_list, os_list, us_list, c.tr_prod.as<uint8_t>(),
+  Resolve rs{pid, m, info, reinterpret_cast<const uint4*>(c.d_pair_rec), c.d_prod_info, g->nproductions, bcap, scap, ob_list, ub_list, os_list, us_list, c.tr_prod.as<uint8_t>(),
              c.tr_parent.as<int64_t>(), c.tr_tokpar.as<int64_t>(), tree ? nullptr : c.ps_d.as<uint8_t>(),
              tree ? 1 : 0, flags};
   if (!c.resolve_lds_set || c.resolve_lds_set < lds_b) {

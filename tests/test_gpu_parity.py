@@ -152,6 +152,57 @@ def test_deep_nesting(backends):
     check(backends, "arithmetic", "both", expr, "parens 64")
 
 
+def nested_text(grammar, rng, budget, depth_cap):
+    """Random nesting with long runs of opens and closes, so matches land at
+    every distance: inside a lane, lanes apart, tiles apart."""
+    out = []
+    depth = 0
+    opn, cls, sep, atom = {"json": ("[", "]", ",", "1"), "sexp": ("(", ")", " ", "a"),
+                           "arithmetic": ("(", ")", "+", "1")}[grammar]
+    need_item = True
+    while len(out) < budget:
+        r = rng.random()
+        if r < 0.45 and depth < depth_cap:
+            if not need_item:
+                out.append(sep)
+            for _ in range(rng.choice([1, 1, 2, 5, 40])):
+                if depth >= depth_cap:
+                    break
+                out.append(opn)
+                depth += 1
+            need_item = True
+        elif r < 0.75 or depth == 0:
+            if not need_item:
+                out.append(sep)
+            out.append(atom)
+            need_item = False
+        else:
+            if need_item:
+                out.append(atom)
+            for _ in range(min(depth, rng.choice([1, 1, 3, 40]))):
+                out.append(cls)
+                depth -= 1
+            need_item = False
+    if need_item:
+        out.append(atom)
+    out.append(cls * depth)
+    return "".join(out).encode()
+
+
+@pytest.mark.parametrize("grammar", ["json", "sexp", "arithmetic"])
+def test_random_nesting(backends, grammar):
+    rng = random.Random(5)
+    for i, (budget, cap) in enumerate([(3000, 50), (20000, 300), (200000, 2000), (400000, 60)]):
+        data = nested_text(grammar, rng, budget, cap)
+        if grammar == "sexp":
+            data = b"(" + data + b")"
+        elif grammar == "json":
+            data = b"[" + data + b"]"
+        check(backends, grammar, "both", data, f"nesting {i}")
+        rec = backends[grammar].run_case("both", data)
+        assert rec[0] == 1, f"nesting {i} should be accepted"
+
+
 @pytest.mark.parametrize("grammar,n", [("json", 64 << 20), ("sexp", 32 << 20), ("arithmetic", 16 << 20)])
 def test_large_conforming(backends, grammar, n):
     data = datagen.generate(grammar, n, seed=3, conforming=True).tobytes()
