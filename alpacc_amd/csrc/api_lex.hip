@@ -21,26 +21,54 @@ namespace alpacc {
 // Positions and counts travel in 31-bit fields of the lookback words.
 constexpr uint64_t LEX_MAX_INPUT = (1ull << 31) - 1;
 
-int lex_on_device(Context& c, int mode, const uint8_t* d_in, uint64_t n, hipStream_t st, uint64_t* count) {
+int lex_batch_on_device(Context& c, int mode, const uint8_t* d_in, const uint64_t* h_off, uint64_t count,
+                        hipStream_t st) {
   const GrammarDesc* g = c.g;
-  *count = 0;
-  if (n == 0) return ALPACC_VALID;
-  if (n > LEX_MAX_INPUT) return fail(ALPACC_ERR_LIMIT, "input of %llu bytes exceeds the %llu-byte launch limit",
-                                     (unsigned long long)n, (unsigned long long)LEX_MAX_INPUT);
-  const int lb = g->length_bytes;
-  const uint32_t tile = lex_tile_bytes();
-  const uint32_t ntiles = (uint32_t)((n + tile - 1) / tile);
+  if (count == 0 || count > 0xFFFFFFFEull) return fail(ALPACC_ERR_ARG, "batch of %llu inputs", (unsigned long long)count);
+  const uint64_t total = h_off[count] - h_off[0];
+  if (h_off[count] > LEX_MAX_INPUT)
+    return fail(ALPACC_ERR_LIMIT, "inputs of %llu bytes exceed the %llu-byte launch limit",
+                (unsigned long long)h_off[count], (unsigned long long)LEX_MAX_INPUT);
+  // every input owns whole tiles, at least one
+  uint64_t ntiles = 0;
+  for (uint64_t i = 0; i < count; ++i) {
+    if (h_off[i + 1] < h_off[i]) return fail(ALPACC_ERR_ARG, "offsets decrease at input %llu", (unsigned long long)i);
+    const uint64_t n = h_off[i + 1] - h_off[i];
+    ntiles += n ? (n + LEX_TILE - 1) / LEX_TILE : 1;
+  }
+  if (ntiles > 0x7FFFFFFFull) return fail(ALPACC_ERR_LIMIT, "too many lexer tiles");
+  const BatchLayout L(count, ntiles);
   int r;
+  if ((r = c.lx_meta.reserve(L.total)) || (r = c.h_meta.reserve(L.total))) return r;
+  uint8_t* hm = c.h_meta.as<uint8_t>();
+  uint8_t* dm = c.lx_meta.as<uint8_t>();
+  uint64_t* h_seg = reinterpret_cast<uint64_t*>(hm + L.off_seg);
+  uint32_t* h_tf = reinterpret_cast<uint32_t*>(hm + L.off_tf);
+  uint32_t t = 0;
+  for (uint64_t i = 0; i < count; ++i) {
+    h_seg[i] = h_off[i];
+    h_tf[i] = t;
+    const uint64_t n = h_off[i + 1] - h_off[i];
+    t += n ? (uint32_t)((n + LEX_TILE - 1) / LEX_TILE) : 1u;
+  }
+  h_seg[count] = h_off[count];
+  h_tf[count] = t;
+
+  const int lb = g->length_bytes;
+  const uint64_t cap = total + 1;
   // scan state: count_status u64[ntiles] | tile_endo u32[ntiles] | tile_prefix u16[ntiles] | lane_prefix
   const size_t st_bytes = (size_t)ntiles * 8, te_off = st_bytes, tp_off = te_off + (size_t)ntiles * 4,
                lp_off = (tp_off + (size_t)ntiles * 2 + 255) & ~size_t(255);
-  if ((r = c.lx_term.reserve(n)) || (r = c.lx_start.reserve(n * 8)) || (r = c.lx_len.reserve(n * lb)) ||
+  if ((r = c.lx_term.reserve(cap)) || (r = c.lx_start.reserve(cap * 8)) || (r = c.lx_len.reserve(cap * lb)) ||
       (r = c.lx_status.reserve(lp_off + (size_t)ntiles * LEX_WG * 2 + 64)))
     return r;
   LexParams p{};
   p.in = d_in;
-  p.n = n;
-  p.aligned16 = ((uintptr_t)d_in & 15) == 0;
+  p.in_bytes = h_off[count];
+  p.seg_off = reinterpret_cast<const uint64_t*>(dm + L.off_seg);
+  p.tile_first = reinterpret_cast<const uint32_t*>(dm + L.off_tf);
+  p.tile_seg = reinterpret_cast<uint32_t*>(dm + L.off_tseg);
+  p.nseg = (uint32_t)count;
   p.byte_class = c.d_class;
   p.step = c.d_step;
   p.compose = c.d_compose;
@@ -57,32 +85,48 @@ int lex_on_device(Context& c, int mode, const uint8_t* d_in, uint64_t n, hipStre
   p.terminals = c.lx_term.as<uint8_t>();
   p.starts = c.lx_start.as<int64_t>();
   p.lengths = c.lx_len.ptr;
-  uint64_t* small = c.small.as<uint64_t>();
-  p.count = small;                                        // [0]
-  p.flags = reinterpret_cast<uint32_t*>(small + 1);       // [8..12)
-  p.ticket = reinterpret_cast<uint32_t*>(small + 1) + 1;  // [12..16)
+  p.seg_flags = reinterpret_cast<uint32_t*>(dm + L.off_flags);
+  p.seg_tok = reinterpret_cast<uint64_t*>(dm + L.off_tok);
+  p.ticket = reinterpret_cast<uint32_t*>(c.small.as<uint64_t>() + 1) + 1;
   uint8_t* sb = c.lx_status.as<uint8_t>();
   p.count_status = reinterpret_cast<uint64_t*>(sb);
   p.tile_endo = reinterpret_cast<uint32_t*>(sb + te_off);
   p.tile_prefix = reinterpret_cast<uint16_t*>(sb + tp_off);
   p.lane_prefix = reinterpret_cast<uint16_t*>(sb + lp_off);
-  p.ntiles = ntiles;
+  p.ntiles = (uint32_t)ntiles;
   if (!c.lex_grid) HIP_TRY(lex_grid_size(p, &c.lex_grid, &c.lex_grid2));
-  if ((r = zero_words(c.small.ptr, 16, st)) || (r = zero_words(c.lx_status.ptr, st_bytes, st))) return r;
+  HIP_TRY(hipMemcpyAsync(dm, hm, L.up_bytes, hipMemcpyHostToDevice, st));
+  if ((r = zero_words(c.small.ptr, 16, st)) || (r = zero_words(c.lx_status.ptr, st_bytes, st)) ||
+      (r = zero_words(dm + L.off_flags, count * 4, st)))
+    return r;
   if (c.timing) HIP_TRY(hipEventRecord(c.ev[0], st));
-  HIP_TRY(lex_launch(p, c.lex_grid, c.lex_grid2, st));
+  HIP_TRY(lex_launch(p, c.lex_grid, c.lex_grid2, st, c.timer));
   if (c.timing) HIP_TRY(hipEventRecord(c.ev[1], st));
-  HIP_TRY(hipMemcpyAsync(c.host_small, c.small.ptr, 16, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(hm + L.off_tok, dm + L.off_tok, L.down_bytes, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
+  HIP_TRY(c.timer.collect());
   if (c.timing) {
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, c.ev[0], c.ev[1]));
     c.lex_ms += ms;
     c.lex_launches++;
   }
-  uint32_t flags = reinterpret_cast<uint32_t*>(c.host_small + 1)[0];
-  *count = c.host_small[0];
-  return flags ? ALPACC_REJECT : ALPACC_VALID;
+  c.lb_count = count;
+  c.lb_tok = reinterpret_cast<const uint64_t*>(hm + L.off_tok);
+  c.lb_flags = reinterpret_cast<const uint32_t*>(hm + L.off_flags);
+  c.d_lb_tok = reinterpret_cast<const uint64_t*>(dm + L.off_tok);
+  c.d_lb_flags = reinterpret_cast<const uint32_t*>(dm + L.off_flags);
+  return 0;
+}
+
+int lex_on_device(Context& c, int mode, const uint8_t* d_in, uint64_t n, hipStream_t st, uint64_t* count) {
+  *count = 0;
+  if (n == 0) return ALPACC_VALID;
+  const uint64_t off[2] = {0, n};
+  int r = lex_batch_on_device(c, mode, d_in, off, 1, st);
+  if (r < 0) return r;
+  *count = c.lb_tok[1] - c.lb_tok[0];
+  return c.lb_flags[0] ? ALPACC_REJECT : ALPACC_VALID;
 }
 
 // Stage a host buffer on the device (grow-only staging buffer).

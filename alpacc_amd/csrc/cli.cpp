@@ -7,8 +7,9 @@
 // Invoked through a link named GRAMMAR or GRAMMAR_MODE (MODE = lexer, parser,
 // both) it needs no -g / mode flag, so it can stand in for a generated
 // program: batch mode reads one counted batch, --server loops over counted
-// batches until EOF and flushes after every response record.  Framing errors
-// go to stderr with exit status 1, as in the reference.
+// batches until EOF and flushes after every batch.  The frames of a batch go
+// to the GPU together (alpacc_hip_run_batch).  Framing errors go to stderr
+// with exit status 1, as in the reference.
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
@@ -25,13 +26,16 @@ bool get_u64(FILE* f, uint64_t* v) { return fread(v, sizeof *v, 1, f) == 1; }
 void put_u64(FILE* f, uint64_t v) { fwrite(&v, sizeof v, 1, f); }
 
 struct Prog {
-  alpacc_hip_ctx* ctx = nullptr;  // opened at the first request
+  alpacc_hip_ctx* ctx = nullptr;     // batch mode: opened at the first request
+  alpacc_hip_server* srv = nullptr;  // --server: opened at the first request
   std::string grammar;
   int mode = ALPACC_MODE_BOTH;
   int device = 0;
 };
 
-int one_frame(Prog& p, FILE* in, FILE* out) {
+// Reads one frame's payload into `packed`.  0 = ok, -1 = framing error
+// (reported on stderr, as backends/c/cli.c:98-143 does).
+int read_frame(Prog& p, FILE* in, std::vector<uint8_t>& packed, std::vector<uint64_t>& off) {
   uint64_t flen;
   if (!get_u64(in, &flen)) {
     fprintf(stderr, "error: truncated input\n");
@@ -41,25 +45,35 @@ int one_frame(Prog& p, FILE* in, FILE* out) {
     fprintf(stderr, "error: bad frame length\n");
     return -1;
   }
-  std::vector<uint8_t> frame(flen);
-  if (fread(frame.data(), 1, flen, in) != flen) {
+  uint64_t n;
+  if (!get_u64(in, &n)) {
     fprintf(stderr, "error: truncated frame\n");
     return -1;
   }
-  uint64_t n;
-  memcpy(&n, frame.data(), 8);
+  const size_t at = packed.size();
+  packed.resize(at + (flen - 8));
+  if (fread(packed.data() + at, 1, flen - 8, in) != flen - 8) {
+    fprintf(stderr, "error: truncated frame\n");
+    return -1;
+  }
   if (flen != 8 + alpacc_hip_grammar_payload_bytes(p.grammar.c_str(), p.mode, n)) {
     fprintf(stderr, "error: frame length mismatch\n");
     return -1;
   }
+  off.push_back(packed.size());
+  return 0;
+}
+
+// Answers the frames read so far with one batch call.
+int answer(Prog& p, const std::vector<uint8_t>& packed, const std::vector<uint64_t>& off, FILE* out) {
+  if (off.size() < 2) return 0;
   if (!p.ctx && alpacc_hip_open(p.grammar.c_str(), p.device, &p.ctx) < 0) {
     fprintf(stderr, "error: %s\n", alpacc_hip_last_error());
     return -1;
   }
   uint8_t* rec = nullptr;
   uint64_t len = 0;
-  int r = alpacc_hip_run_case(p.ctx, p.mode, n, frame.data() + 8, &rec, &len);
-  if (r < 0) {
+  if (alpacc_hip_run_batch(p.ctx, p.mode, packed.data(), off.data(), off.size() - 1, &rec, &len) < 0) {
     fprintf(stderr, "error: %s\n", alpacc_hip_last_error());
     return -1;
   }
@@ -68,11 +82,56 @@ int one_frame(Prog& p, FILE* in, FILE* out) {
   return 0;
 }
 
-// 0 = batch done, 1 = clean EOF before a count (server), -1 = error
-int one_batch(Prog& p, FILE* in, FILE* out, bool eof_ok, bool flush_each) {
+// --server: every frame goes to the serving runtime as soon as it is read,
+// so requests of a batch are on the GPU while later ones still arrive; the
+// records are written in request order.
+int serve_batch(Prog& p, FILE* in, FILE* out, uint64_t count) {
+  std::vector<uint8_t> packed;
+  std::vector<uint64_t> off{0}, tickets;
+  int rc = 0;
+  for (uint64_t t = 0; t < count; ++t) {
+    const size_t before = off.size();
+    if (read_frame(p, in, packed, off) != 0) {
+      rc = -1;
+      break;
+    }
+    if (off.size() == before) continue;
+    if (!p.srv) {
+      const char* g = p.grammar.c_str();
+      if (alpacc_hip_server_open(&g, 1, p.device, 2, 0, 0, &p.srv) < 0) {
+        fprintf(stderr, "error: %s\n", alpacc_hip_last_error());
+        return -1;
+      }
+    }
+    uint64_t ticket;
+    const uint64_t a = off[off.size() - 2], b = off.back();
+    if (alpacc_hip_server_submit(p.srv, 0, p.mode, packed.data() + a, b - a, &ticket) < 0) {
+      fprintf(stderr, "error: %s\n", alpacc_hip_last_error());
+      return -1;
+    }
+    tickets.push_back(ticket);
+  }
+  for (uint64_t t : tickets) {
+    uint8_t* rec = nullptr;
+    uint64_t len = 0;
+    if (alpacc_hip_server_wait(p.srv, t, &rec, &len) < 0) {
+      fprintf(stderr, "error: %s\n", alpacc_hip_last_error());
+      return -1;
+    }
+    fwrite(rec, 1, len, out);
+    alpacc_hip_free(rec);
+  }
+  return rc;
+}
+
+// 0 = batch done, 1 = clean EOF before a count (server), -1 = error.  The
+// frames of a counted batch are read first and answered by one batch call;
+// on a framing error the frames before it are answered, then the error is
+// reported, so the output is the reference's frame-by-frame output.
+int one_batch(Prog& p, FILE* in, FILE* out, bool server) {
   uint64_t count;
   if (!get_u64(in, &count)) {
-    if (eof_ok && feof(in)) return 1;
+    if (server && feof(in)) return 1;
     fprintf(stderr, "error: truncated input\n");
     return -1;
   }
@@ -81,11 +140,18 @@ int one_batch(Prog& p, FILE* in, FILE* out, bool eof_ok, bool flush_each) {
     return -1;
   }
   put_u64(out, count);
+  if (server) return serve_batch(p, in, out, count);
+  std::vector<uint8_t> packed;
+  std::vector<uint64_t> off{0};
   for (uint64_t t = 0; t < count; ++t) {
-    if (one_frame(p, in, out) != 0) return -1;
-    if (flush_each) fflush(out);
+    if (read_frame(p, in, packed, off) != 0) {
+      // the message is already on stderr; answer what came before it
+      fflush(stderr);
+      if (answer(p, packed, off, out) != 0) return -1;
+      return -1;
+    }
   }
-  return 0;
+  return answer(p, packed, off, out);
 }
 
 int parse_mode(const std::string& s) {
@@ -101,7 +167,7 @@ void usage(const char* prog) {
           "  -i FILE    input file  (default: stdin)\n"
           "  -o FILE    output file (default: stdout)\n"
           "  --timeit   print wall-clock elapsed time to stderr\n"
-          "  --server   counted-batch loop until EOF (flush per response)\n"
+          "  --server   counted-batch loop until EOF (flush per batch)\n"
           "  --layout   print native type sizes and exit\n"
           "  --device N GPU ordinal (default 0)\n"
           "  -h/--help  show this message\n",
@@ -186,7 +252,7 @@ int main(int argc, char** argv) {
   int rc = 0;
   if (server) {
     for (;;) {
-      int r = one_batch(p, in, out, true, true);
+      int r = one_batch(p, in, out, true);
       if (r == 1) break;
       if (r < 0) {
         rc = 1;
@@ -195,7 +261,7 @@ int main(int argc, char** argv) {
       fflush(out);
     }
   } else {
-    rc = one_batch(p, in, out, false, false) == 0 ? 0 : 1;
+    rc = one_batch(p, in, out, false) == 0 ? 0 : 1;
     fflush(out);
   }
   if (timeit) {
@@ -205,5 +271,6 @@ int main(int argc, char** argv) {
   if (in != stdin) fclose(in);
   if (out != stdout) fclose(out);
   if (p.ctx) alpacc_hip_close(p.ctx);
+  if (p.srv) alpacc_hip_server_close(p.srv);
   return rc;
 }

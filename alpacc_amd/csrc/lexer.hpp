@@ -4,18 +4,28 @@
 #include <cstddef>
 #include <cstdint>
 
+#include "timer.hpp"
+
 namespace alpacc {
 
 constexpr int LEX_WG = 256;   // lanes per workgroup (4 waves)
 constexpr int LEX_IPT = 128;  // input bytes per lane; a tile is 32 KiB
+constexpr uint32_t LEX_TILE = (uint32_t)LEX_WG * LEX_IPT;
 
 constexpr uint32_t LEX_REJECT_FINAL = 1u;     // last state not accepting
 constexpr uint32_t LEX_REJECT_OVERFLOW = 2u;  // a kept token is longer than length_t
 
+// A launch lexes `nseg` inputs packed back to back in `in` (a single input
+// is a batch of one).  Every input owns whole tiles -- at least one, so an
+// empty input has a tile with no bytes -- and tile_first[i] is the first
+// tile of input i (tile_first[nseg] = ntiles).
 struct LexParams {
   const uint8_t* in;
-  uint64_t n;
-  int aligned16;
+  uint64_t in_bytes;           // bytes of the packed inputs at `in`
+  const uint64_t* seg_off;     // [nseg + 1] byte offset of each input in `in`
+  const uint32_t* tile_first;  // [nseg + 1]
+  uint32_t* tile_seg;          // [ntiles] input of each tile (written by tile_map)
+  uint32_t nseg;
   // tables
   const uint8_t* byte_class;
   const uint16_t* step;
@@ -26,16 +36,16 @@ struct LexParams {
   int ignore, dead;
   uint64_t max_len;
   int length_bytes;
-  // outputs (capacity n tokens)
+  // outputs (capacity: the total input bytes); starts relative to the input
   uint8_t* terminals;
   int64_t* starts;
   void* lengths;
-  uint64_t* count;    // [1] number of kept tokens
-  uint32_t* flags;    // [1] LEX_REJECT_* bits
+  uint32_t* seg_flags;  // [nseg] LEX_REJECT_* bits, zeroed before the launch
+  uint64_t* seg_tok;    // [nseg + 1] first token slot of each input; [nseg] = all tokens
   // scan state
   uint16_t* lane_prefix;   // [ntiles * LEX_WG] exclusive endomorphism of each lane inside its tile
   uint32_t* tile_endo;     // [ntiles] tile aggregates
-  uint16_t* tile_prefix;   // [ntiles] exclusive tile prefixes
+  uint16_t* tile_prefix;   // [ntiles] exclusive tile prefixes inside the input
   uint64_t* count_status;  // [ntiles] lookback words, zeroed before every launch
   uint32_t* ticket;        // [1] zeroed before every launch
   uint32_t ntiles;
@@ -50,10 +60,12 @@ __device__ __forceinline__ void store_length(const LexParams& p, uint64_t slot, 
   }
 }
 
+// tile_seg[t] = the last i with tile_first[i] <= t, for t < ntiles.
+__global__ void tile_map(const uint32_t* tile_first, uint32_t nseg, uint32_t ntiles, uint32_t* tile_seg);
+
 size_t lex_fold_lds(const LexParams& p);
 size_t lex_emit_lds(const LexParams& p);
-uint32_t lex_tile_bytes();
 hipError_t lex_grid_size(const LexParams& p, int* fold_grid, int* emit_grid);
-hipError_t lex_launch(const LexParams& p, int fold_grid, int emit_grid, hipStream_t st);
+hipError_t lex_launch(const LexParams& p, int fold_grid, int emit_grid, hipStream_t st, KernelTimer& tm);
 
 }  // namespace alpacc

@@ -1,0 +1,907 @@
+// LLP parser for gfx950 over a batch of token strings (a single string is a
+// batch of one).
+//
+// Semantics are those of backends/c/parser.c: parse_test (:49-98) checks that
+// every window has a pair and that the concatenated bracket strings are
+// balanced and symbol-matched; compute_parents (:103-118) and compact_tree
+// (:126-153) build the productions-only tree.  Both checks are matching
+// problems over depth levels:
+//   * brackets: a right bracket pops the level its depth-before names;
+//   * derivation slots: every production pops one open child slot of its
+//     parent and pushes `arity` new ones, so its parent is the owner of the
+//     slot its pop takes (the first production pops a virtual root slot).
+// The generator reduces every window to its net effect (pair records: pops
+// reaching below the window, surviving pushes, and each production's parent
+// when it lies inside the window), so a position is a handful of pops and
+// pushes.  The match of a pop at level L is the last earlier position whose
+// depth after its own pops is below L; it holds the push at that level.
+//
+// Every input owns whole tiles of PT framed positions ([start] tokens [end]);
+// one lane per position.
+//   A  seg_profile   pair ids and per-tile sums and (sum, min) depth
+//                    profiles of both streams;
+//   S  tile scans    tile offsets and starting depths (the depth scans run
+//                    across inputs; each input's first tile adds the virtual
+//                    root slot, and every check is relative to the input);
+//   B  seg_resolve   per tile: block scans, a sparse table of the positions'
+//                    post-pop depths in LDS, every pop finds its push inside
+//                    the tile in O(log PT) LDS reads, or goes to the tile's
+//                    unmatched list; pushes still open at the tile end go to
+//                    its open list; outputs staged in LDS, written coalesced;
+//   C  far_resolve   each unmatched pop finds the last earlier tile of its
+//                    input whose minimum is below its level (a 32-ary min
+//                    tree over tiles) and reads the open push there;
+//   F  seg_finish    per input: balance, counts, offsets, verdict.
+#include <hip/hip_runtime.h>
+
+#include "../../include/alpacc_hip.h"
+#include "device.hpp"
+#include "lexer.hpp"
+#include "parser.hpp"
+
+namespace alpacc {
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) return fail(ALPACC_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+namespace {
+
+constexpr int PT = 512;          // framed positions per tile = lanes per workgroup
+constexpr int PNW = PT / WAVE;   // waves per workgroup
+constexpr int PLV = 9;           // sparse-table levels: blocks of 1..256 positions
+// Reject reasons.  parse_test (parser.c:49-98) rejects on pairs and brackets
+// only; the tree checks add the terminal count (compact_tree, parser.c:
+// 126-153) and the derivation-slot structure.
+constexpr uint32_t BAD_PAIR = 1u, BAD_BDEPTH = 2u, BAD_MATCH = 4u, BAD_COUNT = 8u, BAD_ROOT = 16u, BAD_SDEPTH = 32u;
+constexpr uint32_t PARSE_TEST_CHECKS = BAD_PAIR | BAD_BDEPTH | BAD_MATCH;
+constexpr int64_t TOKEN_TARGET = 1ll << 62;
+constexpr uint32_t NO_PAIR = 0xFFFFu;
+
+// Batch and grammar, shared by the passes.
+struct Seg {
+  const uint8_t* tok;          // token ids of all inputs
+  const uint64_t* tok_off;     // [count + 1] first token of each input
+  const uint32_t* tile_first;  // [count + 1] first tile of each input (inputs not parsed have none)
+  const uint32_t* tile_seg;    // [ntiles] input of each tile
+  uint32_t count, ntiles;
+  int q, k, base, empty_t, start_t, end_t;
+  const uint16_t* window;
+  const uint4* rec;  // 64-byte pair records, 4 x uint4 each
+  uint32_t* flags;   // [count] BAD_* bits
+};
+
+// The tile's input and framed length.
+struct TileOf {
+  uint32_t seg, lt;  // input, tile index inside the input
+  int64_t m;         // framed positions of the input: tokens + 2
+  const uint8_t* tk;
+};
+
+__device__ __forceinline__ TileOf tile_of(const Seg& s, uint32_t tile) {
+  TileOf t;
+  t.seg = s.tile_seg[tile];
+  t.lt = tile - s.tile_first[t.seg];
+  const uint64_t a = s.tok_off[t.seg], b = s.tok_off[t.seg + 1];
+  t.m = (int64_t)(b - a) + 2;
+  t.tk = s.tok + a;
+  return t;
+}
+
+// Pair id of framed position p, or NO_PAIR.
+__device__ __forceinline__ uint32_t pair_at(const Seg& s, const TileOf& t, int64_t p) {
+  uint32_t code = 0;
+  const int w = s.q + s.k;
+  for (int j = 0; j < w; ++j) {
+    const int64_t f = p - s.q + j;
+    uint32_t v;
+    if (f < 0 || f >= t.m)
+      v = s.empty_t;
+    else if (f == 0)
+      v = s.start_t;
+    else if (f == t.m - 1)
+      v = s.end_t;
+    else
+      v = t.tk[f - 1];
+    if (v >= (uint32_t)s.base) return NO_PAIR;
+    code = code * s.base + v;
+  }
+  return s.window[code];
+}
+
+// (sum, min-prefix) monoid: a then b.
+struct SM {
+  int32_t s, m;
+};
+__device__ __forceinline__ SM sm_cat(SM a, SM b) { return {a.s + b.s, a.m < a.s + b.m ? a.m : a.s + b.m}; }
+
+__device__ __forceinline__ SM wave_scan_sm(SM v) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) {
+    SM o{__shfl_up(v.s, d, WAVE), __shfl_up(v.m, d, WAVE)};
+    if (lane >= (uint32_t)d) v = sm_cat(o, v);
+  }
+  return v;
+}
+
+struct TileRec {
+  int32_t np, nt, bs, bm, ss, sm, pad0, pad1;
+};
+
+struct TileInfo {
+  int64_t p_off, t_off;    // slots and terminal productions before the tile (all inputs)
+  int64_t bstart, sstart;  // depths at the tile start (sstart includes its input's root slot)
+  int32_t bmin, smin;      // tile-relative minima
+  int32_t ob_off, ub_off, os_off, us_off;
+};
+
+// Per-input values written by the first tile of the input.
+struct SegBase {
+  int64_t p_off, t_off, bstart, sstart;
+};
+
+// Record fields: byte i of the first 8 bytes.
+__device__ __forceinline__ uint32_t hb(const uint4& r0, int i) {
+  return ((i < 4 ? r0.x : r0.y) >> (8 * (i & 3))) & 0xFFu;
+}
+// Bracket symbol j (0..7).
+__device__ __forceinline__ uint32_t rec_br(const uint4& r0, const uint4& r1, int j) {
+  const uint64_t w = j < 4 ? (((uint64_t)r0.w << 32) | r0.z) : (((uint64_t)r1.y << 32) | r1.x);
+  return (uint32_t)(w >> (16 * (j & 3))) & 0xFFFFu;
+}
+__device__ __forceinline__ uint32_t rec_prod(const uint4& r1, int j) {
+  return (uint32_t)((((uint64_t)r1.w << 32) | r1.z) >> (8 * j)) & 0xFFu;
+}
+__device__ __forceinline__ uint32_t rec_parent(const uint4& r2, int j) {
+  return (uint32_t)((((uint64_t)r2.y << 32) | r2.x) >> (8 * j)) & 0xFFu;
+}
+// Owner of surviving child slot e (0..23).
+__device__ __forceinline__ uint32_t rec_owner(const uint4& r2, const uint4& r3, int e) {
+  uint32_t w;
+  switch (e >> 2) {
+    case 0: w = r2.z; break;
+    case 1: w = r2.w; break;
+    case 2: w = r3.x; break;
+    case 3: w = r3.y; break;
+    case 4: w = r3.z; break;
+    default: w = r3.w; break;
+  }
+  return (w >> (8 * (e & 3))) & 0xFFu;
+}
+
+// ---------------------------------------------------------------- pass A
+
+__global__ __launch_bounds__(PT) void seg_profile(Seg s, TileRec* rec) {
+  const uint32_t tile = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const TileOf t = tile_of(s, tile);
+  const int64_t p = (int64_t)t.lt * PT + tid;
+  int32_t np = 0, nt = 0;
+  SM b{0, 0}, sl{0, 0};
+  bool bad = false;
+  if (p < t.m) {
+    const uint32_t id = pair_at(s, t, p);
+    if (id == NO_PAIR) {
+      bad = true;
+    } else {
+      const uint4 r0 = s.rec[4 * id];
+      const int32_t bpop = hb(r0, 0), bpush = hb(r0, 1), spop = hb(r0, 4), spush = hb(r0, 5);
+      np = hb(r0, 2);
+      nt = hb(r0, 3);
+      b = SM{bpush - bpop, -bpop};
+      sl = SM{spush - spop, -spop};
+    }
+  }
+  __shared__ int32_t wc[2][PNW];
+  __shared__ SM wb[PNW], ws[PNW];
+  __shared__ uint32_t s_bad;
+  if (tid == 0) s_bad = 0;
+  const uint32_t cp = wave_scan_add32((uint32_t)np), ct = wave_scan_add32((uint32_t)nt);
+  const SM ib = wave_scan_sm(b), is = wave_scan_sm(sl);
+  if (lane == 63) {
+    wc[0][wid] = (int32_t)cp;
+    wc[1][wid] = (int32_t)ct;
+    wb[wid] = ib;
+    ws[wid] = is;
+  }
+  __syncthreads();
+  if (bad) s_bad = 1;
+  __syncthreads();
+  if (tid == 0) {
+    TileRec r{0, 0, 0, 0, 0, 0, 0, 0};
+    SM tb{0, 0}, tsm{0, 0};
+    for (int w = 0; w < PNW; ++w) {
+      r.np += wc[0][w];
+      r.nt += wc[1][w];
+      tb = sm_cat(tb, wb[w]);
+      tsm = sm_cat(tsm, ws[w]);
+    }
+    r.bs = tb.s;
+    r.bm = tb.m;
+    r.ss = tsm.s;
+    r.sm = tsm.m;
+    rec[tile] = r;
+    if (s_bad) atomicOr(s.flags + t.seg, BAD_PAIR);
+  }
+}
+
+// ---------------------------------------------------------------- pass S
+
+struct ScanOut {
+  int64_t np, nt, bend, cend;  // totals: slots, terminal productions, bracket depth, slot depth
+  int64_t ob, ub, os, us;      // list lengths
+};
+
+// The eight running sums over tiles: slots, terminal productions, bracket
+// depth, slot depth (plus one root slot at each input's first tile), open
+// pushes / unmatched pops of both streams.
+constexpr int NV = 8;
+constexpr int SB = 1024;  // tiles per scan block
+
+__device__ __forceinline__ void tile_terms(const TileRec& r, bool first, int64_t* v) {
+  v[0] = r.np;
+  v[1] = r.nt;
+  v[2] = r.bs;
+  v[3] = r.ss + (first ? 1 : 0);
+  v[4] = r.bs - r.bm;
+  v[5] = -r.bm;
+  v[6] = r.ss - r.sm;
+  v[7] = -r.sm;
+}
+
+// Block-wide exclusive scan of NV int64 values (256 lanes); totals out.
+__device__ __forceinline__ void block_scan_nv(int64_t* v, int64_t* tot) {
+  __shared__ int64_t wsum[NV][4];
+  const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    const int64_t inc = (int64_t)wave_scan_add64((uint64_t)v[k]);
+    if (lane == 63) wsum[k][wid] = inc;
+    v[k] = inc - v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    int64_t o = 0, t = 0;
+    for (uint32_t w = 0; w < 4; ++w) {
+      if (w < wid) o += wsum[k][w];
+      t += wsum[k][w];
+    }
+    v[k] += o;
+    tot[k] = t;
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ bool is_first(const Seg& s, uint32_t t) { return s.tile_first[s.tile_seg[t]] == t; }
+
+__global__ __launch_bounds__(256) void tile_block_sums(Seg s, const TileRec* rec, int64_t* bsum) {
+  int64_t v[NV] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const uint32_t t0 = blockIdx.x * SB + threadIdx.x * (SB / 256);
+  for (int i = 0; i < SB / 256; ++i) {
+    if (t0 + i >= s.ntiles) break;
+    int64_t x[NV];
+    tile_terms(rec[t0 + i], is_first(s, t0 + i), x);
+    for (int k = 0; k < NV; ++k) v[k] += x[k];
+  }
+  int64_t tot[NV];
+  block_scan_nv(v, tot);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < NV; ++k) bsum[(uint64_t)blockIdx.x * NV + k] = tot[k];
+}
+
+__global__ __launch_bounds__(256) void tile_block_scan(int64_t* bsum, uint32_t nblk, ScanOut* out) {
+  __shared__ int64_t carry[NV];
+  if (threadIdx.x < NV) carry[threadIdx.x] = 0;
+  __syncthreads();
+  for (uint32_t b0 = 0; b0 < nblk; b0 += 256) {
+    const uint32_t b = b0 + threadIdx.x;
+    int64_t v[NV];
+    for (int k = 0; k < NV; ++k) v[k] = b < nblk ? bsum[(uint64_t)b * NV + k] : 0;
+    int64_t tot[NV];
+    block_scan_nv(v, tot);
+    if (b < nblk)
+      for (int k = 0; k < NV; ++k) bsum[(uint64_t)b * NV + k] = carry[k] + v[k];
+    __syncthreads();
+    if (threadIdx.x < NV) carry[threadIdx.x] += tot[threadIdx.x];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = ScanOut{carry[0], carry[1], carry[2], carry[3], carry[4], carry[5], carry[6], carry[7]};
+}
+
+__global__ __launch_bounds__(256) void tile_downsweep(Seg s, const TileRec* rec, const int64_t* bsum, TileInfo* info,
+                                                      SegBase* sb, int32_t* mb, int32_t* ms) {
+  int64_t v[NV] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const uint32_t t0 = blockIdx.x * SB + threadIdx.x * (SB / 256);
+  for (int i = 0; i < SB / 256; ++i) {
+    if (t0 + i >= s.ntiles) break;
+    int64_t x[NV];
+    tile_terms(rec[t0 + i], is_first(s, t0 + i), x);
+    for (int k = 0; k < NV; ++k) v[k] += x[k];
+  }
+  int64_t tot[NV];
+  block_scan_nv(v, tot);
+  for (int k = 0; k < NV; ++k) v[k] += bsum[(uint64_t)blockIdx.x * NV + k];
+  for (int i = 0; i < SB / 256; ++i) {
+    const uint32_t t = t0 + i;
+    if (t >= s.ntiles) break;
+    const TileRec r = rec[t];
+    const bool first = is_first(s, t);
+    TileInfo ti;
+    ti.p_off = v[0];
+    ti.t_off = v[1];
+    ti.bstart = v[2];
+    ti.sstart = v[3] + (first ? 1 : 0);
+    ti.bmin = r.bm;
+    ti.smin = r.sm;
+    ti.ob_off = (int32_t)v[4];
+    ti.ub_off = (int32_t)v[5];
+    ti.os_off = (int32_t)v[6];
+    ti.us_off = (int32_t)v[7];
+    info[t] = ti;
+    mb[t] = (int32_t)(ti.bstart + r.bm);
+    ms[t] = (int32_t)(ti.sstart + r.sm);
+    if (first) sb[s.tile_seg[t]] = SegBase{ti.p_off, ti.t_off, ti.bstart, ti.sstart};
+    int64_t x[NV];
+    tile_terms(r, first, x);
+    for (int k = 0; k < NV; ++k) v[k] += x[k];
+  }
+}
+
+// ---------------------------------------------------------------- pass B
+
+struct Resolve {
+  Seg s;
+  const TileInfo* info;
+  const SegBase* sb;
+  int mbpush, mspush;  // rows of the LDS push lists (grammar maxima)
+  int ncap;            // staged slots per tile: PT * max productions per window
+  uint16_t* ob_list;   // open pushes (bracket symbols)
+  uint16_t* ub_list;   // unmatched pops (bracket symbols)
+  int64_t* os_list;    // open child slots (owner's compact index, all inputs)
+  int64_t* us_list;    // unmatched slots (output target)
+  uint8_t* tree_prod;
+  int64_t* tree_par;
+  int64_t* tok_par;
+  uint8_t* deriv;  // flat derivation (parser-only mode) or null
+  int tree;
+};
+
+// Last position l <= from whose post-pop depth (tb[0][l]) is below L, or a
+// negative value: tb[k][l] is the minimum over [l - 2^k + 1, l] (clipped at
+// 0) and maximal blocks with minimum >= L are skipped, largest first.
+__device__ __forceinline__ int prev_below(const int16_t (*tb)[PT], int from, int32_t L) {
+  int l = from;
+  if (l < 0 || tb[0][l] < L) return l;
+#pragma unroll
+  for (int k = PLV - 1; k >= 0; --k)
+    if (l >= 0 && tb[k][l] >= L) l -= 1 << k;
+  return l;
+}
+
+// Block-wide exclusive scans of four int32 values.
+__device__ __forceinline__ void block_scan4(int32_t* v, int32_t* tot, int32_t (*ws)[PNW]) {
+  const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int32_t inc[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    inc[k] = (int32_t)wave_scan_add32((uint32_t)v[k]);
+    if (lane == 63) ws[k][wid] = inc[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    int32_t o = 0, t = 0;
+    for (uint32_t w = 0; w < (uint32_t)PNW; ++w) {
+      if (w < wid) o += ws[k][w];
+      t += ws[k][w];
+    }
+    v[k] = o + inc[k] - v[k];
+    tot[k] = t;
+  }
+}
+
+// Exclusive minimum over the later lanes of the workgroup.
+__device__ __forceinline__ int32_t suffix_min_after(int32_t v, int32_t* wtot) {
+  const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int32_t inc = v;
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) {
+    const int32_t o = __shfl_down(inc, d, WAVE);
+    if (lane + d < (uint32_t)WAVE) inc = o < inc ? o : inc;
+  }
+  if (lane == 0) wtot[wid] = inc;
+  __syncthreads();
+  int32_t ex = __shfl_down(inc, 1, WAVE);
+  if (lane == WAVE - 1) ex = 0x7FFFFFFF;
+  for (uint32_t w = wid + 1; w < (uint32_t)PNW; ++w) ex = wtot[w] < ex ? wtot[w] : ex;
+  return ex;
+}
+
+__global__ __launch_bounds__(PT) void seg_resolve(Resolve r) {
+  // dynamic LDS: bracket push symbols [mbpush][PT], child-slot owners
+  // [mspush][PT] (tile-local compact index), staged parents of nodes [ncap]
+  // and of terminal productions [ncap] (tile-local compact index), staged
+  // productions [ncap]
+  extern __shared__ __align__(16) uint16_t dyn[];
+  __shared__ int16_t tbb[PLV][PT], tbs[PLV][PT];
+  __shared__ int32_t ws[4][PNW];
+  __shared__ int32_t wmin[2][PNW];
+  const Seg& s = r.s;
+  const uint32_t tile = blockIdx.x, tid = threadIdx.x;
+  uint16_t* const bpl = dyn;
+  uint16_t* const spl = bpl + r.mbpush * PT;
+  uint16_t* const s_par = spl + r.mspush * PT;
+  uint16_t* const s_tpar = s_par + r.ncap;
+  uint8_t* const s_prod = reinterpret_cast<uint8_t*>(s_tpar + r.ncap);
+  const TileOf t = tile_of(s, tile);
+  const TileInfo ti = r.info[tile];
+  const SegBase in = r.sb[t.seg];
+  const int64_t p = (int64_t)t.lt * PT + tid;
+
+  uint4 r0 = make_uint4(0, 0, 0, 0), r1 = r0, r2 = r0, r3 = r0;
+  if (p < t.m) {
+    const uint32_t id = pair_at(s, t, p);  // every position has a pair: pass A rejected the input otherwise
+    if (id != NO_PAIR) {
+      r0 = s.rec[4 * id];
+      r1 = s.rec[4 * id + 1];
+      r2 = s.rec[4 * id + 2];
+      r3 = s.rec[4 * id + 3];
+    }
+  }
+  const int32_t nbpop = hb(r0, 0), nbpush = hb(r0, 1), np = hb(r0, 2), nt = hb(r0, 3);
+  const int32_t nspop = hb(r0, 4), nspush = hb(r0, 5);
+  const uint32_t tmask = hb(r0, 6);
+
+  // exclusive scans: slots, terminal productions, bracket depth, slot depth
+  int32_t v[4] = {np, nt, nbpush - nbpop, nspush - nspop}, tot[4];
+  block_scan4(v, tot, ws);
+  const int32_t slot0 = v[0], term0 = v[1], bD = v[2], sD = v[3];
+  const int32_t tile_np = tot[0], tile_nt = tot[1];
+  const int32_t cbase = slot0 - term0;  // tile-local compact index of the position's first node
+  const int32_t m_b = bD - nbpop, m_s = sD - nspop;
+  tbb[0][tid] = (int16_t)m_b;
+  tbs[0][tid] = (int16_t)m_s;
+  for (int e = 0; e < nbpush; ++e) bpl[e * PT + tid] = (uint16_t)rec_br(r0, r1, nbpop + e);
+  for (int e = 0; e < nspush; ++e) spl[e * PT + tid] = (uint16_t)(cbase + rec_owner(r2, r3, e));
+  const int32_t bsuf = suffix_min_after(m_b, wmin[0]);
+  const int32_t ssuf = suffix_min_after(m_s, wmin[1]);
+  // level 0 and the push lists are visible: suffix_min_after synchronised
+#pragma unroll
+  for (int k = 1; k < PLV; ++k) {
+    const int h = 1 << (k - 1);
+    int16_t a = tbb[k - 1][tid], c = tbs[k - 1][tid];
+    if ((int)tid >= h) {
+      const int16_t a2 = tbb[k - 1][tid - h], c2 = tbs[k - 1][tid - h];
+      a = a2 < a ? a2 : a;
+      c = c2 < c ? c2 : c;
+    }
+    tbb[k][tid] = a;
+    tbs[k][tid] = c;
+    __syncthreads();
+  }
+
+  bool bad = false;
+  const int64_t c_off = ti.p_off - ti.t_off;  // compact index of the tile's first node (all inputs)
+  const int64_t c_in = in.p_off - in.t_off;   // ... of the input's first node
+  // brackets: pops (depth falling, so the matching position only moves back)
+  int l = (int)tid - 1;
+  for (int e = 0; e < nbpop; ++e) {
+    const int32_t L = bD - e;
+    const uint32_t sym = rec_br(r0, r1, e);
+    l = prev_below(tbb, l, L);
+    if (l >= 0)
+      bad |= bpl[(L - tbb[0][l] - 1) * PT + l] != sym;
+    else
+      r.ub_list[ti.ub_off + (-L)] = (uint16_t)sym;
+  }
+  for (int e = 0; e < nbpush; ++e) {
+    const int32_t lv = m_b + 1 + e;
+    if (lv <= bsuf) r.ob_list[ti.ob_off + (lv - ti.bmin - 1)] = (uint16_t)rec_br(r0, r1, nbpop + e);
+  }
+  // productions and their parents
+  l = (int)tid - 1;
+  int32_t tseen = 0;
+  for (int j = 0; j < np; ++j) {
+    const uint32_t pr = rec_prod(r1, j);
+    const bool term = (tmask >> j) & 1u;
+    const int32_t c = cbase + j - tseen;   // tile-local compact index (nodes)
+    const int32_t tl = term0 + tseen;      // tile-local terminal production
+    tseen += term;
+    if (!r.tree) {
+      s_prod[slot0 + j] = (uint8_t)pr;
+      continue;
+    }
+    if (!term) s_prod[c] = (uint8_t)pr;
+    const uint32_t code = rec_parent(r2, j);
+    int32_t par;
+    if (code & 0x80u) {
+      const int32_t L = sD - (int32_t)(code & 0x7Fu);
+      l = prev_below(tbs, l, L);
+      if (l < 0) {
+        r.us_list[ti.us_off + (-L)] = term ? (TOKEN_TARGET | (ti.t_off + tl)) : (c_off + c);
+        continue;
+      }
+      par = spl[(L - tbs[0][l] - 1) * PT + l];
+    } else {
+      par = cbase + (int32_t)code;
+    }
+    if (term)
+      s_tpar[tl] = (uint16_t)par;
+    else
+      s_par[c] = (uint16_t)par;
+  }
+  if (r.tree) {
+    for (int e = 0; e < nspush; ++e) {
+      const int32_t lv = m_s + 1 + e;
+      if (lv <= ssuf) r.os_list[ti.os_off + (lv - ti.smin - 1)] = c_off + spl[e * PT + tid];
+    }
+  }
+  if (bad) atomicOr(s.flags + t.seg, BAD_MATCH);
+  if (tid == 0 && ti.bstart + ti.bmin < in.bstart) atomicOr(s.flags + t.seg, BAD_BDEPTH);
+  if (tid == 0 && ti.sstart + ti.smin < in.sstart - 1) atomicOr(s.flags + t.seg, BAD_SDEPTH);
+
+  // coalesced write-out; entries whose parent lies before the tile are
+  // written again by far_resolve, which runs after this kernel
+  __syncthreads();
+  if (r.tree) {
+    const int32_t nn = tile_np - tile_nt;
+    for (int32_t i = tid; i < nn; i += PT) {
+      r.tree_prod[c_off + i] = s_prod[i];
+      r.tree_par[c_off + i] = c_off + s_par[i] - c_in;
+    }
+    for (int32_t i = tid; i < tile_nt; i += PT) r.tok_par[ti.t_off + i] = c_off + s_tpar[i] - c_in;
+  } else {
+    for (int32_t i = tid; i < tile_np; i += PT) r.deriv[ti.p_off + i] = s_prod[i];
+  }
+}
+
+// ---------------------------------------------------------------- pass C
+
+constexpr int MT_LEVELS = 8;
+struct TileTree {
+  const int32_t* lv[MT_LEVELS];
+  uint64_t n[MT_LEVELS];
+  int levels;
+};
+
+// last tile t' < t with min < v, or -1
+__device__ int64_t tile_prev_less(const TileTree& t, uint64_t i, int32_t v) {
+  uint64_t idx = i;
+  for (int L = 0; L < t.levels; ++L) {
+    const int32_t* a = t.lv[L];
+    const uint64_t lo = idx & ~31ull;
+    for (uint64_t j = idx; j > lo;) {
+      --j;
+      if (a[j] < v) {
+        uint64_t k = j;
+        for (int D = L; D > 0; --D) {
+          const int32_t* b = t.lv[D - 1];
+          const uint64_t c0 = k * 32, c1 = c0 + 32 < t.n[D - 1] ? c0 + 32 : t.n[D - 1];
+          uint64_t c = c1;
+          while (c > c0) {
+            --c;
+            if (b[c] < v) break;
+          }
+          k = c;
+        }
+        return (int64_t)k;
+      }
+    }
+    idx >>= 5;
+  }
+  return -1;
+}
+
+struct Far {
+  Seg s;
+  const TileInfo* info;
+  const SegBase* sb;
+  TileTree tb, ts;
+  const uint16_t* ob_list;
+  const uint16_t* ub_list;
+  const int64_t* os_list;
+  const int64_t* us_list;
+  int64_t* tree_par;
+  int64_t* tok_par;
+  int tree;
+};
+
+__global__ __launch_bounds__(256) void far_resolve(Far f) {
+  const uint32_t tile = blockIdx.x;
+  const TileInfo ti = f.info[tile];
+  const uint32_t seg = f.s.tile_seg[tile];
+  const int64_t first = f.s.tile_first[seg];
+  const SegBase in = f.sb[seg];
+  const int64_t c_in = in.p_off - in.t_off;
+  bool bad = false, bad_root = false;
+  for (int u = threadIdx.x; u < -ti.bmin; u += 256) {
+    const int32_t D = (int32_t)ti.bstart - u;  // depth before the pop
+    const uint16_t b = f.ub_list[ti.ub_off + u];
+    const int64_t t2 = tile_prev_less(f.tb, tile, D);
+    if (t2 < first) {
+      bad = true;
+      continue;
+    }
+    const TileInfo& o = f.info[t2];
+    const uint16_t push = f.ob_list[o.ob_off + (D - (int32_t)o.bstart - o.bmin - 1)];
+    bad |= push != b;
+  }
+  for (int u = threadIdx.x; f.tree && u < -ti.smin; u += 256) {
+    const int32_t D = (int32_t)ti.sstart - u;  // open child slots before the pop
+    const int64_t target = f.us_list[ti.us_off + u];
+    const int64_t t2 = tile_prev_less(f.ts, tile, D);
+    int64_t pc;
+    if (t2 < first) {
+      if (D != (int32_t)in.sstart) bad_root = true;
+      pc = 0;  // the root is its own parent
+    } else {
+      const TileInfo& o = f.info[t2];
+      pc = f.os_list[o.os_off + (D - (int32_t)o.sstart - o.smin - 1)] - c_in;
+    }
+    if (target & TOKEN_TARGET)
+      f.tok_par[target & ~TOKEN_TARGET] = pc;
+    else
+      f.tree_par[target] = pc;
+  }
+  if (bad) atomicOr(f.s.flags + seg, BAD_MATCH);
+  if (bad_root) atomicOr(f.s.flags + seg, BAD_ROOT);
+}
+
+__global__ __launch_bounds__(256) void tiles_min_level(const int32_t* in, uint64_t n_in, int32_t* out, uint64_t n_out) {
+  const uint64_t c = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= n_out) return;
+  const uint64_t lo = c * 32, hi = lo + 32 < n_in ? lo + 32 : n_in;
+  int32_t m = in[lo];
+  for (uint64_t i = lo + 1; i < hi; ++i) m = in[i] < m ? in[i] : m;
+  out[c] = m;
+}
+
+int build_tile_tree(const int32_t* lv0, uint64_t n, int32_t* scratch, hipStream_t st, TileTree* t) {
+  t->levels = 1;
+  t->lv[0] = lv0;
+  t->n[0] = n;
+  int32_t* p = scratch;
+  uint64_t m = n;
+  while (m > 32) {
+    if (t->levels == MT_LEVELS) return fail(ALPACC_ERR_LIMIT, "tile tree too deep");
+    const uint64_t nn = (m + 31) / 32;
+    hipLaunchKernelGGL(tiles_min_level, dim3((uint32_t)((nn + 255) / 256)), dim3(256), 0, st, t->lv[t->levels - 1], m,
+                       p, nn);
+    t->lv[t->levels] = p;
+    t->n[t->levels] = nn;
+    t->levels++;
+    p += nn;
+    m = nn;
+  }
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+uint64_t tree_scratch(uint64_t n) {
+  uint64_t tot = 0;
+  while (n > 32) {
+    n = (n + 31) / 32;
+    tot += n;
+  }
+  return tot + 1;
+}
+
+// ---------------------------------------------------------------- pass F
+
+__global__ void seg_finish(Seg s, const TileInfo* info, const SegBase* sb, const ScanOut* so, int tree,
+                           ParseResult* res) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= s.count) return;
+  const uint32_t f0 = s.tile_first[i], f1 = s.tile_first[i + 1];
+  ParseResult o{};
+  if (f0 == f1) {  // not parsed
+    o.flags = BAD_PAIR;
+    res[i] = o;
+    return;
+  }
+  const SegBase in = sb[i];
+  int64_t p_end, t_end, b_end, c_end;
+  if (f1 < s.ntiles) {
+    const TileInfo& e = info[f1];
+    p_end = e.p_off;
+    t_end = e.t_off;
+    b_end = e.bstart;
+    c_end = e.sstart - 1;  // the next tile starts an input: drop its root slot
+  } else {
+    p_end = so->np;
+    t_end = so->nt;
+    b_end = so->bend;
+    c_end = so->cend;
+  }
+  uint32_t fl = s.flags[i];
+  if (b_end != in.bstart) fl |= BAD_BDEPTH;
+  if (c_end != in.sstart - 1) fl |= BAD_SDEPTH;
+  const int64_t ntok = (int64_t)(s.tok_off[i + 1] - s.tok_off[i]);
+  if (t_end - in.t_off != ntok) fl |= BAD_COUNT;
+  o.flags = tree ? fl : (fl & PARSE_TEST_CHECKS);
+  o.slot_off = (uint64_t)in.p_off;
+  o.slots = (uint64_t)(p_end - in.p_off);
+  o.node_off = (uint64_t)(in.p_off - in.t_off);
+  o.nodes = (uint64_t)((p_end - in.p_off) - (t_end - in.t_off));
+  o.term_off = (uint64_t)in.t_off;
+  res[i] = o;
+}
+
+}  // namespace
+
+int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_tok_off, const uint64_t* h_tok_off,
+                          const uint32_t* h_skip, uint64_t count, bool tree, hipStream_t st) {
+  const GrammarDesc* g = c.g;
+  if (count == 0 || count > 0xFFFFFFFEull) return fail(ALPACC_ERR_ARG, "batch of %llu inputs", (unsigned long long)count);
+  const int ncap = PT * g->max_prods;
+  const size_t lds_b = (size_t)(g->max_bpush + g->max_spush) * PT * 2 + (size_t)ncap * 5;
+  // tile-local depths are int16 and tile-local compact indices u16
+  if ((int64_t)PT * 8 > 0x7FFF || ncap > 0xFFFF || lds_b > 96 * 1024)
+    return fail(ALPACC_ERR_LIMIT, "grammar windows too large for the parser tiles");
+  // tiles per input: framed positions = tokens + 2; inputs skipped get none
+  uint64_t ntiles = 0, positions = 0;
+  for (uint64_t i = 0; i < count; ++i) {
+    if (h_skip && h_skip[i]) continue;
+    const uint64_t m = h_tok_off[i + 1] - h_tok_off[i] + 2;
+    ntiles += (m + PT - 1) / PT;
+    positions += m;
+  }
+  if (ntiles > 0x7FFFFFFFull || positions * 8 > 0x7FFFFFFFull)
+    return fail(ALPACC_ERR_LIMIT, "batch of %llu framed positions exceeds the parser's 31-bit depths",
+                (unsigned long long)positions);
+  int r;
+  // host metadata: tile_first u32[count + 1]
+  if ((r = c.h_pmeta.reserve((count + 1) * 4 + 64))) return r;
+  uint32_t* h_tf = c.h_pmeta.as<uint32_t>();
+  uint32_t tt = 0;
+  for (uint64_t i = 0; i < count; ++i) {
+    h_tf[i] = tt;
+    if (h_skip && h_skip[i]) continue;
+    tt += (uint32_t)((h_tok_off[i + 1] - h_tok_off[i] + 2 + PT - 1) / PT);
+  }
+  h_tf[count] = tt;
+
+  // device workspace: tile_first | tile_seg | TileRec | TileInfo | SegBase |
+  // mb | ms | trees | block sums | ScanOut | ParseResult | flags
+  const uint64_t tts = tree_scratch(ntiles ? ntiles : 1);
+  const uint32_t nsb = (uint32_t)((ntiles + SB - 1) / SB);
+  auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+  const size_t off_tf = 0;
+  const size_t off_ts = al(off_tf + (count + 1) * 4);
+  const size_t off_rec = al(off_ts + ntiles * 4);
+  const size_t off_info = al(off_rec + ntiles * sizeof(TileRec));
+  const size_t off_sb = al(off_info + ntiles * sizeof(TileInfo));
+  const size_t off_mb = al(off_sb + count * sizeof(SegBase));
+  const size_t off_ms = al(off_mb + ntiles * 4);
+  const size_t off_tb = al(off_ms + ntiles * 4);
+  const size_t off_tsx = al(off_tb + tts * 4);
+  const size_t off_bs = al(off_tsx + tts * 4);
+  const size_t off_out = al(off_bs + (size_t)(nsb ? nsb : 1) * NV * 8);
+  const size_t off_res = al(off_out + sizeof(ScanOut));
+  const size_t off_fl = al(off_res + count * sizeof(ParseResult));
+  const size_t total = off_fl + count * 4 + 64;
+  if ((r = c.ps_a.reserve(total)) || (r = c.h_pres.reserve(count * sizeof(ParseResult) + sizeof(ScanOut) + 64)))
+    return r;
+  uint8_t* base = c.ps_a.as<uint8_t>();
+  Seg s{};
+  s.tok = d_tok;
+  s.tok_off = d_tok_off;
+  s.tile_first = reinterpret_cast<const uint32_t*>(base + off_tf);
+  s.tile_seg = reinterpret_cast<const uint32_t*>(base + off_ts);
+  s.count = (uint32_t)count;
+  s.ntiles = (uint32_t)ntiles;
+  s.q = g->q;
+  s.k = g->k;
+  s.base = g->window_base;
+  s.empty_t = g->empty_t;
+  s.start_t = g->start_t;
+  s.end_t = g->end_t;
+  s.window = c.d_window;
+  s.rec = reinterpret_cast<const uint4*>(c.d_pair_rec);
+  s.flags = reinterpret_cast<uint32_t*>(base + off_fl);
+  TileRec* rec = reinterpret_cast<TileRec*>(base + off_rec);
+  TileInfo* info = reinterpret_cast<TileInfo*>(base + off_info);
+  SegBase* sb = reinterpret_cast<SegBase*>(base + off_sb);
+  int32_t* mb = reinterpret_cast<int32_t*>(base + off_mb);
+  int32_t* ms = reinterpret_cast<int32_t*>(base + off_ms);
+  int64_t* bsum = reinterpret_cast<int64_t*>(base + off_bs);
+  ScanOut* sout = reinterpret_cast<ScanOut*>(base + off_out);
+  ParseResult* res = reinterpret_cast<ParseResult*>(base + off_res);
+
+  HIP_TRY(hipMemcpyAsync(base + off_tf, h_tf, (count + 1) * 4, hipMemcpyHostToDevice, st));
+  if ((r = zero_words(s.flags, count * 4, st)) || (r = zero_words(sout, sizeof(ScanOut), st))) return r;
+  KernelTimer& tm = c.timer;
+  if (c.timing) HIP_TRY(hipEventRecord(c.ev[2], st));
+  if (ntiles) {
+    hipLaunchKernelGGL(tile_map, dim3((uint32_t)((ntiles + 255) / 256)), dim3(256), 0, st, s.tile_first, s.count,
+                       s.ntiles, const_cast<uint32_t*>(s.tile_seg));
+    HIP_TRY(tm.begin(st, "pair_profile"));
+    hipLaunchKernelGGL(seg_profile, dim3((uint32_t)ntiles), dim3(PT), 0, st, s, rec);
+    HIP_TRY(tm.end(st));
+    HIP_TRY(tm.begin(st, "tile_scan"));
+    hipLaunchKernelGGL(tile_block_sums, dim3(nsb), dim3(256), 0, st, s, rec, bsum);
+    hipLaunchKernelGGL(tile_block_scan, dim3(1), dim3(256), 0, st, bsum, nsb, sout);
+    hipLaunchKernelGGL(tile_downsweep, dim3(nsb), dim3(256), 0, st, s, rec, bsum, info, sb, mb, ms);
+    HIP_TRY(tm.end(st));
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(c.h_pres.ptr, sout, sizeof(ScanOut), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  HIP_TRY(tm.collect());
+  const ScanOut so = *c.h_pres.as<ScanOut>();
+  const uint64_t np = (uint64_t)so.np, nterm = (uint64_t)so.nt;
+  if (so.np > 0x7FFFFFFF) return fail(ALPACC_ERR_LIMIT, "derivation beyond 2^31 entries");
+  const uint64_t nodes = np - nterm;
+  if ((r = c.ps_b.reserve((size_t)(so.ob + so.ub) * 2 + 64)) || (r = c.ps_c.reserve((size_t)(so.os + so.us) * 8 + 64)) ||
+      (tree && ((r = c.tr_prod.reserve(nodes + 16)) || (r = c.tr_parent.reserve(nodes * 8 + 16)) ||
+                (r = c.tr_tokpar.reserve(nterm * 8 + 16)))) ||
+      (!tree && (r = c.ps_d.reserve(np + 16))))
+    return r;
+  uint16_t* ob_list = c.ps_b.as<uint16_t>();
+  uint16_t* ub_list = ob_list + so.ob;
+  int64_t* os_list = c.ps_c.as<int64_t>();
+  int64_t* us_list = os_list + so.os;
+  if (ntiles) {
+    TileTree tb{}, ts{};
+    HIP_TRY(tm.begin(st, "tile_tree"));
+    if ((r = build_tile_tree(mb, ntiles, reinterpret_cast<int32_t*>(base + off_tb), st, &tb)) ||
+        (r = build_tile_tree(ms, ntiles, reinterpret_cast<int32_t*>(base + off_tsx), st, &ts)))
+      return r;
+    HIP_TRY(tm.end(st));
+    Resolve rs{s,         info,      sb,       g->max_bpush,  g->max_spush, ncap,
+               ob_list,   ub_list,   os_list,  us_list,       c.tr_prod.as<uint8_t>(),
+               c.tr_parent.as<int64_t>(), c.tr_tokpar.as<int64_t>(), tree ? nullptr : c.ps_d.as<uint8_t>(),
+               tree ? 1 : 0};
+    if (!c.resolve_lds_set || c.resolve_lds_set < lds_b) {
+      HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(seg_resolve), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds_b));
+      c.resolve_lds_set = lds_b;
+    }
+    HIP_TRY(tm.begin(st, "tile_resolve"));
+    hipLaunchKernelGGL(seg_resolve, dim3((uint32_t)ntiles), dim3(PT), lds_b, st, rs);
+    HIP_TRY(tm.end(st));
+    Far fr{s, info, sb, tb, ts, ob_list, ub_list, os_list, us_list, c.tr_parent.as<int64_t>(),
+           c.tr_tokpar.as<int64_t>(), tree ? 1 : 0};
+    HIP_TRY(tm.begin(st, "far_resolve"));
+    hipLaunchKernelGGL(far_resolve, dim3((uint32_t)ntiles), dim3(256), 0, st, fr);
+    HIP_TRY(tm.end(st));
+  }
+  hipLaunchKernelGGL(seg_finish, dim3((uint32_t)((count + 255) / 256)), dim3(256), 0, st, s, info, sb, sout,
+                     tree ? 1 : 0, res);
+  HIP_TRY(hipGetLastError());
+  if (c.timing) HIP_TRY(hipEventRecord(c.ev[3], st));
+  HIP_TRY(hipMemcpyAsync(c.h_pres.ptr, res, count * sizeof(ParseResult), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  HIP_TRY(tm.collect());
+  if (c.timing) {
+    float ms_ = 0;
+    HIP_TRY(hipEventElapsedTime(&ms_, c.ev[2], c.ev[3]));
+    c.parse_ms += ms_;
+    c.parse_launches++;
+  }
+  c.pb_count = count;
+  c.pb_res = c.h_pres.as<ParseResult>();
+  return 0;
+}
+
+int parse_on_device(Context& c, const uint8_t* d_tok, uint64_t n, bool tree, hipStream_t st, uint64_t* num_prods) {
+  *num_prods = 0;
+  int r;
+  if ((r = c.pmeta.reserve(64))) return r;
+  uint64_t* h_off = c.host_small + 8;  // pinned
+  h_off[0] = 0;
+  h_off[1] = n;
+  uint64_t* d_off = c.pmeta.as<uint64_t>();
+  HIP_TRY(hipMemcpyAsync(d_off, h_off, 16, hipMemcpyHostToDevice, st));
+  r = parse_batch_on_device(c, d_tok, d_off, h_off, nullptr, 1, tree, st);
+  if (r < 0) return r;
+  const ParseResult& o = c.pb_res[0];
+  if (o.flags) return ALPACC_REJECT;
+  *num_prods = o.slots;
+  return ALPACC_VALID;
+}
+
+}  // namespace alpacc

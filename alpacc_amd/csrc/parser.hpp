@@ -1,4 +1,4 @@
-// LLP parser pipeline on the device (parser.hip).
+// LLP parser pipeline on the device (parse_seg.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -7,11 +7,29 @@
 
 namespace alpacc {
 
-// Parse n terminal ids (u8, device) with the combined-mode numbering.
-// Returns 1 (valid) / 0 (rejected) / < 0 (error) and the derivation length in
-// *num_prods.  With `tree` the parents and the compact productions-only tree
-// land in c.tr_* (num_nodes = *num_prods - n); without it the flat
-// derivation (u8 productions) lands in c.ps_d.
+// Per-input result of a parser launch.  flags == 0: valid.  Slots are the
+// full derivation (every production, flat order); nodes are the compact
+// tree's non-terminal productions; terminal productions ("term") index the
+// token parents.  Offsets are into the launch's output arrays.
+struct ParseResult {
+  uint32_t flags, pad;
+  uint64_t slot_off, slots;
+  uint64_t node_off, nodes;
+  uint64_t term_off;
+};
+
+// Parse `count` strings of terminal ids (combined numbering, u8) packed in
+// device memory at d_tok, string i being ids [tok_off[i], tok_off[i + 1])
+// (d_tok_off on the device, h_tok_off its host copy).  Strings with
+// h_skip[i] != 0 are not parsed (h_skip may be null).  With `tree` the
+// compact trees land in c.tr_* (parents relative to their input), else the
+// flat derivations in c.ps_d.  Returns 0 or an error; per-input results in
+// c.pb_res[count].
+int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_tok_off, const uint64_t* h_tok_off,
+                          const uint32_t* h_skip, uint64_t count, bool tree, hipStream_t st);
+
+// One string of n ids: returns 1 (valid) / 0 (rejected) / < 0 (error) and
+// the derivation length in *num_prods (trees: nodes = *num_prods - n).
 int parse_on_device(Context& c, const uint8_t* d_tok, uint64_t n, bool tree, hipStream_t st, uint64_t* num_prods);
 
 }  // namespace alpacc

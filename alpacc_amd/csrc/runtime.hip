@@ -56,6 +56,28 @@ void DevBuf::release() {
   cap = 0;
 }
 
+int PinnedBuf::reserve(size_t want) {
+  if (want <= cap) return 0;
+  if (ptr) (void)hipHostFree(ptr);
+  ptr = nullptr;
+  cap = 0;
+  size_t sz = want < 4096 ? 4096 : want;
+  sz = (sz + 4095) & ~size_t(4095);
+  hipError_t e = hipHostMalloc(&ptr, sz, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    ptr = nullptr;
+    return fail(ALPACC_ERR_NOMEM, "hipHostMalloc(%zu): %s", sz, hipGetErrorString(e));
+  }
+  cap = sz;
+  return 0;
+}
+
+void PinnedBuf::release() {
+  if (ptr) (void)hipHostFree(ptr);
+  ptr = nullptr;
+  cap = 0;
+}
+
 template <typename T>
 static int upload(T** dst, const T* src, size_t count) {
   size_t bytes = (count ? count : 1) * sizeof(T);
@@ -119,7 +141,7 @@ int alpacc_hip_open(const char* grammar, int device, alpacc_hip_ctx** out) {
       (r = upload(&c.d_window, g->window, (size_t)g->nwindow)) ||
       (r = upload(&c.d_spans, g->spans, (size_t)g->npairs * 4)) ||
       (r = upload(&c.d_profile, g->profile, (size_t)g->npairs * 8)) ||
-      (r = upload(&c.d_pair_rec, g->pair_rec, (size_t)g->npairs * 32)) ||
+      (r = upload(&c.d_pair_rec, g->pair_rec, (size_t)g->npairs * 64)) ||
       (r = upload(&c.d_prod_info, g->prod_info, (size_t)g->nproductions)) ||
       (r = upload(&c.d_stacks, g->stacks, (size_t)g->nstacks)) ||
       (r = upload(&c.d_productions, g->productions, (size_t)g->nprods_flat)) ||
@@ -150,6 +172,10 @@ void alpacc_hip_close(alpacc_hip_ctx* h) {
   if (c.host_small) (void)hipHostFree(c.host_small);
   for (hipEvent_t ev : c.ev)
     if (ev) (void)hipEventDestroy(ev);
+  c.timer.destroy();
+  c.h_meta.release();
+  c.h_pmeta.release();
+  c.h_pres.release();
   if (c.stream) (void)hipStreamDestroy(c.stream);
   delete h;
 }
@@ -208,6 +234,8 @@ int alpacc_hip_set_timing(alpacc_hip_ctx* h, int on) {
   c.timing = on != 0;
   c.lex_ms = c.parse_ms = 0;
   c.lex_launches = c.parse_launches = 0;
+  c.timer.on = on != 0;
+  c.timer.reset();
   return 0;
 }
 
@@ -220,12 +248,88 @@ int alpacc_hip_timing(const alpacc_hip_ctx* h, const char* stage, double* total_
   } else if (!strcmp(stage, "parse")) {
     *total_ms = c.parse_ms;
     *runs = c.parse_launches;
-  } else {
-    return fail(ALPACC_ERR_ARG, "unknown stage '%s'", stage);
+  } else if (!c.timer.get(stage, total_ms, runs)) {
+    return fail(ALPACC_ERR_ARG, "unknown stage or kernel '%s'", stage);
   }
   return 0;
+}
+
+const char* alpacc_hip_timed_kernels(const alpacc_hip_ctx* h) {
+  static thread_local std::string out;
+  out.clear();
+  if (!h) return "";
+  for (const std::string& n : h->c.timer.names()) {
+    if (!out.empty()) out += ',';
+    out += n;
+  }
+  return out.c_str();
 }
 
 const char* const* alpacc_hip_grammars(void) { return grammar_names(); }
 
 }  // extern "C"
+
+namespace alpacc {
+
+hipError_t KernelTimer::begin(hipStream_t st, const char* name) {
+  if (!on) return hipSuccess;
+  const size_t i = pend_.size();
+  while (ev_.size() < 2 * (i + 1)) {
+    hipEvent_t e;
+    hipError_t r = hipEventCreate(&e);
+    if (r != hipSuccess) return r;
+    ev_.push_back(e);
+  }
+  pend_.emplace_back(name);
+  ended_.push_back(false);
+  return hipEventRecord(ev_[2 * i], st);
+}
+
+hipError_t KernelTimer::end(hipStream_t st) {
+  if (!on || pend_.empty()) return hipSuccess;
+  ended_.back() = true;
+  return hipEventRecord(ev_[2 * (pend_.size() - 1) + 1], st);
+}
+
+hipError_t KernelTimer::collect() {
+  for (size_t i = 0; i < pend_.size(); ++i) {
+    float ms = 0;
+    if (!ended_[i] || hipEventElapsedTime(&ms, ev_[2 * i], ev_[2 * i + 1]) != hipSuccess) continue;
+    size_t k = 0;
+    while (k < order_.size() && order_[k] != pend_[i]) ++k;
+    if (k == order_.size()) {
+      order_.push_back(pend_[i]);
+      acc_.emplace_back();
+    }
+    acc_[k].ms += ms;
+    acc_[k].runs++;
+  }
+  pend_.clear();
+  ended_.clear();
+  return hipSuccess;
+}
+
+void KernelTimer::reset() {
+  pend_.clear();
+  ended_.clear();
+  order_.clear();
+  acc_.clear();
+}
+
+bool KernelTimer::get(const char* name, double* ms, uint64_t* runs) const {
+  for (size_t k = 0; k < order_.size(); ++k)
+    if (order_[k] == name) {
+      *ms = acc_[k].ms;
+      *runs = acc_[k].runs;
+      return true;
+    }
+  return false;
+}
+
+void KernelTimer::destroy() {
+  for (hipEvent_t e : ev_) (void)hipEventDestroy(e);
+  ev_.clear();
+  reset();
+}
+
+}  // namespace alpacc

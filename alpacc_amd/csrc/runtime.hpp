@@ -7,8 +7,11 @@
 #include <vector>
 
 #include "grammar.hpp"
+#include "timer.hpp"
 
 namespace alpacc {
+
+struct ParseResult;
 
 // Grow-only device allocation: steady-state calls allocate nothing.
 struct DevBuf {
@@ -19,6 +22,35 @@ struct DevBuf {
   template <typename T>
   T* as() const {
     return static_cast<T*>(ptr);
+  }
+};
+
+// Grow-only pinned host allocation (hipHostMalloc).
+struct PinnedBuf {
+  void* ptr = nullptr;
+  size_t cap = 0;
+  int reserve(size_t bytes);
+  void release();
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(ptr);
+  }
+};
+
+// Byte layout of a batch's metadata, the same on the device and in the
+// pinned host mirror: uploaded [seg_off u64 | tile_first u32], read back
+// [seg_tok u64 | seg_flags u32], then (device only) tile_seg u32.
+struct BatchLayout {
+  size_t off_seg, off_tf, off_tok, off_flags, off_tseg, up_bytes, down_bytes, total;
+  BatchLayout(uint64_t count, uint64_t ntiles) {
+    off_seg = 0;
+    off_tf = (count + 1) * 8;
+    up_bytes = off_tf + (count + 1) * 4;
+    off_tok = (up_bytes + 7) & ~size_t(7);
+    off_flags = off_tok + (count + 1) * 8;
+    down_bytes = off_flags + count * 4 - off_tok;
+    off_tseg = (off_flags + count * 4 + 255) & ~size_t(255);
+    total = off_tseg + ntiles * 4 + 64;
   }
 };
 
@@ -51,16 +83,34 @@ struct Context {
   DevBuf in_copy;                     // staging of host inputs
   DevBuf ps_a, ps_b, ps_c, ps_d, ps_e, ps_f, ps_g, ps_h, ps_i;  // parser temporaries
   DevBuf tr_prod, tr_parent, tr_tokpar;                         // tree outputs
+  DevBuf lx_meta;     // batch metadata of the lexer (BatchLayout)
+  PinnedBuf h_meta;   // its host mirror
+  // per-input results of the last lexer launch, in h_meta / lx_meta
+  uint64_t lb_count = 0;
+  const uint32_t* lb_flags = nullptr;  // [count] LEX_REJECT_* bits
+  const uint64_t* lb_tok = nullptr;    // [count + 1] first token of each input; [count] = all
+  const uint32_t* d_lb_flags = nullptr;
+  const uint64_t* d_lb_tok = nullptr;
+  // parser batches
+  DevBuf pmeta;       // offsets of single-string parses
+  PinnedBuf h_pmeta;  // tile_first of the parser tiles
+  PinnedBuf h_pres;   // per-input results read back
+  uint64_t pb_count = 0;
+  const struct ParseResult* pb_res = nullptr;
+  // host arrays of the last device batch (alpacc_hip_lex_parse_batch_device)
+  std::vector<uint8_t> bh_valid;
+  std::vector<uint64_t> bh_node_off, bh_nodes, bh_tp_off;
   uint64_t* host_small = nullptr;    // pinned: count / flags read-back
   // optional per-kernel timing with events on the launch stream
   bool timing = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double lex_ms = 0, parse_ms = 0;
   uint64_t lex_launches = 0, parse_launches = 0;
+  KernelTimer timer;
 
   std::vector<DevBuf*> all_buffers() {
-    return {&small, &lx_term, &lx_start, &lx_len, &lx_status, &in_copy, &ps_a, &ps_b, &ps_c,
-            &ps_d,  &ps_e,    &ps_f,     &ps_g,   &ps_h,      &ps_i,    &tr_prod, &tr_parent, &tr_tokpar};
+    return {&small, &lx_term, &lx_start, &lx_len, &lx_status, &in_copy, &ps_a,      &ps_b,      &ps_c,   &ps_d,
+            &ps_e,  &ps_f,    &ps_g,     &ps_h,   &ps_i,      &tr_prod, &tr_parent, &tr_tokpar, &lx_meta, &pmeta};
   }
 };
 
@@ -75,8 +125,19 @@ int fail(int code, const char* fmt, ...);
 int zero_words(void* p, size_t bytes, hipStream_t st);
 const char* const* grammar_names();
 
-// Lexer on device memory.  On success returns 1/0 (valid / rejected) and
-// fills count; outputs live in c.lx_*.
+// Lexer over `count` inputs packed in device memory at d_in, input i being
+// bytes [h_off[i], h_off[i + 1]) (host array).  Returns 0 or an error;
+// per-input verdicts and token offsets are in c.lb_* (host) and c.d_lb_*
+// (device), the tokens in c.lx_* with starts relative to their input.
+int lex_batch_on_device(Context& c, int mode, const uint8_t* d_in, const uint64_t* h_off, uint64_t count,
+                        hipStream_t st);
+
+// Wire records of a batch of host payloads, concatenated; rec_off (if not
+// null) receives the start of every input's record.
+int batch_records(Context& c, int mode, const uint8_t* packed, const uint64_t* off, uint64_t count,
+                  std::vector<uint8_t>& out, std::vector<size_t>* rec_off);
+
+// One input: returns 1/0 (valid / rejected) and fills count.
 int lex_on_device(Context& c, int mode, const uint8_t* d_in, uint64_t n, hipStream_t st, uint64_t* count);
 
 }  // namespace alpacc

@@ -1,0 +1,302 @@
+// Serving runtime: continuous batching over resident grammars.
+//
+// The reference serves one request at a time over the wire protocol
+// (backends/c/cli.c:167-189, docs/wire-protocols.md): every request is a
+// separate run_case.  Here every resident grammar has `lanes` contexts, each
+// with its own stream, device buffers and worker thread.  A worker that is
+// free takes every request queued for its grammar in one mode -- up to the
+// device-memory budget still unadmitted -- and answers them with one batched
+// launch sequence (batch_records).  Requests that arrive while other batches
+// are on the GPU wait only for the next free lane, and the batches of
+// different lanes overlap on the device.  Every record is the one
+// alpacc_hip_run_case returns for the request alone.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/alpacc_hip.h"
+#include "runtime.hpp"
+
+using Clock = std::chrono::steady_clock;
+
+namespace {
+
+struct Req {
+  uint32_t g = 0;
+  int mode = 0;
+  uint64_t seq = 0;  // arrival order
+  const uint8_t* data = nullptr;
+  uint64_t n = 0;
+  std::vector<uint8_t> own;  // payload copy of a submitted request
+  uint64_t est = 0;          // device bytes admitted for it
+  std::vector<uint8_t> rec;
+  int status = 0;  // 0 waiting, 1 answered, < 0 error
+  std::string err;
+  Clock::time_point done;
+};
+
+// Device bytes a request may hold while its batch runs: the input, up to one
+// token per byte (terminal, start, length, token parent: 19 bytes), the
+// tree and the parser's intermediates; a conservative 64 per input byte.
+uint64_t estimate(uint64_t n) { return n * 64 + 4096; }
+
+}  // namespace
+
+struct alpacc_hip_server {
+  std::vector<std::string> names;
+  int device = 0;
+  int lanes = 1;
+  uint64_t budget = 0, max_batch = 0;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::vector<std::deque<Req*>> q;  // [grammar * 3 + mode]
+  uint64_t in_use = 0, seq = 0;
+  bool stop = false;
+  std::vector<std::thread> workers;
+  std::vector<alpacc_hip_ctx*> ctxs;
+  std::unordered_map<uint64_t, std::unique_ptr<Req>> tickets;
+  uint64_t next_ticket = 1;
+  uint64_t n_batches = 0, n_reqs = 0, biggest = 0;
+  std::string start_error;
+};
+
+namespace {
+
+// Queue of grammar g to serve next: the one whose head arrived first among
+// those whose head fits the budget (or anything when nothing is admitted).
+int pick(alpacc_hip_server* s, uint32_t g) {
+  int best = -1;
+  for (int m = 0; m < 3; ++m) {
+    const auto& d = s->q[g * 3 + m];
+    if (d.empty()) continue;
+    if (s->in_use && s->in_use + d.front()->est > s->budget) continue;
+    if (best < 0 || d.front()->seq < s->q[g * 3 + best].front()->seq) best = m;
+  }
+  return best;
+}
+
+void worker(alpacc_hip_server* s, uint32_t g, alpacc_hip_ctx* ctx) {
+  (void)hipSetDevice(s->device);
+  std::vector<Req*> batch;
+  std::vector<uint8_t> packed, out;
+  std::vector<uint64_t> off;
+  std::vector<size_t> rec_off;
+  for (;;) {
+    int mode;
+    uint64_t est = 0;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv_work.wait(lk, [&] { return s->stop || pick(s, g) >= 0; });
+      mode = pick(s, g);
+      if (mode < 0) return;  // stopping with nothing left for this grammar
+      auto& d = s->q[g * 3 + mode];
+      batch.clear();
+      while (!d.empty() && (s->max_batch == 0 || batch.size() < s->max_batch)) {
+        Req* r = d.front();
+        if (!batch.empty() && s->in_use + est + r->est > s->budget) break;
+        batch.push_back(r);
+        est += r->est;
+        d.pop_front();
+      }
+      s->in_use += est;
+    }
+    packed.clear();
+    off.assign(1, 0);
+    for (Req* r : batch) {
+      packed.insert(packed.end(), r->data, r->data + r->n);
+      off.push_back(packed.size());
+    }
+    const int rc = alpacc::batch_records(ctx->c, mode, packed.data(), off.data(), batch.size(), out, &rec_off);
+    const std::string err = rc < 0 ? alpacc_hip_last_error() : "";
+    const Clock::time_point now = Clock::now();
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      for (size_t i = 0; i < batch.size(); ++i) {
+        Req* r = batch[i];
+        if (rc < 0) {
+          r->status = rc;
+          r->err = err;
+        } else {
+          const size_t a = rec_off[i], b = i + 1 < batch.size() ? rec_off[i + 1] : out.size();
+          r->rec.assign(out.begin() + a, out.begin() + b);
+          r->status = 1;
+        }
+        r->done = now;
+      }
+      s->in_use -= est;
+      s->n_batches++;
+      s->n_reqs += batch.size();
+      if (batch.size() > s->biggest) s->biggest = batch.size();
+    }
+    s->cv_done.notify_all();
+    s->cv_work.notify_all();
+  }
+}
+
+void enqueue(alpacc_hip_server* s, Req* r) {
+  r->est = estimate(r->n);
+  r->seq = s->seq++;
+  s->q[r->g * 3 + r->mode].push_back(r);
+}
+
+}  // namespace
+
+extern "C" {
+
+int alpacc_hip_server_open(const char* const* grammars, int ngrammars, int device, int lanes, uint64_t hbm_budget,
+                           uint64_t max_batch, alpacc_hip_server** out) {
+  if (!grammars || ngrammars <= 0 || lanes <= 0 || !out) return alpacc::fail(ALPACC_ERR_ARG, "bad server arguments");
+  *out = nullptr;
+  auto* s = new alpacc_hip_server();
+  s->device = device;
+  s->lanes = lanes;
+  s->budget = hbm_budget ? hbm_budget : (64ull << 30);
+  s->max_batch = max_batch;
+  s->q.resize((size_t)ngrammars * 3);
+  for (int g = 0; g < ngrammars; ++g) {
+    s->names.emplace_back(grammars[g]);
+    for (int l = 0; l < lanes; ++l) {
+      alpacc_hip_ctx* c = nullptr;
+      const int r = alpacc_hip_open(grammars[g], device, &c);
+      if (r < 0) {
+        for (alpacc_hip_ctx* x : s->ctxs) alpacc_hip_close(x);
+        delete s;
+        return r;
+      }
+      s->ctxs.push_back(c);
+    }
+  }
+  for (int g = 0; g < ngrammars; ++g)
+    for (int l = 0; l < lanes; ++l) s->workers.emplace_back(worker, s, (uint32_t)g, s->ctxs[(size_t)g * lanes + l]);
+  *out = s;
+  return 0;
+}
+
+void alpacc_hip_server_close(alpacc_hip_server* s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  for (std::thread& t : s->workers) t.join();
+  for (alpacc_hip_ctx* c : s->ctxs) alpacc_hip_close(c);
+  delete s;
+}
+
+int alpacc_hip_server_submit(alpacc_hip_server* s, int grammar, int mode, const uint8_t* payload, uint64_t n,
+                             uint64_t* ticket) {
+  if (!s || !ticket || (n && !payload)) return alpacc::fail(ALPACC_ERR_ARG, "null argument");
+  if (grammar < 0 || (size_t)grammar >= s->names.size()) return alpacc::fail(ALPACC_ERR_ARG, "bad grammar %d", grammar);
+  if (mode != ALPACC_MODE_LEXER && mode != ALPACC_MODE_PARSER && mode != ALPACC_MODE_BOTH)
+    return alpacc::fail(ALPACC_ERR_ARG, "bad mode %d", mode);
+  auto r = std::make_unique<Req>();
+  r->g = (uint32_t)grammar;
+  r->mode = mode;
+  r->own.assign(payload, payload + n);
+  r->data = r->own.data();
+  r->n = n;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    *ticket = s->next_ticket++;
+    enqueue(s, r.get());
+    s->tickets.emplace(*ticket, std::move(r));
+  }
+  s->cv_work.notify_all();
+  return 0;
+}
+
+int alpacc_hip_server_wait(alpacc_hip_server* s, uint64_t ticket, uint8_t** record, uint64_t* record_len) {
+  if (!s || !record || !record_len) return alpacc::fail(ALPACC_ERR_ARG, "null argument");
+  std::unique_ptr<Req> r;
+  {
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->tickets.find(ticket);
+    if (it == s->tickets.end()) return alpacc::fail(ALPACC_ERR_ARG, "unknown ticket %llu", (unsigned long long)ticket);
+    Req* p = it->second.get();
+    s->cv_done.wait(lk, [&] { return p->status != 0; });
+    r = std::move(it->second);
+    s->tickets.erase(it);
+  }
+  if (r->status < 0) return alpacc::fail(r->status, "%s", r->err.c_str());
+  uint8_t* o = static_cast<uint8_t*>(malloc(r->rec.size() ? r->rec.size() : 1));
+  if (!o) return alpacc::fail(ALPACC_ERR_NOMEM, "host allocation failed");
+  memcpy(o, r->rec.data(), r->rec.size());
+  *record = o;
+  *record_len = r->rec.size();
+  return 0;
+}
+
+int alpacc_hip_server_replay(alpacc_hip_server* s, uint64_t count, const int32_t* grammar, const int32_t* mode,
+                             const uint8_t* packed, const uint64_t* offsets, const uint64_t* arrival_ns,
+                             uint64_t* latency_ns, uint8_t** records, uint64_t* records_len) {
+  if (!s || !grammar || !mode || !offsets || !arrival_ns || !latency_ns || !records || !records_len)
+    return alpacc::fail(ALPACC_ERR_ARG, "null argument");
+  for (uint64_t i = 0; i < count; ++i) {
+    if (grammar[i] < 0 || (size_t)grammar[i] >= s->names.size() || mode[i] < 0 || mode[i] > 2 ||
+        offsets[i + 1] < offsets[i])
+      return alpacc::fail(ALPACC_ERR_ARG, "bad request %llu", (unsigned long long)i);
+  }
+  std::vector<Req> reqs(count);
+  for (uint64_t i = 0; i < count; ++i) {
+    reqs[i].g = (uint32_t)grammar[i];
+    reqs[i].mode = mode[i];
+    reqs[i].data = packed + offsets[i];
+    reqs[i].n = offsets[i + 1] - offsets[i];
+  }
+  const Clock::time_point t0 = Clock::now();
+  std::thread submitter([&] {
+    for (uint64_t i = 0; i < count; ++i) {
+      std::this_thread::sleep_until(t0 + std::chrono::nanoseconds(arrival_ns[i]));
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        enqueue(s, &reqs[i]);
+      }
+      s->cv_work.notify_all();
+    }
+  });
+  {
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (uint64_t i = 0; i < count; ++i) s->cv_done.wait(lk, [&] { return reqs[i].status != 0; });
+  }
+  submitter.join();
+  size_t total = 0;
+  for (uint64_t i = 0; i < count; ++i) {
+    if (reqs[i].status < 0) return alpacc::fail(reqs[i].status, "%s", reqs[i].err.c_str());
+    total += reqs[i].rec.size();
+    latency_ns[i] = (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+                        reqs[i].done - (t0 + std::chrono::nanoseconds(arrival_ns[i])))
+                        .count();
+  }
+  uint8_t* o = static_cast<uint8_t*>(malloc(total ? total : 1));
+  if (!o) return alpacc::fail(ALPACC_ERR_NOMEM, "host allocation failed");
+  size_t at = 0;
+  for (uint64_t i = 0; i < count; ++i) {
+    memcpy(o + at, reqs[i].rec.data(), reqs[i].rec.size());
+    at += reqs[i].rec.size();
+  }
+  *records = o;
+  *records_len = total;
+  return 0;
+}
+
+int alpacc_hip_server_stats(alpacc_hip_server* s, uint64_t* batches, uint64_t* requests, uint64_t* max_batch) {
+  if (!s || !batches || !requests || !max_batch) return alpacc::fail(ALPACC_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  *batches = s->n_batches;
+  *requests = s->n_reqs;
+  *max_batch = s->biggest;
+  return 0;
+}
+
+}  // extern "C"

@@ -112,19 +112,71 @@ def device_tables(g: Grammar) -> dict:
             if not -128 <= v <= 127:
                 raise EmitError(f"{g.name}: pair depth profile {v} exceeds i8")
         profile += [d & 0xFF, dmin & 0xFF, e & 0xFF, emin & 0xFF, nt, 0, 0, 0]
-    # Compiled pair records, 32 bytes each: counts and depth profile (bytes
-    # 0-7), up to 8 brackets (u16, bytes 8-23), up to 8 productions (u8,
-    # bytes 24-31).  One load per window position gives the parser all it
-    # needs; PROD_INFO packs arity | terminal << 7 per production.
+    # Pair records, 64 bytes each: the window's net effect.  Inside one
+    # window a right bracket that meets an earlier left bracket of the same
+    # window cancels it (a mismatch there rejects every input containing
+    # the window, so such windows get no pair id), and a derivation slot
+    # whose pop meets a child slot opened earlier in the window gets its
+    # parent inside the window.  What is left: bracket pops reaching below
+    # the window (in pop order), the window's surviving pushes (bottom to
+    # top), slot pops reaching below it and surviving child slots.
+    #   0 bracket pops   1 bracket pushes   2 productions   3 terminal productions
+    #   4 slot pops      5 slot pushes      6 terminal-production mask   7 0
+    #   8..23   u16 bracket symbols: the pops, then the pushes
+    #   24..31  productions
+    #   32..39  parent of production j: 0x80 | e for the window's e-th slot
+    #           pop, else the window-local compact index of its parent
+    #   40..63  owners of the surviving child slots (window-local compact index)
+    # Window-local compact index: production index minus the terminal
+    # productions before it (nodes of the compact tree).
     rec = []
+    eff = dict(bpop=0, bpush=0, spop=0, spush=0)
     for i, (s0, s1, p0, p1) in enumerate(p.spans):
-        if s1 - s0 > 8 or p1 - p0 > 8:
-            raise EmitError(f"{g.name}: a window has more than 8 brackets or productions")
-        pf = profile[8 * i:8 * i + 8]
-        head = [s1 - s0, p1 - p0, pf[4], pf[0], pf[1], pf[2], pf[3], 0]
-        br = stacks16[s0:s1] + [0] * (8 - (s1 - s0))
-        pr = list(p.productions[p0:p1]) + [0] * (8 - (p1 - p0))
-        rec += head + [b for v in br for b in (v & 0xFF, v >> 8)] + pr
+        st, ext = [], []
+        ok = True
+        for v in stacks16[s0:s1]:
+            if v & 0x8000:
+                st.append(v & 0x7FFF)
+            elif st:
+                ok &= st.pop() == v
+            else:
+                ext.append(v)
+        prods = list(p.productions[p0:p1])
+        if len(prods) > 8:
+            raise EmitError(f"{g.name}: a window has more than 8 productions")
+        if len(ext) + len(st) > 8:
+            raise EmitError(f"{g.name}: a window keeps more than 8 brackets")
+        compact, terms, tmask = [], 0, 0
+        for j, prod in enumerate(prods):
+            compact.append(j - terms)
+            if p.prod_terminal[prod] is not None:
+                terms += 1
+                tmask |= 1 << j
+        slots, parent, spop = [], [], 0
+        for j, prod in enumerate(prods):
+            if slots:
+                parent.append(slots.pop())
+            else:
+                parent.append(0x80 | spop)
+                spop += 1
+            if p.arity[prod] and tmask >> j & 1:
+                raise EmitError(f"{g.name}: terminal production with children")
+            slots += [compact[j]] * p.arity[prod]
+        if len(slots) > 24:
+            raise EmitError(f"{g.name}: a window keeps more than 24 child slots")
+        if not ok:
+            for code in [c for c, v in enumerate(window) if v == i]:
+                window[code] = 0xFFFF
+        eff["bpop"] = max(eff["bpop"], len(ext))
+        eff["bpush"] = max(eff["bpush"], len(st))
+        eff["spop"] = max(eff["spop"], spop)
+        eff["spush"] = max(eff["spush"], len(slots))
+        br = ext + st + [0] * (8 - len(ext) - len(st))
+        rec += [len(ext), len(st), len(prods), terms, spop, len(slots), tmask, 0]
+        rec += [b for v in br for b in (v & 0xFF, v >> 8)]
+        rec += prods + [0] * (8 - len(prods))
+        rec += parent + [0] * (8 - len(parent))
+        rec += slots + [0] * (24 - len(slots))
     prod_info = []
     for prod in range(p.n_productions):
         if p.arity[prod] > 127:
@@ -143,6 +195,7 @@ def device_tables(g: Grammar) -> dict:
         window=window, base=base, spans=spans, stacks=stacks16, productions=p.productions,
         arity=p.arity, prod_is_term=[1 if t is not None else 0 for t in p.prod_terminal],
         profile=profile, max_slot_items=max_slot_items, pair_rec=rec, prod_info=prod_info,
+        max_bpop=eff["bpop"], max_bpush=eff["bpush"], max_spop=eff["spop"], max_spush=eff["spush"],
     )
 
 
@@ -164,7 +217,7 @@ def emit(g: Grammar) -> str:
         _arr("uint16_t", "WINDOW", d["window"]),
         _arr("int32_t", "SPANS", d["spans"]),
         _arr("uint8_t", "PROFILE", d["profile"]),
-        "__attribute__((aligned(32)))\n" + _arr("uint8_t", "PAIR_REC", d["pair_rec"]),
+        "__attribute__((aligned(64)))\n" + _arr("uint8_t", "PAIR_REC", d["pair_rec"]),
         _arr("uint8_t", "PROD_INFO", d["prod_info"]),
         _arr("uint16_t", "STACKS", d["stacks"]),
         _arr("uint8_t", "PRODUCTIONS", d["productions"]),
@@ -186,6 +239,7 @@ def emit(g: Grammar) -> str:
         f"  {p.n_productions}, {p.production_bytes}, {ns}::ARITY, {ns}::PROD_IS_TERMINAL,",
         f"  {max(1, p.max_brackets)}, {max(1, p.max_prods)}, {d['max_slot_items']},",
         f"  {ns}::PAIR_REC, {ns}::PROD_INFO,",
+        f"  {max(1, d['max_bpop'])}, {max(1, d['max_bpush'])}, {max(1, d['max_spop'])}, {max(1, d['max_spush'])},",
         "};",
         "}  // namespace alpacc",
         "",

@@ -37,6 +37,13 @@ class Tree(C.Structure):
                 ("parents", C.c_void_p), ("token_parents", C.c_void_p)]
 
 
+class Batch(C.Structure):
+    _fields_ = [("count", C.c_uint64), ("valid", C.c_void_p), ("token_offsets", C.c_void_p),
+                ("node_offsets", C.c_void_p), ("node_counts", C.c_void_p), ("token_parent_offsets", C.c_void_p),
+                ("tokens", Lexemes), ("productions", C.c_void_p), ("parents", C.c_void_p),
+                ("token_parents", C.c_void_p)]
+
+
 class HipError(RuntimeError):
     pass
 
@@ -50,6 +57,15 @@ def load_library(path: str = LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise ImportError(f"{path} is missing: run `python -m alpacc_amd.build` first")
+    # PyTorch ships its own HIP runtime under the same soname as the system
+    # one, and a process keeps whichever is loaded first.  Loading torch's
+    # first (when torch is installed) gives the library and torch one shared
+    # runtime, so device pointers and streams pass between them and torch
+    # still finds the GPU after the library has opened it.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(path)
     vp, u64, i32 = C.c_void_p, C.c_uint64, C.c_int
     sig = {
@@ -71,6 +87,17 @@ def load_library(path: str = LIB_PATH):
         "alpacc_hip_grammar_payload_bytes": (u64, [C.c_char_p, i32, u64]),
         "alpacc_hip_set_timing": (i32, [vp, i32]),
         "alpacc_hip_timing": (i32, [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(u64)]),
+        "alpacc_hip_timed_kernels": (C.c_char_p, [vp]),
+        "alpacc_hip_run_batch": (i32, [vp, i32, vp, C.POINTER(u64), u64, C.POINTER(vp), C.POINTER(u64)]),
+        "alpacc_hip_lex_parse_batch_device": (i32, [vp, vp, C.POINTER(u64), u64, C.POINTER(Batch), vp]),
+        "alpacc_hip_server_open": (i32, [C.POINTER(C.c_char_p), i32, i32, i32, u64, u64, C.POINTER(vp)]),
+        "alpacc_hip_server_close": (None, [vp]),
+        "alpacc_hip_server_submit": (i32, [vp, i32, i32, vp, u64, C.POINTER(u64)]),
+        "alpacc_hip_server_wait": (i32, [vp, u64, C.POINTER(vp), C.POINTER(u64)]),
+        "alpacc_hip_server_replay": (i32, [vp, u64, C.POINTER(C.c_int32), C.POINTER(C.c_int32), vp,
+                                           C.POINTER(u64), C.POINTER(u64), C.POINTER(u64), C.POINTER(vp),
+                                           C.POINTER(u64)]),
+        "alpacc_hip_server_stats": (i32, [vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -104,6 +131,8 @@ def _check(rc: int) -> bool:
 
 def _take(ptr, nbytes: int, dtype) -> np.ndarray:
     if nbytes == 0:
+        if ptr:
+            load_library().alpacc_hip_free(ptr)
         return np.zeros(0, dtype=dtype)
     buf = (C.c_uint8 * nbytes).from_address(ptr)
     arr = np.frombuffer(bytes(buf), dtype=dtype).copy()
@@ -162,6 +191,31 @@ class Backend:
         _check(self.lib.alpacc_hip_run_case(self.h, MODES[mode], n, src, C.byref(rec), C.byref(ln)))
         return _take(rec.value, ln.value, np.uint8).tobytes()
 
+    def run_batch(self, mode: str, payloads) -> bytes:
+        """One launch sequence for many requests: the concatenated response
+        records, each identical to run_case's record for that payload."""
+        offs = np.zeros(len(payloads) + 1, dtype=np.uint64)
+        offs[1:] = np.cumsum([len(x) for x in payloads], dtype=np.uint64)
+        packed = b"".join(payloads)
+        rec = C.c_void_p()
+        ln = C.c_uint64()
+        rc = self.lib.alpacc_hip_run_batch(self.h, MODES[mode], C.c_char_p(packed),
+                                           offs.ctypes.data_as(C.POINTER(C.c_uint64)), len(payloads),
+                                           C.byref(rec), C.byref(ln))
+        _check(min(rc, 0))
+        return _take(rec.value, ln.value, np.uint8).tobytes() if ln.value else b""
+
+    def lex_parse_batch_device(self, d_ptr: int, offsets: np.ndarray, stream: int = 0) -> Batch:
+        """Combined mode over inputs packed in device memory; offsets is a
+        host uint64 array of count + 1 entries.  Results stay on the device."""
+        offs = np.ascontiguousarray(offsets, dtype=np.uint64)
+        out = Batch()
+        rc = self.lib.alpacc_hip_lex_parse_batch_device(self.h, C.c_void_p(d_ptr),
+                                                        offs.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                                        len(offs) - 1, C.byref(out), C.c_void_p(stream or None))
+        _check(min(rc, 0))
+        return out
+
     def lex_string(self, data: bytes, mode: str = "lexer"):
         """lex_string: (accepted, terminals, starts, lengths)."""
         out = Lexemes()
@@ -204,6 +258,11 @@ class Backend:
     def set_timing(self, on: bool = True):
         _check(self.lib.alpacc_hip_set_timing(self.h, int(on)) >= 0 and 1 or -1)
 
+    def timed_kernels(self):
+        """Names of the kernels timed since set_timing(True)."""
+        names = self.lib.alpacc_hip_timed_kernels(self.h)
+        return [x for x in (names or b"").decode().split(",") if x]
+
     def timing(self, stage: str):
         """(total milliseconds, runs) of a stage since set_timing."""
         ms = C.c_double()
@@ -224,3 +283,61 @@ class Backend:
         ok = _check(self.lib.alpacc_hip_lex_parse_device(self.h, C.c_void_p(d_ptr), n, C.byref(out),
                                                          C.c_void_p(stream or None)))
         return ok, out
+
+
+class Server:
+    """Continuous-batching request loop over resident grammars (see
+    alpacc_hip_server_open)."""
+
+    def __init__(self, grammars, device: int = 0, lanes: int = 2, hbm_budget: int = 64 << 30, max_batch: int = 0):
+        self.lib = load_library()
+        self.grammars = list(grammars)
+        arr = (C.c_char_p * len(self.grammars))(*[g.encode() for g in self.grammars])
+        h = C.c_void_p()
+        _check(self.lib.alpacc_hip_server_open(arr, len(self.grammars), device, lanes, hbm_budget, max_batch,
+                                               C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.alpacc_hip_server_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, grammar: str, mode: str, payload: bytes) -> int:
+        t = C.c_uint64()
+        _check(self.lib.alpacc_hip_server_submit(self.h, self.grammars.index(grammar), MODES[mode],
+                                                 C.c_char_p(payload), len(payload), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int) -> bytes:
+        rec = C.c_void_p()
+        ln = C.c_uint64()
+        _check(self.lib.alpacc_hip_server_wait(self.h, ticket, C.byref(rec), C.byref(ln)))
+        return _take(rec.value, ln.value, np.uint8).tobytes()
+
+    def replay(self, grammar_idx, modes, payloads, arrival_ns):
+        """Open-loop replay: (records concatenated, latencies in ns)."""
+        count = len(payloads)
+        g = np.ascontiguousarray(grammar_idx, dtype=np.int32)
+        m = np.ascontiguousarray([MODES[x] if isinstance(x, str) else x for x in modes], dtype=np.int32)
+        offs = np.zeros(count + 1, dtype=np.uint64)
+        offs[1:] = np.cumsum([len(x) for x in payloads])
+        packed = b"".join(payloads)
+        arr = np.ascontiguousarray(arrival_ns, dtype=np.uint64)
+        lat = np.zeros(count, dtype=np.uint64)
+        rec = C.c_void_p()
+        ln = C.c_uint64()
+        u64p = C.POINTER(C.c_uint64)
+        _check(self.lib.alpacc_hip_server_replay(
+            self.h, count, g.ctypes.data_as(C.POINTER(C.c_int32)), m.ctypes.data_as(C.POINTER(C.c_int32)),
+            C.c_char_p(packed), offs.ctypes.data_as(u64p), arr.ctypes.data_as(u64p), lat.ctypes.data_as(u64p),
+            C.byref(rec), C.byref(ln)))
+        return _take(rec.value, ln.value, np.uint8).tobytes(), lat
+
+    def stats(self):
+        b, r, m = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        _check(self.lib.alpacc_hip_server_stats(self.h, C.byref(b), C.byref(r), C.byref(m)))
+        return {"batches": b.value, "requests": r.value, "max_batch": m.value}

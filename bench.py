@@ -25,6 +25,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
+import ctypes
+
+C_u8 = ctypes.c_uint8
+
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
@@ -52,6 +56,193 @@ def cpu_baseline(grammar: str, mode: str, seed: int, conforming: bool, sample_by
     }
 
 
+# Algorithmic bytes of one launch of each kernel: input read plus final
+# outputs written (JSON widths: terminal 1, index 8, length 2, production 1);
+# intermediates (lane prefixes, pair ids, tile records) are not counted.
+def kernel_bytes(name: str, n: int, T: int, N: int) -> int:
+    return {
+        "lex_fold": n,                 # input
+        "lex_emit": n + 11 * T,        # input + (terminal, start, length) per token
+        "pair_profile": T,             # tokens
+        "tile_resolve": 8 * T + 9 * N,  # token parents + (production, parent) per node
+    }.get(name, 0)
+
+
+def load_traffic(path: str, kernel: str):
+    """HBM bytes per launch of `kernel` from a summary written by
+    --pmc-summary (FETCH_SIZE and WRITE_SIZE, each scaled by a copy kernel)."""
+    if not path or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else k["bytes_per_launch"]
+
+
+def copy_calibration(nbytes: int, reps: int):
+    """A plain copy of a known size, to scale the FETCH_SIZE / WRITE_SIZE
+    counters: each dispatch reads and writes exactly nbytes."""
+    import torch
+    src = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    src.fill_(7)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps}), flush=True)
+
+
+def read_pmc(csv_dir: str, counter: str):
+    """{kernel short name: mean counter value per dispatch} from a rocprofv3
+    --pmc run written with --output-format csv."""
+    import csv
+    import glob
+    files = glob.glob(os.path.join(csv_dir, "**", "*counter_collection.csv"), recursive=True)
+    acc = {}
+    for fn in files:
+        with open(fn) as f:
+            for row in csv.DictReader(f):
+                if row.get("Counter_Name") != counter:
+                    continue
+                name = row["Kernel_Name"].replace("(anonymous namespace)::", "")
+                short = name.split("(")[0].split("::")[-1]
+                acc.setdefault(short, []).append(float(row["Counter_Value"]))
+    return {k: (sum(v) / len(v), max(v)) for k, v in acc.items()}
+
+
+def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, copy_bytes, out_path):
+    """Per-kernel HBM bytes per launch: FETCH_SIZE and WRITE_SIZE (KiB per
+    dispatch), each scaled so a plain copy of copy_bytes reads as copy_bytes."""
+    fetch = {k: v[0] for k, v in read_pmc(fetch_dir, "FETCH_SIZE").items()}
+    write = {k: v[0] for k, v in read_pmc(write_dir, "WRITE_SIZE").items()}
+    cf, cw = read_pmc(copy_fetch_dir, "FETCH_SIZE"), read_pmc(copy_write_dir, "WRITE_SIZE")
+    ck = "__amd_rocclr_copyBuffer"  # the device-to-device copy; its largest dispatch is the calibration copy
+    f_scale = copy_bytes / cf[ck][1]
+    w_scale = copy_bytes / cw[ck][1]
+    out = {"copy_kernel": ck, "copy_bytes": copy_bytes, "fetch_scale_bytes_per_unit": f_scale,
+           "write_scale_bytes_per_unit": w_scale, "kernels": {}}
+    for k in sorted(set(fetch) | set(write)):
+        fb = fetch.get(k, 0.0) * f_scale
+        wb = write.get(k, 0.0) * w_scale
+        out["kernels"][k] = {"fetch_bytes": round(fb), "write_bytes": round(wb), "bytes_per_launch": round(fb + wb)}
+    with open(out_path, "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+    print(json.dumps(out, sort_keys=True))
+
+
+def batch_inputs(count: int, seed: int, median: int = 4096, sigma: float = 1.5, lo: int = 64, hi: int = 4 << 20):
+    """`count` independent grammar-conforming JSON inputs, byte lengths
+    lognormal(ln median, sigma) clipped to [lo, hi], packed into one buffer
+    with a u64 offsets array (count + 1 entries)."""
+    import numpy as np
+    from alpacc_amd import datagen
+    rng = np.random.default_rng(seed)
+    sizes = np.clip(np.round(rng.lognormal(np.log(median), sigma, count)), lo, hi).astype(np.int64)
+    offs = np.zeros(count + 1, dtype=np.uint64)
+    offs[1:] = np.cumsum(sizes)
+    host = np.empty(int(offs[-1]) + 16, dtype=np.uint8)
+    for i in range(count):
+        datagen.generate("json", int(sizes[i]), seed=seed * 1000003 + i, conforming=True, wrap=True,
+                         out=host[int(offs[i]):int(offs[i + 1])])
+    return host, offs
+
+
+def batch_workload(be, count: int, steps: int, seed: int = 3):
+    """Batching: all inputs lexed and parsed by one batched call per step,
+    against the plain path (one lex+parse call, i.e. one launch sequence,
+    per input) over the same device-resident inputs."""
+    import numpy as np
+    import torch
+    host, offs = batch_inputs(count, seed)
+    total = int(offs[-1])
+    dev = torch.from_numpy(host).to("cuda")
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream().cuda_stream
+    base = dev.data_ptr()
+    b = be.lex_parse_batch_device(base, offs, stream=stream)  # warmup
+    valid = int(np.ctypeslib.as_array((C_u8 * count).from_address(b.valid)).sum())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        be.lex_parse_batch_device(base, offs, stream=stream)
+    torch.cuda.synchronize()
+    batched_s = (time.perf_counter() - t0) / steps
+    # plain: one call per input, in input order
+    for i in range(min(count, 200)):  # warmup of the per-input path
+        be.lex_parse_device(base + int(offs[i]), int(offs[i + 1] - offs[i]), stream=stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain_valid = 0
+    for i in range(count):
+        ok, _ = be.lex_parse_device(base + int(offs[i]), int(offs[i + 1] - offs[i]), stream=stream)
+        plain_valid += bool(ok)
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - t0
+    del dev
+    return {
+        "workload": f"{count} json inputs, lognormal(ln 4096, 1.5) bytes clipped to [64, 4 MiB], seed {seed}",
+        "count": count, "bytes": total, "valid": valid, "plain_valid": plain_valid,
+        "batched_ms": round(batched_s * 1e3, 3), "batched_GBps": round(total / batched_s / 1e9, 3),
+        "plain_ms": round(plain_s * 1e3, 3), "plain_GBps": round(total / plain_s / 1e9, 3),
+        "speedup_vs_plain": round(plain_s / batched_s, 2),
+    }
+
+
+def serve_trace(count: int, rate: float, seed: int, max_bytes: int = 16 << 20):
+    """Requests of the serving workload: 70% json / 30% arithmetic, sizes
+    lognormal(ln 2048, 2) clipped to [128, max_bytes], 3% with one random
+    byte flipped, open-loop Poisson arrivals at `rate` per second."""
+    import numpy as np
+    from alpacc_amd import datagen
+    rng = np.random.default_rng(seed)
+    gidx = (rng.random(count) >= 0.7).astype(np.int32)  # 0 json, 1 arithmetic
+    sizes = np.clip(np.round(rng.lognormal(np.log(2048), 2.0, count)), 128, max_bytes).astype(np.int64)
+    flips = rng.random(count) < 0.03
+    arrival = np.cumsum(rng.exponential(1.0 / rate, count)) * 1e9
+    payloads = []
+    for i in range(count):
+        data = datagen.generate(("json", "arithmetic")[gidx[i]], int(sizes[i]), seed=seed * 7919 + i,
+                                conforming=True).copy()
+        if flips[i]:
+            data[rng.integers(0, len(data))] = rng.integers(0, 256)
+        payloads.append(data.tobytes())
+    return gidx, payloads, arrival.astype(np.uint64)
+
+
+def serve_workload(rates, seconds: float, seed: int = 4, plain: bool = True):
+    """Serving: the trace replayed open-loop against the continuous-batching
+    server (two lanes per grammar) and against the plain server (one request
+    per launch sequence, one lane), both with json and arithmetic resident."""
+    import numpy as np
+    from alpacc_amd import hip
+    names = ["json", "arithmetic"]
+    out = []
+    for rate in rates:
+        count = max(1, int(rate * seconds))
+        gidx, payloads, arrival = serve_trace(count, rate, seed + int(rate))
+        total = sum(len(x) for x in payloads)
+        modes = ["both"] * count
+        row = {"rate_req_s": rate, "requests": count, "bytes": total}
+        for label, lanes, max_batch in (("served", 2, 0), ("plain", 1, 1)) if plain else (("served", 2, 0),):
+            srv = hip.Server(names, lanes=lanes, hbm_budget=64 << 30, max_batch=max_batch)
+            srv.replay(gidx[:min(count, 64)], modes[:min(count, 64)], payloads[:min(count, 64)],
+                       np.zeros(min(count, 64), dtype=np.uint64))  # warmup
+            recs, lat = srv.replay(gidx, modes, payloads, arrival)
+            st = srv.stats()
+            srv.close()
+            span_s = float(np.max(arrival + lat)) / 1e9
+            row[label] = {"p50_ms": round(float(np.percentile(lat, 50)) / 1e6, 3),
+                          "p99_ms": round(float(np.percentile(lat, 99)) / 1e6, 3),
+                          "GBps": round(total / span_s / 1e9, 3), "req_s": round(count / span_s, 1),
+                          "batches": st["batches"], "max_batch": st["max_batch"]}
+            log(f"serve {rate:.0f} req/s {label}: {row[label]}")
+        if plain:
+            row["throughput_vs_plain"] = round(row["served"]["GBps"] / max(row["plain"]["GBps"], 1e-9), 2)
+        out.append(row)
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -61,11 +252,29 @@ def main(argv=None):
     ap.add_argument("--variant", choices=["conforming", "literal"], default="conforming")
     ap.add_argument("--cpu-sample", type=int, default=256 << 20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=20000, help="inputs of the batching workload (0 = skip)")
+    ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--serve-rates", default="2000,10000,50000", help="request rates of the serving workload ('' = skip)")
+    ap.add_argument("--serve-seconds", type=float, default=1.0)
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_json_lexparse_1GiB.json"),
+                    help="per-kernel HBM bytes per launch written by --pmc-summary")
+    ap.add_argument("--copy-calibration", type=int, metavar="BYTES", default=0,
+                    help="only run a plain device copy of BYTES (counter calibration)")
+    ap.add_argument("--pmc-summary", nargs=5, metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR",
+                                                       "COPY_WRITE_DIR", "OUT"),
+                    help="summarise rocprofv3 FETCH_SIZE/WRITE_SIZE runs into OUT (no GPU)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported by this build "
               f"(the lex/parse path runs on a single MI355X)", file=sys.stderr)
         return 2
+    if a.pmc_summary:
+        fd, wd, cfd, cwd, out = a.pmc_summary
+        pmc_summary(fd, wd, cfd, cwd, a.copy_calibration or (1 << 30), out)
+        return 0
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration, 3)
+        return 0
 
     import numpy as np
     import torch
@@ -88,9 +297,7 @@ def main(argv=None):
     for i in range(a.warmup):
         ok, tree = step()
         log(f"warmup {i}: valid={ok}")
-    T = tree.tokens.count if ok else 0
-    N = tree.num_nodes if ok else 0
-    be.set_timing(True)
+    # timed steps: no per-kernel events inside the timed region
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(a.steps):
@@ -98,34 +305,54 @@ def main(argv=None):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / a.steps
+    T = tree.tokens.count if ok else 0
+    N = tree.num_nodes if ok else 0
+
+    # per-kernel durations: the same steps again with HIP events around
+    # every launch on the launch stream
+    be.set_timing(True)
+    for i in range(a.steps):
+        step()
+    kernels = {}
+    for k in be.timed_kernels():
+        tot, runs = be.timing(k)
+        kernels[k] = tot / max(runs, 1)
     lex_ms, lex_runs = be.timing("lex")
     par_ms, par_runs = be.timing("parse")
-    lex_avg = lex_ms / max(lex_runs, 1)
-    par_avg = par_ms / max(par_runs, 1)
-    log(f"step {ms:.3f} ms; lexer kernel {lex_avg:.3f} ms; parser stage {par_avg:.3f} ms; T={T} N={N} valid={ok}")
+    be.set_timing(False)
+    log(f"step {ms:.3f} ms; lexer {lex_ms / max(lex_runs, 1):.3f} ms; parser {par_ms / max(par_runs, 1):.3f} ms; "
+        f"T={T} N={N} valid={ok}")
+    log("kernels (avg ms): " + ", ".join(f"{k} {v:.3f}" for k, v in sorted(kernels.items(), key=lambda x: -x[1])))
 
-    # algorithmic bytes (JSON widths: terminal 1, index 8, length 2, production 1)
-    lex_bytes = n + 11 * T
-    par_bytes = T + 8 * T + 9 * N
-    if lex_avg >= par_avg or not ok:
-        stage, sbytes, savg = "tile lexer (lex_tiles)", lex_bytes, lex_avg
-    else:
-        stage, sbytes, savg = "parser stage (pair lookup .. compact tree)", par_bytes, par_avg
+    dom = max(kernels, key=kernels.get)
+    savg = kernels[dom]
+    sbytes = kernel_bytes(dom, n, T, N)
     achieved = sbytes / (savg / 1e3) / 1e9 if savg > 0 else 0.0
     roofline = {
         "bound": "hbm",
-        "kernel": stage,
+        "kernel": dom,
         "achieved": round(achieved, 2),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": load_traffic(a.traffic, dom),
         "algorithmic_bytes": sbytes,
         "avg_ms": round(savg, 4),
     }
+    batched = None
+    if a.batch_count:
+        log(f"batching workload: {a.batch_count} inputs")
+        batched = batch_workload(be, a.batch_count, a.batch_steps)
+        log(f"batched {batched['batched_GBps']} GB/s, plain {batched['plain_GBps']} GB/s, "
+            f"x{batched['speedup_vs_plain']}")
+    served = None
+    if a.serve_rates:
+        rates = [float(x) for x in a.serve_rates.split(",") if x]
+        served = serve_workload(rates, a.serve_seconds)
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline("json", "both", 1, conforming, min(a.cpu_sample, n))
+    path_bytes = n + 19 * T + 9 * N  # input + tokens + tree (combined-mode outputs)
     out = {
         "metric": "lex+parse input GB/s (1 GiB, %HBM peak); batched GB/s; served p50/p99 ms",
         "value": round(n / (ms / 1e3) / 1e9, 3),
@@ -143,7 +370,11 @@ def main(argv=None):
                    "grammar": "grammars/json.alp", "bytes": n, "tokens": T, "tree_nodes": N,
                    "valid": bool(ok), "variant": a.variant},
         "pct_hbm_peak_input": round(100 * n / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 2),
+        "path_algorithmic_GBps": round(path_bytes / (ms / 1e3) / 1e9, 2),
+        "kernels_ms": {k: round(v, 4) for k, v in sorted(kernels.items(), key=lambda x: -x[1])},
         "roofline": roofline,
+        "batched": batched,
+        "served": served,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

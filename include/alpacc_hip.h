@@ -106,13 +106,16 @@ int alpacc_hip_run_case(alpacc_hip_ctx *ctx, int mode, uint64_t n, const uint8_t
 
 void alpacc_hip_free(void *p);
 
-/* Stage timing with HIP events recorded on the launch stream (no reference
+/* Timing with HIP events recorded on the launch stream (no reference
  * counterpart; the reference's CUDA CLI times with events in --benchmark,
- * backends/cuda/cli.cu:459-493).  Stages: "lex" = the lexer kernel, "parse" =
- * the parser's kernels from the pair lookup to the tree compaction.
- * set_timing resets the totals. */
+ * backends/cuda/cli.cu:459-493).  Stages: "lex" = the lexer's launches,
+ * "parse" = the parser's launches from the pair lookup to the tree; every
+ * kernel is also timed by its own name (alpacc_hip_timed_kernels lists the
+ * names seen, comma-separated).  set_timing resets the totals; while on,
+ * each launch is bracketed by its own pair of events. */
 int alpacc_hip_set_timing(alpacc_hip_ctx *ctx, int on);
 int alpacc_hip_timing(const alpacc_hip_ctx *ctx, const char *stage, double *total_ms, uint64_t *runs);
+const char *alpacc_hip_timed_kernels(const alpacc_hip_ctx *ctx);
 
 /* ---- device-memory entry points ---------------------------------------- */
 /* Input already in device memory; results stay in device buffers owned by the
@@ -127,6 +130,72 @@ int alpacc_hip_parse_device(alpacc_hip_ctx *ctx, const void *d_tokens, uint64_t 
 
 int alpacc_hip_lex_parse_device(alpacc_hip_ctx *ctx, const uint8_t *d_str, uint64_t n,
                                 alpacc_hip_tree *d_out, void *stream);
+
+/* ---- batches -------------------------------------------------------------
+ * `count` inputs packed back to back: input i is bytes [offsets[i],
+ * offsets[i + 1]) of the packed buffer (offsets: a HOST array of count + 1
+ * entries; inputs need no alignment).  The whole batch runs as one launch
+ * sequence with per-input tiles.  The reference answers a counted batch one
+ * request at a time (generated run_case per frame, Generator/C/
+ * Generator.hs:64-152, driven by backends/c/cli.c:167-189); these calls
+ * replace that loop. */
+
+/* Response records of every input, concatenated in input order; each is
+ * byte-identical to alpacc_hip_run_case's record for that input alone.
+ * Returns 0 or an error; *records is malloc'd (alpacc_hip_free). */
+int alpacc_hip_run_batch(alpacc_hip_ctx *ctx, int mode, const uint8_t *packed, const uint64_t *offsets,
+                         uint64_t count, uint8_t **records, uint64_t *records_len);
+
+/* Lex + parse (combined mode) of a batch already in device memory.  Results
+ * stay on the device; the per-input index arrays are host memory owned by
+ * the context.  Valid until the next call on the context. */
+typedef struct {
+  uint64_t count;
+  const uint8_t *valid;                  /* [count] 1 = accepted */
+  const uint64_t *token_offsets;         /* [count + 1] first token of each input in tokens */
+  const uint64_t *node_offsets;          /* [count] first node of each input in productions/parents */
+  const uint64_t *node_counts;           /* [count] */
+  const uint64_t *token_parent_offsets;  /* [count] first entry of each input in token_parents */
+  alpacc_hip_lexemes tokens;             /* device; starts relative to their input */
+  void *productions;                     /* device, production_t per node */
+  int64_t *parents;                      /* device, relative to the input's first node */
+  int64_t *token_parents;                /* device, relative to the input's first node */
+} alpacc_hip_batch;
+
+int alpacc_hip_lex_parse_batch_device(alpacc_hip_ctx *ctx, const uint8_t *d_packed, const uint64_t *offsets,
+                                      uint64_t count, alpacc_hip_batch *out, void *stream);
+
+/* ---- serving ---------------------------------------------------------------
+ * Continuous batching over resident grammars (no reference counterpart: the
+ * reference's server answers one request at a time, backends/c/cli.c:
+ * 167-189).  Every grammar gets `lanes` contexts (own stream, buffers and
+ * worker thread); a free lane takes every request queued for its grammar in
+ * one mode, up to `hbm_budget` bytes of estimated device memory admitted at
+ * once (0 = 64 GiB) and `max_batch` requests (0 = no limit), and answers them
+ * with one batched launch sequence.  Records are byte-identical to
+ * alpacc_hip_run_case's for each request alone. */
+typedef struct alpacc_hip_server alpacc_hip_server;
+
+int alpacc_hip_server_open(const char *const *grammars, int ngrammars, int device, int lanes, uint64_t hbm_budget,
+                           uint64_t max_batch, alpacc_hip_server **out);
+void alpacc_hip_server_close(alpacc_hip_server *srv);
+
+/* Asynchronous request (payload copied); *ticket names it for _wait. */
+int alpacc_hip_server_submit(alpacc_hip_server *srv, int grammar, int mode, const uint8_t *payload, uint64_t n,
+                             uint64_t *ticket);
+/* Blocks until the request is answered; *record is malloc'd (alpacc_hip_free). */
+int alpacc_hip_server_wait(alpacc_hip_server *srv, uint64_t ticket, uint8_t **record, uint64_t *record_len);
+
+/* Open-loop replay for measurements: request i (grammar[i], mode[i], bytes
+ * [offsets[i], offsets[i + 1]) of packed) is submitted arrival_ns[i] after
+ * the start; latency_ns[i] = answer time - scheduled arrival.  The records
+ * are concatenated in request order (*records malloc'd). */
+int alpacc_hip_server_replay(alpacc_hip_server *srv, uint64_t count, const int32_t *grammar, const int32_t *mode,
+                             const uint8_t *packed, const uint64_t *offsets, const uint64_t *arrival_ns,
+                             uint64_t *latency_ns, uint8_t **records, uint64_t *records_len);
+
+/* Batches run, requests answered and the largest batch so far. */
+int alpacc_hip_server_stats(alpacc_hip_server *srv, uint64_t *batches, uint64_t *requests, uint64_t *max_batch);
 
 #ifdef __cplusplus
 }

@@ -141,6 +141,26 @@ def test_edge_inputs(backends):
     check(backends, "json", "both", b"[1," + b" " * 70000 + b"2]", "long ignore")
 
 
+def test_dense_and_spanning_tokens(backends):
+    """A token at every byte (tiles of 32 KiB hold up to 32768 tokens, far
+    more than one staging round of the lexer) and tokens spanning tiles."""
+    n = 1 << 20
+    check(backends, "json", "both", b"[" + b"1," * (n // 2) + b"1]", "dense json")
+    check(backends, "json", "lexer", b"[" * n, "open brackets")
+    check(backends, "sexp", "both", b"(" + b"a " * (n // 2) + b")", "dense sexp")
+    check(backends, "arithmetic", "both", b"1" + b"+1" * (n // 2), "dense arithmetic")
+    check(backends, "arithmetic", "lexer", b"1+" * (n // 2), "dense arithmetic, trailing op")
+    # dense and sparse tiles alternating
+    parts = []
+    for i in range(40):
+        parts.append(b"1," * 20000 if i % 2 else b'"' + b"a" * 60000 + b'",')
+    check(backends, "json", "both", b"[" + b"".join(parts) + b"1]", "alternating density")
+    # strings of 65,535 bytes in total are the longest a u16 length holds
+    for ln in (65533, 65534):
+        data = b"[" + (b'"' + b"x" * ln + b'",') * 5 + b"1]"
+        check(backends, "json", "lexer", data, f"string {ln + 2} bytes")
+
+
 def test_deep_nesting(backends):
     depth = 10000
     data = b"(" * depth + b"a" + b")" * depth
@@ -229,3 +249,169 @@ def test_cli_batch_and_server_replay():
             srv = subprocess.run([cli, "-g", grammar, "--server"] + flag, input=batch + batch, capture_output=True,
                                  timeout=120)
             assert srv.returncode == 0 and srv.stdout == ref.stdout + ref.stdout, (grammar, mode)
+
+
+def batch_payloads(grammar, mode, rng, count):
+    """A mixed batch: empty, tiny, tile-boundary and large inputs, valid and
+    rejected, at unaligned offsets when packed."""
+    out = []
+    for i in range(count):
+        kind = rng.random()
+        if kind < 0.05:
+            n = 0
+        elif kind < 0.5:
+            n = rng.randrange(1, 200)
+        elif kind < 0.9:
+            n = rng.randrange(200, 20000)
+        else:
+            n = rng.choice([32767, 32768, 32769, 65537, 200000])
+        data = datagen.generate(grammar, n, seed=rng.randrange(1 << 30), conforming=rng.random() < 0.8).tobytes() if n else b""
+        if data and rng.random() < 0.1:
+            b = bytearray(data)
+            b[rng.randrange(len(b))] = rng.randrange(256)
+            data = bytes(b)
+        if mode == "parser":
+            toks = lexed_tokens(grammar, data)
+            data = toks if toks is not None else bytes(rng.randrange(8) for _ in range(rng.randrange(12)))
+        out.append(data)
+    return out
+
+
+@pytest.mark.parametrize("grammar", ["json", "arithmetic", "sexp"])
+@pytest.mark.parametrize("mode", MODES)
+def test_batch_records(backends, grammar, mode):
+    """alpacc_hip_run_batch answers every input of a batch exactly as the
+    oracle answers it alone."""
+    rng = random.Random(hash((grammar, mode)) & 0xFFFF)
+    for count in (1, 3, 57):
+        payloads = batch_payloads(grammar, mode, rng, count)
+        print(f"batch {grammar}/{mode} count={count} bytes={sum(map(len, payloads))}", flush=True)
+        got = backends[grammar].run_batch(mode, payloads)
+        exp = b"".join(pyoracle.record(grammar, mode, x) for x in payloads)
+        assert got == exp, f"{grammar}/{mode} batch of {count}"
+
+
+def test_batch_alone_and_in_mix(backends):
+    """Each result is the same alone and in a mix: every input of a batch
+    against run_case on that input."""
+    rng = random.Random(99)
+    payloads = batch_payloads("json", "both", rng, 40)
+    be = backends["json"]
+    got = be.run_batch("both", payloads)
+    alone = b"".join(be.run_case("both", x) for x in payloads)
+    assert got == alone
+
+
+def test_batch_device_arrays(backends):
+    import torch
+    from alpacc_amd import wire
+    rng = random.Random(3)
+    payloads = batch_payloads("json", "both", rng, 25)
+    offs = np.zeros(len(payloads) + 1, dtype=np.uint64)
+    offs[1:] = np.cumsum([len(x) for x in payloads])
+    dev = torch.from_numpy(np.frombuffer(b"".join(payloads) + b"\0", dtype=np.uint8).copy()).to("cuda")
+    be = backends["json"]
+    b = be.lex_parse_batch_device(dev.data_ptr(), offs)
+    torch.cuda.synchronize()
+    lay = wire.Layout("both", 1, 8, 2, 1)
+    valid = np.ctypeslib.as_array((C_u8 * len(payloads)).from_address(b.valid))
+    tok_off = np.ctypeslib.as_array((C_u64 * (len(payloads) + 1)).from_address(b.token_offsets))
+    node_off = np.ctypeslib.as_array((C_u64 * len(payloads)).from_address(b.node_offsets))
+    nodes = np.ctypeslib.as_array((C_u64 * len(payloads)).from_address(b.node_counts))
+    tp_off = np.ctypeslib.as_array((C_u64 * len(payloads)).from_address(b.token_parent_offsets))
+    T = int(tok_off[-1])
+    N = int(max([node_off[i] + nodes[i] for i in range(len(payloads))] + [0]))
+    for i, x in enumerate(payloads):
+        rec, _ = wire.decode_record(pyoracle.record("json", "both", x), 0, lay)
+        assert bool(valid[i]) == rec.valid, i
+        if not rec.valid:
+            continue
+        a, n = int(tok_off[i]), int(tok_off[i + 1] - tok_off[i])
+        assert n == len(rec.terminals)
+        got_t = _dev_bytes(b.tokens.terminals + a, n)
+        got_s = np.frombuffer(_dev_bytes(b.tokens.starts + 8 * a, 8 * n), dtype=np.int64)
+        got_p = _dev_bytes(b.productions + int(node_off[i]), int(nodes[i]))
+        got_par = np.frombuffer(_dev_bytes(b.parents + 8 * int(node_off[i]), 8 * int(nodes[i])), dtype=np.int64)
+        got_tp = np.frombuffer(_dev_bytes(b.token_parents + 8 * int(tp_off[i]), 8 * n), dtype=np.int64)
+        assert np.array_equal(np.frombuffer(got_t, np.uint8), rec.terminals), i
+        assert np.array_equal(got_s, rec.starts), i
+        assert np.array_equal(np.frombuffer(got_p, np.uint8), rec.nodes), i
+        assert np.array_equal(got_par, rec.parents), i
+        assert np.array_equal(got_tp, rec.token_parents), i
+
+
+import ctypes as _ct
+
+C_u8 = _ct.c_uint8
+C_u64 = _ct.c_uint64
+
+
+class _DevView:
+    """A device byte range as a __cuda_array_interface__ object."""
+
+    def __init__(self, ptr, n):
+        self.__cuda_array_interface__ = {"shape": (n,), "typestr": "|u1", "data": (ptr, False), "version": 2,
+                                         "strides": None}
+
+
+def _dev_bytes(ptr, n):
+    import torch
+    if n == 0:
+        return b""
+    return torch.as_tensor(_DevView(ptr, n), device="cuda").cpu().numpy().tobytes()
+
+
+def serve_trace(rng, count, rate, max_bytes=1 << 20):
+    """A mixed request trace: 70% json / 30% arithmetic, lognormal sizes
+    (median 2 KiB, sigma 2), 3% with one random byte flipped, Poisson
+    arrivals at `rate` per second."""
+    import math
+    names = ["json", "arithmetic"]
+    gidx, modes, payloads, arrival = [], [], [], []
+    t = 0.0
+    for i in range(count):
+        g = 0 if rng.random() < 0.7 else 1
+        n = int(min(max(rng.lognormvariate(math.log(2048), 2.0), 128), max_bytes))
+        data = bytearray(datagen.generate(names[g], n, seed=rng.randrange(1 << 30), conforming=True).tobytes())
+        if rng.random() < 0.03:
+            data[rng.randrange(len(data))] = rng.randrange(256)
+        t += rng.expovariate(rate)
+        gidx.append(g)
+        modes.append("both" if rng.random() < 0.8 else "lexer")
+        payloads.append(bytes(data))
+        arrival.append(int(t * 1e9))
+    return names, gidx, modes, payloads, arrival
+
+
+def test_server_replay_matches_alone_and_oracle(backends):
+    """Continuous batching: every answer of a mixed open-loop trace equals
+    the oracle's and the answer to the request alone."""
+    from alpacc_amd import hip
+    rng = random.Random(11)
+    names, gidx, modes, payloads, arrival = serve_trace(rng, 400, 5000.0)
+    srv = hip.Server(names, lanes=2)
+    try:
+        recs, lat = srv.replay(gidx, modes, payloads, arrival)
+        stats = srv.stats()
+    finally:
+        srv.close()
+    exp = b"".join(pyoracle.record(names[g], m, x) for g, m, x in zip(gidx, modes, payloads))
+    assert recs == exp
+    alone = b"".join(backends[names[g]].run_case(m, x) for g, m, x in zip(gidx, modes, payloads))
+    assert recs == alone
+    assert stats["requests"] == len(payloads)
+    print("server stats", stats, "p50 ms", float(np.percentile(lat, 50)) / 1e6, flush=True)
+
+
+def test_server_submit_wait_any_order():
+    from alpacc_amd import hip
+    srv = hip.Server(["sexp", "json"], lanes=2)
+    try:
+        items = [("sexp", "both", b"(a (b c))"), ("json", "lexer", b'[1, "x"]'), ("json", "both", b"[1,"),
+                 ("sexp", "parser", bytes([1, 0, 2])), ("json", "both", b"")]
+        tickets = [srv.submit(g, m, x) for g, m, x in items]
+        got = {t: srv.wait(t) for t in reversed(tickets)}
+        for t, (g, m, x) in zip(tickets, items):
+            assert got[t] == pyoracle.record(g, m, x), (g, m, x)
+    finally:
+        srv.close()
