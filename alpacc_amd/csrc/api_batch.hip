@@ -28,13 +28,6 @@ int stage_input(Context& c, const void* host, uint64_t bytes, hipStream_t st, co
 
 namespace {
 
-// Copy `bytes` from the device into a growing host vector.
-int pull(std::vector<uint8_t>& v, const void* d, size_t bytes, hipStream_t st) {
-  v.resize(bytes);
-  if (bytes) HIP_TRY(hipMemcpyAsync(v.data(), d, bytes, hipMemcpyDeviceToHost, st));
-  return 0;
-}
-
 // Lexer + parser over a batch in device memory: fills c.lb_* and c.pb_res.
 int lex_parse_batch(Context& c, const uint8_t* d_in, const uint64_t* off, uint64_t count, hipStream_t st) {
   int r = lex_batch_on_device(c, ALPACC_MODE_BOTH, d_in, off, count, st);
@@ -42,48 +35,80 @@ int lex_parse_batch(Context& c, const uint8_t* d_in, const uint64_t* off, uint64
   return parse_batch_on_device(c, c.lx_term.as<uint8_t>(), c.d_lb_tok, c.lb_tok, c.lb_flags, count, true, st);
 }
 
+// Device arrays copied into one pinned host buffer (c.h_out), in order.
+struct Pull {
+  const void* d;
+  size_t bytes;
+  const uint8_t* h = nullptr;
+};
+
+int pull_all(Context& c, Pull* v, int n, hipStream_t st) {
+  size_t tot = 0;
+  for (int i = 0; i < n; ++i) tot += (v[i].bytes + 255) & ~size_t(255);
+  int r;
+  if ((r = c.h_out.reserve(tot + 256))) return r;
+  uint8_t* h = c.h_out.as<uint8_t>();
+  for (int i = 0; i < n; ++i) {
+    if (v[i].bytes) HIP_TRY(hipMemcpyAsync(h, v[i].d, v[i].bytes, hipMemcpyDeviceToHost, st));
+    v[i].h = h;
+    h += (v[i].bytes + 255) & ~size_t(255);
+  }
+  HIP_TRY(hipStreamSynchronize(st));
+  return 0;
+}
+
 }  // namespace
 
-// Records of every input, concatenated.  `off` is rebased so that input i
-// is bytes [off[i] - off[0], off[i + 1] - off[0]) of `packed`.
-int batch_records(Context& c, int mode, const uint8_t* packed, const uint64_t* off_in, uint64_t count,
-                  std::vector<uint8_t>& out, std::vector<size_t>* rec_off) {
+// Records of every input, concatenated into one malloc'd buffer (*out,
+// *out_len); rec_off (if not null) receives the start of every record.
+// `off` is rebased so that input i is bytes [off[i] - off[0],
+// off[i + 1] - off[0]) of `packed`.
+int batch_records(Context& c, int mode, const uint8_t* packed, const uint64_t* off_in, uint64_t count, uint8_t** out,
+                  size_t* out_len, std::vector<size_t>* rec_off) {
   hipStream_t st = c.stream;
   const GrammarDesc* g = c.g;
+  *out = nullptr;
+  *out_len = 0;
   std::vector<uint64_t> off(count + 1);
   for (uint64_t i = 0; i <= count; ++i) off[i] = off_in[i] - off_in[0];
   const uint8_t* d_in = nullptr;
   int r;
   if ((r = stage_input(c, packed + off_in[0], off[count], st, &d_in))) return r;
-  out.clear();
-  if (rec_off) rec_off->clear();
+  if (rec_off) rec_off->resize(count);
   const int ib = g->index_bytes, lb = g->length_bytes;
+  auto alloc = [&](size_t n) -> uint8_t* {
+    uint8_t* p = static_cast<uint8_t*>(malloc(n ? n : 1));
+    if (p) {
+      *out = p;
+      *out_len = n;
+    }
+    return p;
+  };
   if (mode == ALPACC_MODE_LEXER) {
     if ((r = lex_batch_on_device(c, mode, d_in, off.data(), count, st)) < 0) return r;
     const uint64_t T = c.lb_tok[count];
     const int tb = g->terminal_bytes_lex;
-    std::vector<uint8_t> ht, hs, hl;
-    if ((r = pull(ht, c.lx_term.ptr, T * tb, st)) || (r = pull(hs, c.lx_start.ptr, T * 8, st)) ||
-        (r = pull(hl, c.lx_len.ptr, T * lb, st)))
-      return r;
-    HIP_TRY(hipStreamSynchronize(st));
+    Pull v[3] = {{c.lx_term.ptr, T * tb}, {c.lx_start.ptr, T * 8}, {c.lx_len.ptr, T * lb}};
+    if ((r = pull_all(c, v, 3, st))) return r;
+    size_t total = 0;
+    for (uint64_t i = 0; i < count; ++i)
+      total += c.lb_flags[i] ? 1 : 9 + (c.lb_tok[i + 1] - c.lb_tok[i]) * (tb + ib + lb);
+    uint8_t* o = alloc(total);
+    if (!o) return fail(ALPACC_ERR_NOMEM, "host allocation failed");
     for (uint64_t i = 0; i < count; ++i) {
-      if (rec_off) rec_off->push_back(out.size());
+      if (rec_off) (*rec_off)[i] = o - *out;
       if (c.lb_flags[i]) {
-        out.push_back(0);
+        *o++ = 0;
         continue;
       }
       const uint64_t a = c.lb_tok[i], n = c.lb_tok[i + 1] - a;
-      const size_t at = out.size();
-      out.resize(at + 9 + n * (tb + ib + lb));
-      uint8_t* o = out.data() + at;
       *o++ = 1;
       memcpy(o, &n, 8);
       o += 8;
       for (uint64_t k = a; k < a + n; ++k) {
-        memcpy(o, ht.data() + k * tb, tb);
-        memcpy(o + tb, hs.data() + k * 8, ib);
-        memcpy(o + tb + ib, hl.data() + k * lb, lb);
+        memcpy(o, v[0].h + k * tb, tb);
+        memcpy(o + tb, v[1].h + k * 8, ib);
+        memcpy(o + tb + ib, v[2].h + k * lb, lb);
         o += tb + ib + lb;
       }
     }
@@ -97,24 +122,29 @@ int batch_records(Context& c, int mode, const uint8_t* packed, const uint64_t* o
     HIP_TRY(hipMemcpyAsync(c.pmeta.ptr, h_off, (count + 1) * 8, hipMemcpyHostToDevice, st));
     if ((r = parse_batch_on_device(c, d_in, c.pmeta.as<uint64_t>(), h_off, nullptr, count, false, st)) < 0)
       return r;
-    std::vector<uint8_t> hd;
     uint64_t np = 0;
-    for (uint64_t i = 0; i < count; ++i)
-      if (!c.pb_res[i].flags) np = c.pb_res[i].slot_off + c.pb_res[i].slots > np ? c.pb_res[i].slot_off + c.pb_res[i].slots : np;
-    if ((r = pull(hd, c.ps_d.ptr, np, st))) return r;
-    HIP_TRY(hipStreamSynchronize(st));
+    size_t total = 0;
     for (uint64_t i = 0; i < count; ++i) {
       const ParseResult& o = c.pb_res[i];
-      if (rec_off) rec_off->push_back(out.size());
+      if (!o.flags && o.slot_off + o.slots > np) np = o.slot_off + o.slots;
+      total += o.flags ? 1 : 9 + o.slots;
+    }
+    Pull v[1] = {{c.ps_d.ptr, np}};
+    if ((r = pull_all(c, v, 1, st))) return r;
+    uint8_t* w = alloc(total);
+    if (!w) return fail(ALPACC_ERR_NOMEM, "host allocation failed");
+    for (uint64_t i = 0; i < count; ++i) {
+      const ParseResult& o = c.pb_res[i];
+      if (rec_off) (*rec_off)[i] = w - *out;
       if (o.flags) {
-        out.push_back(0);
+        *w++ = 0;
         continue;
       }
-      const size_t at = out.size();
-      out.resize(at + 9 + o.slots);
-      out[at] = 1;
-      memcpy(out.data() + at + 1, &o.slots, 8);
-      if (o.slots) memcpy(out.data() + at + 9, hd.data() + o.slot_off, o.slots);
+      *w++ = 1;
+      memcpy(w, &o.slots, 8);
+      w += 8;
+      memcpy(w, v[0].h + o.slot_off, o.slots);
+      w += o.slots;
     }
     return 0;
   }
@@ -123,49 +153,57 @@ int batch_records(Context& c, int mode, const uint8_t* packed, const uint64_t* o
   const uint64_t T = c.lb_tok[count];
   const int tb = g->terminal_bytes_both, pb = g->production_bytes;
   uint64_t nodes = 0, terms = 0;
+  size_t total = 0;
   for (uint64_t i = 0; i < count; ++i) {
     const ParseResult& o = c.pb_res[i];
-    if (c.lb_flags[i] || o.flags) continue;
-    nodes = o.node_off + o.nodes > nodes ? o.node_off + o.nodes : nodes;
-    const uint64_t te = o.term_off + (c.lb_tok[i + 1] - c.lb_tok[i]);
-    terms = te > terms ? te : terms;
-  }
-  std::vector<uint8_t> ht, hs, hl, hp, hpar, htp;
-  if ((r = pull(ht, c.lx_term.ptr, T * tb, st)) || (r = pull(hs, c.lx_start.ptr, T * 8, st)) ||
-      (r = pull(hl, c.lx_len.ptr, T * lb, st)) || (r = pull(hp, c.tr_prod.ptr, nodes * pb, st)) ||
-      (r = pull(hpar, c.tr_parent.ptr, nodes * 8, st)) || (r = pull(htp, c.tr_tokpar.ptr, terms * 8, st)))
-    return r;
-  HIP_TRY(hipStreamSynchronize(st));
-  for (uint64_t i = 0; i < count; ++i) {
-    const ParseResult& o = c.pb_res[i];
-    if (rec_off) rec_off->push_back(out.size());
     if (c.lb_flags[i] || o.flags) {
-      out.push_back(0);
+      total += 1;
+      continue;
+    }
+    const uint64_t n = c.lb_tok[i + 1] - c.lb_tok[i];
+    nodes = o.node_off + o.nodes > nodes ? o.node_off + o.nodes : nodes;
+    terms = o.term_off + n > terms ? o.term_off + n : terms;
+    total += 1 + 8 + n * (tb + ib + lb) + 8 + o.nodes * (pb + ib) + n * ib;
+  }
+  Pull v[6] = {{c.lx_term.ptr, T * tb},       {c.lx_start.ptr, T * 8},     {c.lx_len.ptr, T * lb},
+               {c.tr_prod.ptr, nodes * pb},   {c.tr_parent.ptr, nodes * 8}, {c.tr_tokpar.ptr, terms * 8}};
+  if ((r = pull_all(c, v, 6, st))) return r;
+  uint8_t* w = alloc(total);
+  if (!w) return fail(ALPACC_ERR_NOMEM, "host allocation failed");
+  for (uint64_t i = 0; i < count; ++i) {
+    const ParseResult& o = c.pb_res[i];
+    if (rec_off) (*rec_off)[i] = w - *out;
+    if (c.lb_flags[i] || o.flags) {
+      *w++ = 0;
       continue;
     }
     const uint64_t a = c.lb_tok[i], n = c.lb_tok[i + 1] - a, N = o.nodes;
-    const size_t at = out.size();
-    out.resize(at + 1 + 8 + n * (tb + ib + lb) + 8 + N * (pb + ib) + n * ib);
-    uint8_t* w = out.data() + at;
     *w++ = 1;
     memcpy(w, &n, 8);
     w += 8;
-    memcpy(w, ht.data() + a * tb, n * tb);
+    memcpy(w, v[0].h + a * tb, n * tb);
     w += n * tb;
     if (ib == 8) {
-      memcpy(w, hs.data() + a * 8, n * 8);
+      memcpy(w, v[1].h + a * 8, n * 8);
       w += n * 8;
     } else {
-      for (uint64_t k = 0; k < n; ++k, w += ib) memcpy(w, hs.data() + (a + k) * 8, ib);
+      for (uint64_t k = 0; k < n; ++k, w += ib) memcpy(w, v[1].h + (a + k) * 8, ib);
     }
-    memcpy(w, hl.data() + a * lb, n * lb);
+    memcpy(w, v[2].h + a * lb, n * lb);
     w += n * lb;
     memcpy(w, &N, 8);
     w += 8;
-    memcpy(w, hp.data() + o.node_off * pb, N * pb);
+    memcpy(w, v[3].h + o.node_off * pb, N * pb);
     w += N * pb;
-    for (uint64_t k = 0; k < N; ++k, w += ib) memcpy(w, hpar.data() + (o.node_off + k) * 8, ib);
-    for (uint64_t k = 0; k < n; ++k, w += ib) memcpy(w, htp.data() + (o.term_off + k) * 8, ib);
+    if (ib == 8) {
+      memcpy(w, v[4].h + o.node_off * 8, N * 8);
+      w += N * 8;
+      memcpy(w, v[5].h + o.term_off * 8, n * 8);
+      w += n * 8;
+    } else {
+      for (uint64_t k = 0; k < N; ++k, w += ib) memcpy(w, v[4].h + (o.node_off + k) * 8, ib);
+      for (uint64_t k = 0; k < n; ++k, w += ib) memcpy(w, v[5].h + (o.term_off + k) * 8, ib);
+    }
   }
   return 0;
 }
@@ -189,14 +227,15 @@ int alpacc_hip_run_batch(alpacc_hip_ctx* h, int mode, const uint8_t* packed, con
     if (offsets[i + 1] < offsets[i]) return fail(ALPACC_ERR_ARG, "offsets decrease at input %llu", (unsigned long long)i);
   Context& c = h->c;
   HIP_TRY(hipSetDevice(c.device));
-  std::vector<uint8_t> out;
-  int r = batch_records(c, mode, packed, offsets, count, out, nullptr);
-  if (r < 0) return r;
-  uint8_t* o = static_cast<uint8_t*>(malloc(out.size() ? out.size() : 1));
-  if (!o) return fail(ALPACC_ERR_NOMEM, "host allocation failed");
-  memcpy(o, out.data(), out.size());
-  *records = o;
-  *records_len = out.size();
+  uint8_t* out = nullptr;
+  size_t len = 0;
+  int r = batch_records(c, mode, packed, offsets, count, &out, &len, nullptr);
+  if (r < 0) {
+    free(out);
+    return r;
+  }
+  *records = out;
+  *records_len = len;
   return 0;
 }
 

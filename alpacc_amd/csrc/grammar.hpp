@@ -39,7 +39,8 @@ struct GrammarDesc {
   const uint8_t* prod_is_terminal;  // [nproductions]
   int max_brackets, max_prods;   // per window
   int max_slot_items;            // per window: productions + child slots they open
-  const uint8_t* pair_rec;       // [npairs * 64] window net effects (layout in emit_hip.py)
+  const uint8_t* pair_rec;       // [(npairs + 1) * 64] window net effects (layout in emit_hip.py);
+                                 // record npairs is all zero
   const uint8_t* prod_info;      // [nproductions] arity | terminal << 7
   int max_bpop, max_bpush;       // per window, after cancellation inside the window
   int max_spop, max_spush;

@@ -69,7 +69,8 @@ struct Seg {
   uint32_t count, ntiles;
   int q, k, base, empty_t, start_t, end_t;
   const uint16_t* window;
-  const uint4* rec;  // 64-byte pair records, 4 x uint4 each
+  const uint4* rec;  // 64-byte pair records, 4 x uint4 each; record npairs is empty
+  int npairs;
   uint32_t* flags;   // [count] BAD_* bits
 };
 
@@ -147,55 +148,46 @@ struct SegBase {
 __device__ __forceinline__ uint32_t hb(const uint4& r0, int i) {
   return ((i < 4 ? r0.x : r0.y) >> (8 * (i & 3))) & 0xFFu;
 }
-// Bracket symbol j (0..7).
-__device__ __forceinline__ uint32_t rec_br(const uint4& r0, const uint4& r1, int j) {
-  const uint64_t w = j < 4 ? (((uint64_t)r0.w << 32) | r0.z) : (((uint64_t)r1.y << 32) | r1.x);
-  return (uint32_t)(w >> (16 * (j & 3))) & 0xFFFFu;
-}
-__device__ __forceinline__ uint32_t rec_prod(const uint4& r1, int j) {
-  return (uint32_t)((((uint64_t)r1.w << 32) | r1.z) >> (8 * j)) & 0xFFu;
-}
-__device__ __forceinline__ uint32_t rec_parent(const uint4& r2, int j) {
-  return (uint32_t)((((uint64_t)r2.y << 32) | r2.x) >> (8 * j)) & 0xFFu;
-}
-// Owner of surviving child slot e (0..23).
-__device__ __forceinline__ uint32_t rec_owner(const uint4& r2, const uint4& r3, int e) {
-  uint32_t w;
-  switch (e >> 2) {
-    case 0: w = r2.z; break;
-    case 1: w = r2.w; break;
-    case 2: w = r3.x; break;
-    case 3: w = r3.y; break;
-    case 4: w = r3.z; break;
-    default: w = r3.w; break;
-  }
-  return (w >> (8 * (e & 3))) & 0xFFu;
-}
-
 // ---------------------------------------------------------------- pass A
 
-__global__ __launch_bounds__(PT) void seg_profile(Seg s, TileRec* rec) {
+// APL consecutive positions per lane, PT / APL lanes per tile.
+constexpr int APL = 4;
+constexpr int AWG = PT / APL;
+constexpr int ANW = AWG / WAVE;
+
+__global__ __launch_bounds__(AWG) void seg_profile(Seg s, TileRec* rec, uint16_t* pid) {
   const uint32_t tile = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const TileOf t = tile_of(s, tile);
-  const int64_t p = (int64_t)t.lt * PT + tid;
+  const int64_t p0 = (int64_t)t.lt * PT + (int64_t)tid * APL;
   int32_t np = 0, nt = 0;
   SM b{0, 0}, sl{0, 0};
   bool bad = false;
-  if (p < t.m) {
-    const uint32_t id = pair_at(s, t, p);
-    if (id == NO_PAIR) {
-      bad = true;
-    } else {
-      const uint4 r0 = s.rec[4 * id];
-      const int32_t bpop = hb(r0, 0), bpush = hb(r0, 1), spop = hb(r0, 4), spush = hb(r0, 5);
-      np = hb(r0, 2);
-      nt = hb(r0, 3);
-      b = SM{bpush - bpop, -bpop};
-      sl = SM{spush - spop, -spop};
+  uint32_t ids[APL];
+#pragma unroll
+  for (int r = 0; r < APL; ++r) {
+    const int64_t p = p0 + r;
+    uint32_t id = NO_PAIR;
+    if (p < t.m) {
+      id = pair_at(s, t, p);
+      if (id == NO_PAIR) {
+        bad = true;
+      } else {
+        const uint2 h = *reinterpret_cast<const uint2*>(s.rec + 4 * id);
+        const int32_t bpop = h.x & 0xFF, bpush = (h.x >> 8) & 0xFF, spop = h.y & 0xFF, spush = (h.y >> 8) & 0xFF;
+        np += (h.x >> 16) & 0xFF;
+        nt += h.x >> 24;
+        b = sm_cat(b, SM{bpush - bpop, -bpop});
+        sl = sm_cat(sl, SM{spush - spop, -spop});
+      }
     }
+    ids[r] = id;
   }
-  __shared__ int32_t wc[2][PNW];
-  __shared__ SM wb[PNW], ws[PNW];
+  uint2 v;
+  v.x = ids[0] | (ids[1] << 16);
+  v.y = ids[2] | (ids[3] << 16);
+  *reinterpret_cast<uint2*>(pid + (uint64_t)tile * PT + (uint64_t)tid * APL) = v;
+  __shared__ int32_t wc[2][ANW];
+  __shared__ SM wb[ANW], ws[ANW];
   __shared__ uint32_t s_bad;
   if (tid == 0) s_bad = 0;
   const uint32_t cp = wave_scan_add32((uint32_t)np), ct = wave_scan_add32((uint32_t)nt);
@@ -212,7 +204,7 @@ __global__ __launch_bounds__(PT) void seg_profile(Seg s, TileRec* rec) {
   if (tid == 0) {
     TileRec r{0, 0, 0, 0, 0, 0, 0, 0};
     SM tb{0, 0}, tsm{0, 0};
-    for (int w = 0; w < PNW; ++w) {
+    for (int w = 0; w < ANW; ++w) {
       r.np += wc[0][w];
       r.nt += wc[1][w];
       tb = sm_cat(tb, wb[w]);
@@ -354,6 +346,7 @@ __global__ __launch_bounds__(256) void tile_downsweep(Seg s, const TileRec* rec,
 
 struct Resolve {
   Seg s;
+  const uint16_t* pid;  // [ntiles * PT] pair ids from pass A
   const TileInfo* info;
   const SegBase* sb;
   int mbpush, mspush;  // rows of the LDS push lists (grammar maxima)
@@ -439,18 +432,16 @@ __global__ __launch_bounds__(PT) void seg_resolve(Resolve r) {
   const TileOf t = tile_of(s, tile);
   const TileInfo ti = r.info[tile];
   const SegBase in = r.sb[t.seg];
-  const int64_t p = (int64_t)t.lt * PT + tid;
 
-  uint4 r0 = make_uint4(0, 0, 0, 0), r1 = r0, r2 = r0, r3 = r0;
-  if (p < t.m) {
-    const uint32_t id = pair_at(s, t, p);  // every position has a pair: pass A rejected the input otherwise
-    if (id != NO_PAIR) {
-      r0 = s.rec[4 * id];
-      r1 = s.rec[4 * id + 1];
-      r2 = s.rec[4 * id + 2];
-      r3 = s.rec[4 * id + 3];
-    }
-  }
+  // pair id from pass A (NO_PAIR past the input's end); a position without
+  // a pair rejected the input in pass A and gets the empty record here
+  const uint32_t id = r.pid[(uint64_t)tile * PT + tid];
+  // the counts stay in registers; fields picked by a loop index are read
+  // from the record itself (a few KB of table, cache-resident)
+  const uint4* rp = s.rec + 4 * (id == NO_PAIR ? (uint32_t)s.npairs : id);
+  const uint4 r0 = rp[0];
+  const uint16_t* rbr = reinterpret_cast<const uint16_t*>(rp) + 4;  // bracket symbols
+  const uint8_t* rb = reinterpret_cast<const uint8_t*>(rp);         // 24 productions, 32 parents, 40 owners
   const int32_t nbpop = hb(r0, 0), nbpush = hb(r0, 1), np = hb(r0, 2), nt = hb(r0, 3);
   const int32_t nspop = hb(r0, 4), nspush = hb(r0, 5);
   const uint32_t tmask = hb(r0, 6);
@@ -464,8 +455,10 @@ __global__ __launch_bounds__(PT) void seg_resolve(Resolve r) {
   const int32_t m_b = bD - nbpop, m_s = sD - nspop;
   tbb[0][tid] = (int16_t)m_b;
   tbs[0][tid] = (int16_t)m_s;
-  for (int e = 0; e < nbpush; ++e) bpl[e * PT + tid] = (uint16_t)rec_br(r0, r1, nbpop + e);
-  for (int e = 0; e < nspush; ++e) spl[e * PT + tid] = (uint16_t)(cbase + rec_owner(r2, r3, e));
+  #pragma unroll 1
+  for (int e = 0; e < nbpush; ++e) bpl[e * PT + tid] = rbr[nbpop + e];
+  #pragma unroll 1
+  for (int e = 0; e < nspush; ++e) spl[e * PT + tid] = (uint16_t)(cbase + rb[40 + e]);
   const int32_t bsuf = suffix_min_after(m_b, wmin[0]);
   const int32_t ssuf = suffix_min_after(m_s, wmin[1]);
   // level 0 and the push lists are visible: suffix_min_after synchronised
@@ -488,24 +481,27 @@ __global__ __launch_bounds__(PT) void seg_resolve(Resolve r) {
   const int64_t c_in = in.p_off - in.t_off;   // ... of the input's first node
   // brackets: pops (depth falling, so the matching position only moves back)
   int l = (int)tid - 1;
+  #pragma unroll 1
   for (int e = 0; e < nbpop; ++e) {
     const int32_t L = bD - e;
-    const uint32_t sym = rec_br(r0, r1, e);
+    const uint32_t sym = rbr[e];
     l = prev_below(tbb, l, L);
     if (l >= 0)
       bad |= bpl[(L - tbb[0][l] - 1) * PT + l] != sym;
     else
       r.ub_list[ti.ub_off + (-L)] = (uint16_t)sym;
   }
+  #pragma unroll 1
   for (int e = 0; e < nbpush; ++e) {
     const int32_t lv = m_b + 1 + e;
-    if (lv <= bsuf) r.ob_list[ti.ob_off + (lv - ti.bmin - 1)] = (uint16_t)rec_br(r0, r1, nbpop + e);
+    if (lv <= bsuf) r.ob_list[ti.ob_off + (lv - ti.bmin - 1)] = rbr[nbpop + e];
   }
   // productions and their parents
   l = (int)tid - 1;
   int32_t tseen = 0;
+  #pragma unroll 1
   for (int j = 0; j < np; ++j) {
-    const uint32_t pr = rec_prod(r1, j);
+    const uint32_t pr = rb[24 + j];
     const bool term = (tmask >> j) & 1u;
     const int32_t c = cbase + j - tseen;   // tile-local compact index (nodes)
     const int32_t tl = term0 + tseen;      // tile-local terminal production
@@ -515,7 +511,7 @@ __global__ __launch_bounds__(PT) void seg_resolve(Resolve r) {
       continue;
     }
     if (!term) s_prod[c] = (uint8_t)pr;
-    const uint32_t code = rec_parent(r2, j);
+    const uint32_t code = rb[32 + j];
     int32_t par;
     if (code & 0x80u) {
       const int32_t L = sD - (int32_t)(code & 0x7Fu);
@@ -534,6 +530,7 @@ __global__ __launch_bounds__(PT) void seg_resolve(Resolve r) {
       s_par[c] = (uint16_t)par;
   }
   if (r.tree) {
+    #pragma unroll 1
     for (int e = 0; e < nspush; ++e) {
       const int32_t lv = m_s + 1 + e;
       if (lv <= ssuf) r.os_list[ti.os_off + (lv - ti.smin - 1)] = c_off + spl[e * PT + tid];
@@ -548,12 +545,15 @@ __global__ __launch_bounds__(PT) void seg_resolve(Resolve r) {
   __syncthreads();
   if (r.tree) {
     const int32_t nn = tile_np - tile_nt;
+    #pragma unroll 1
     for (int32_t i = tid; i < nn; i += PT) {
       r.tree_prod[c_off + i] = s_prod[i];
       r.tree_par[c_off + i] = c_off + s_par[i] - c_in;
     }
+    #pragma unroll 1
     for (int32_t i = tid; i < tile_nt; i += PT) r.tok_par[ti.t_off + i] = c_off + s_tpar[i] - c_in;
   } else {
+    #pragma unroll 1
     for (int32_t i = tid; i < tile_np; i += PT) r.deriv[ti.p_off + i] = s_prod[i];
   }
 }
@@ -782,7 +782,8 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
   const size_t off_out = al(off_bs + (size_t)(nsb ? nsb : 1) * NV * 8);
   const size_t off_res = al(off_out + sizeof(ScanOut));
   const size_t off_fl = al(off_res + count * sizeof(ParseResult));
-  const size_t total = off_fl + count * 4 + 64;
+  const size_t off_pid = al(off_fl + count * 4);
+  const size_t total = off_pid + ntiles * PT * 2 + 64;
   if ((r = c.ps_a.reserve(total)) || (r = c.h_pres.reserve(count * sizeof(ParseResult) + sizeof(ScanOut) + 64)))
     return r;
   uint8_t* base = c.ps_a.as<uint8_t>();
@@ -801,6 +802,7 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
   s.end_t = g->end_t;
   s.window = c.d_window;
   s.rec = reinterpret_cast<const uint4*>(c.d_pair_rec);
+  s.npairs = g->npairs;
   s.flags = reinterpret_cast<uint32_t*>(base + off_fl);
   TileRec* rec = reinterpret_cast<TileRec*>(base + off_rec);
   TileInfo* info = reinterpret_cast<TileInfo*>(base + off_info);
@@ -819,7 +821,8 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
     hipLaunchKernelGGL(tile_map, dim3((uint32_t)((ntiles + 255) / 256)), dim3(256), 0, st, s.tile_first, s.count,
                        s.ntiles, const_cast<uint32_t*>(s.tile_seg));
     HIP_TRY(tm.begin(st, "pair_profile"));
-    hipLaunchKernelGGL(seg_profile, dim3((uint32_t)ntiles), dim3(PT), 0, st, s, rec);
+    hipLaunchKernelGGL(seg_profile, dim3((uint32_t)ntiles), dim3(AWG), 0, st, s, rec,
+                       reinterpret_cast<uint16_t*>(base + off_pid));
     HIP_TRY(tm.end(st));
     HIP_TRY(tm.begin(st, "tile_scan"));
     hipLaunchKernelGGL(tile_block_sums, dim3(nsb), dim3(256), 0, st, s, rec, bsum);
@@ -851,7 +854,8 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
         (r = build_tile_tree(ms, ntiles, reinterpret_cast<int32_t*>(base + off_tsx), st, &ts)))
       return r;
     HIP_TRY(tm.end(st));
-    Resolve rs{s,         info,      sb,       g->max_bpush,  g->max_spush, ncap,
+    Resolve rs{s,         reinterpret_cast<const uint16_t*>(base + off_pid),
+               info,      sb,       g->max_bpush,  g->max_spush, ncap,
                ob_list,   ub_list,   os_list,  us_list,       c.tr_prod.as<uint8_t>(),
                c.tr_parent.as<int64_t>(), c.tr_tokpar.as<int64_t>(), tree ? nullptr : c.ps_d.as<uint8_t>(),
                tree ? 1 : 0};

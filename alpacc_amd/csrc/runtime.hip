@@ -141,7 +141,7 @@ int alpacc_hip_open(const char* grammar, int device, alpacc_hip_ctx** out) {
       (r = upload(&c.d_window, g->window, (size_t)g->nwindow)) ||
       (r = upload(&c.d_spans, g->spans, (size_t)g->npairs * 4)) ||
       (r = upload(&c.d_profile, g->profile, (size_t)g->npairs * 8)) ||
-      (r = upload(&c.d_pair_rec, g->pair_rec, (size_t)g->npairs * 64)) ||
+      (r = upload(&c.d_pair_rec, g->pair_rec, (size_t)(g->npairs + 1) * 64)) ||
       (r = upload(&c.d_prod_info, g->prod_info, (size_t)g->nproductions)) ||
       (r = upload(&c.d_stacks, g->stacks, (size_t)g->nstacks)) ||
       (r = upload(&c.d_productions, g->productions, (size_t)g->nprods_flat)) ||
@@ -176,6 +176,7 @@ void alpacc_hip_close(alpacc_hip_ctx* h) {
   c.h_meta.release();
   c.h_pmeta.release();
   c.h_pres.release();
+  c.h_out.release();
   if (c.stream) (void)hipStreamDestroy(c.stream);
   delete h;
 }

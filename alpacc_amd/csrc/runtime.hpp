@@ -95,6 +95,7 @@ struct Context {
   DevBuf pmeta;       // offsets of single-string parses
   PinnedBuf h_pmeta;  // tile_first of the parser tiles
   PinnedBuf h_pres;   // per-input results read back
+  PinnedBuf h_out;    // batch outputs read back for the wire records
   uint64_t pb_count = 0;
   const struct ParseResult* pb_res = nullptr;
   // host arrays of the last device batch (alpacc_hip_lex_parse_batch_device)
@@ -132,10 +133,10 @@ const char* const* grammar_names();
 int lex_batch_on_device(Context& c, int mode, const uint8_t* d_in, const uint64_t* h_off, uint64_t count,
                         hipStream_t st);
 
-// Wire records of a batch of host payloads, concatenated; rec_off (if not
-// null) receives the start of every input's record.
-int batch_records(Context& c, int mode, const uint8_t* packed, const uint64_t* off, uint64_t count,
-                  std::vector<uint8_t>& out, std::vector<size_t>* rec_off);
+// Wire records of a batch of host payloads, concatenated into one malloc'd
+// buffer; rec_off (if not null) receives the start of every input's record.
+int batch_records(Context& c, int mode, const uint8_t* packed, const uint64_t* off, uint64_t count, uint8_t** out,
+                  size_t* out_len, std::vector<size_t>* rec_off);
 
 // One input: returns 1/0 (valid / rejected) and fills count.
 int lex_on_device(Context& c, int mode, const uint8_t* d_in, uint64_t n, hipStream_t st, uint64_t* count);

@@ -39,7 +39,8 @@ struct Req {
   uint64_t n = 0;
   std::vector<uint8_t> own;  // payload copy of a submitted request
   uint64_t est = 0;          // device bytes admitted for it
-  std::vector<uint8_t> rec;
+  std::shared_ptr<uint8_t> buf;  // its batch's records; this one is [rec_at, rec_at + rec_len)
+  size_t rec_at = 0, rec_len = 0;
   int status = 0;  // 0 waiting, 1 answered, < 0 error
   std::string err;
   Clock::time_point done;
@@ -88,7 +89,7 @@ int pick(alpacc_hip_server* s, uint32_t g) {
 void worker(alpacc_hip_server* s, uint32_t g, alpacc_hip_ctx* ctx) {
   (void)hipSetDevice(s->device);
   std::vector<Req*> batch;
-  std::vector<uint8_t> packed, out;
+  std::vector<uint8_t> packed;
   std::vector<uint64_t> off;
   std::vector<size_t> rec_off;
   for (;;) {
@@ -116,8 +117,11 @@ void worker(alpacc_hip_server* s, uint32_t g, alpacc_hip_ctx* ctx) {
       packed.insert(packed.end(), r->data, r->data + r->n);
       off.push_back(packed.size());
     }
-    const int rc = alpacc::batch_records(ctx->c, mode, packed.data(), off.data(), batch.size(), out, &rec_off);
+    uint8_t* out = nullptr;
+    size_t out_len = 0;
+    const int rc = alpacc::batch_records(ctx->c, mode, packed.data(), off.data(), batch.size(), &out, &out_len, &rec_off);
     const std::string err = rc < 0 ? alpacc_hip_last_error() : "";
+    std::shared_ptr<uint8_t> shared(out, free);
     const Clock::time_point now = Clock::now();
     {
       std::lock_guard<std::mutex> lk(s->mu);
@@ -127,8 +131,10 @@ void worker(alpacc_hip_server* s, uint32_t g, alpacc_hip_ctx* ctx) {
           r->status = rc;
           r->err = err;
         } else {
-          const size_t a = rec_off[i], b = i + 1 < batch.size() ? rec_off[i + 1] : out.size();
-          r->rec.assign(out.begin() + a, out.begin() + b);
+          const size_t a = rec_off[i], b = i + 1 < batch.size() ? rec_off[i + 1] : out_len;
+          r->buf = shared;
+          r->rec_at = a;
+          r->rec_len = b - a;
           r->status = 1;
         }
         r->done = now;
@@ -229,11 +235,11 @@ int alpacc_hip_server_wait(alpacc_hip_server* s, uint64_t ticket, uint8_t** reco
     s->tickets.erase(it);
   }
   if (r->status < 0) return alpacc::fail(r->status, "%s", r->err.c_str());
-  uint8_t* o = static_cast<uint8_t*>(malloc(r->rec.size() ? r->rec.size() : 1));
+  uint8_t* o = static_cast<uint8_t*>(malloc(r->rec_len ? r->rec_len : 1));
   if (!o) return alpacc::fail(ALPACC_ERR_NOMEM, "host allocation failed");
-  memcpy(o, r->rec.data(), r->rec.size());
+  if (r->rec_len) memcpy(o, r->buf.get() + r->rec_at, r->rec_len);
   *record = o;
-  *record_len = r->rec.size();
+  *record_len = r->rec_len;
   return 0;
 }
 
@@ -273,7 +279,7 @@ int alpacc_hip_server_replay(alpacc_hip_server* s, uint64_t count, const int32_t
   size_t total = 0;
   for (uint64_t i = 0; i < count; ++i) {
     if (reqs[i].status < 0) return alpacc::fail(reqs[i].status, "%s", reqs[i].err.c_str());
-    total += reqs[i].rec.size();
+    total += reqs[i].rec_len;
     latency_ns[i] = (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
                         reqs[i].done - (t0 + std::chrono::nanoseconds(arrival_ns[i])))
                         .count();
@@ -282,8 +288,8 @@ int alpacc_hip_server_replay(alpacc_hip_server* s, uint64_t count, const int32_t
   if (!o) return alpacc::fail(ALPACC_ERR_NOMEM, "host allocation failed");
   size_t at = 0;
   for (uint64_t i = 0; i < count; ++i) {
-    memcpy(o + at, reqs[i].rec.data(), reqs[i].rec.size());
-    at += reqs[i].rec.size();
+    if (reqs[i].rec_len) memcpy(o + at, reqs[i].buf.get() + reqs[i].rec_at, reqs[i].rec_len);
+    at += reqs[i].rec_len;
   }
   *records = o;
   *records_len = total;

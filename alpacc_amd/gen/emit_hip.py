@@ -177,6 +177,7 @@ def device_tables(g: Grammar) -> dict:
         rec += prods + [0] * (8 - len(prods))
         rec += parent + [0] * (8 - len(parent))
         rec += slots + [0] * (24 - len(slots))
+    rec += [0] * 64  # record npairs: no effect (positions past an input's end)
     prod_info = []
     for prod in range(p.n_productions):
         if p.arity[prod] > 127:
