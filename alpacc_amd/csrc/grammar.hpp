@@ -44,6 +44,7 @@ struct GrammarDesc {
   const uint8_t* prod_info;      // [nproductions] arity | terminal << 7
   int max_bpop, max_bpush;       // per window, after cancellation inside the window
   int max_spop, max_spush;
+  int max_nt;                    // terminal productions per window
 };
 
 const GrammarDesc* find_grammar(const char* name);

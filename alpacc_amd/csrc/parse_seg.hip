@@ -49,9 +49,7 @@ namespace alpacc {
 
 namespace {
 
-constexpr int PT = 512;          // framed positions per tile = lanes per workgroup
-constexpr int PNW = PT / WAVE;   // waves per workgroup
-constexpr int PLV = 9;           // sparse-table levels: blocks of 1..256 positions
+constexpr int PT = 1024;         // framed positions per tile
 // Reject reasons.  parse_test (parser.c:49-98) rejects on pairs and brackets
 // only; the tree checks add the terminal count (compact_tree, parser.c:
 // 126-153) and the derivation-slot structure.
@@ -150,7 +148,7 @@ __device__ __forceinline__ uint32_t hb(const uint4& r0, int i) {
 }
 // ---------------------------------------------------------------- pass A
 
-// APL consecutive positions per lane, PT / APL lanes per tile.
+// Both tile passes: APL consecutive positions per lane, PT / APL lanes.
 constexpr int APL = 4;
 constexpr int AWG = PT / APL;
 constexpr int ANW = AWG / WAVE;
@@ -344,13 +342,18 @@ __global__ __launch_bounds__(256) void tile_downsweep(Seg s, const TileRec* rec,
 
 // ---------------------------------------------------------------- pass B
 
+constexpr int RWG = AWG;          // lanes per tile
+constexpr int RNW = RWG / WAVE;   // waves per tile
+constexpr int RLV = 8;            // sparse-table levels: blocks of 1..128 lanes
+constexpr uint8_t TERM_SLOT = 0x80u;
+
 struct Resolve {
   Seg s;
   const uint16_t* pid;  // [ntiles * PT] pair ids from pass A
   const TileInfo* info;
   const SegBase* sb;
-  int mbpush, mspush;  // rows of the LDS push lists (grammar maxima)
-  int ncap;            // staged slots per tile: PT * max productions per window
+  int bcap, scap;      // per-lane stack room: APL * (pops + pushes) of a window, brackets and slots
+  int ncap, tcap;      // staged nodes / terminal productions per tile
   uint16_t* ob_list;   // open pushes (bracket symbols)
   uint16_t* ub_list;   // unmatched pops (bracket symbols)
   int64_t* os_list;    // open child slots (owner's compact index, all inputs)
@@ -362,20 +365,20 @@ struct Resolve {
   int tree;
 };
 
-// Last position l <= from whose post-pop depth (tb[0][l]) is below L, or a
-// negative value: tb[k][l] is the minimum over [l - 2^k + 1, l] (clipped at
-// 0) and maximal blocks with minimum >= L are skipped, largest first.
-__device__ __forceinline__ int prev_below(const int16_t (*tb)[PT], int from, int32_t L) {
+// Last lane l <= from whose minimum (tb[0][l]) is below L, or a negative
+// value: tb[k][l] is the minimum over lanes [l - 2^k + 1, l] (clipped at 0)
+// and maximal blocks with minimum >= L are skipped, largest first.
+__device__ __forceinline__ int prev_below(const int16_t (*tb)[RWG], int from, int32_t L) {
   int l = from;
   if (l < 0 || tb[0][l] < L) return l;
 #pragma unroll
-  for (int k = PLV - 1; k >= 0; --k)
+  for (int k = RLV - 1; k >= 0; --k)
     if (l >= 0 && tb[k][l] >= L) l -= 1 << k;
   return l;
 }
 
 // Block-wide exclusive scans of four int32 values.
-__device__ __forceinline__ void block_scan4(int32_t* v, int32_t* tot, int32_t (*ws)[PNW]) {
+__device__ __forceinline__ void block_scan4(int32_t* v, int32_t* tot, int32_t (*ws)[RNW]) {
   const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   int32_t inc[4];
 #pragma unroll
@@ -387,7 +390,7 @@ __device__ __forceinline__ void block_scan4(int32_t* v, int32_t* tot, int32_t (*
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     int32_t o = 0, t = 0;
-    for (uint32_t w = 0; w < (uint32_t)PNW; ++w) {
+    for (uint32_t w = 0; w < (uint32_t)RNW; ++w) {
       if (w < wid) o += ws[k][w];
       t += ws[k][w];
     }
@@ -409,65 +412,147 @@ __device__ __forceinline__ int32_t suffix_min_after(int32_t v, int32_t* wtot) {
   __syncthreads();
   int32_t ex = __shfl_down(inc, 1, WAVE);
   if (lane == WAVE - 1) ex = 0x7FFFFFFF;
-  for (uint32_t w = wid + 1; w < (uint32_t)PNW; ++w) ex = wtot[w] < ex ? wtot[w] : ex;
+  for (uint32_t w = wid + 1; w < (uint32_t)RNW; ++w) ex = wtot[w] < ex ? wtot[w] : ex;
   return ex;
 }
 
-__global__ __launch_bounds__(PT) void seg_resolve(Resolve r) {
-  // dynamic LDS: bracket push symbols [mbpush][PT], child-slot owners
-  // [mspush][PT] (tile-local compact index), staged parents of nodes [ncap]
-  // and of terminal productions [ncap] (tile-local compact index), staged
-  // productions [ncap]
+// Every lane runs its APL positions' window effects in order on two short
+// stacks in LDS -- bracket symbols, and open child slots as lane-relative
+// owner indices -- interleaved by lane (entry e of lane t at e * RWG + t, so
+// a wave's lanes touch distinct banks).  Pops that find the lane's stack
+// empty are recorded from the top entry down.  A lane's open entries sit at
+// consecutive levels just above its minimum, so a pop left over by a later
+// lane at tile-relative level D reads entry D - min - 1 of the last earlier
+// lane whose minimum is below D (a sparse table over the lanes' minima).
+__global__ __launch_bounds__(RWG) void seg_resolve(Resolve r) {
+  // dynamic LDS: bracket stacks u16 [bcap][RWG], slot stacks u8 [scap][RWG],
+  // staged parents of nodes u16 [ncap] and of terminal productions u16
+  // [tcap] (tile-local compact indices), staged productions u8 [ncap]
   extern __shared__ __align__(16) uint16_t dyn[];
-  __shared__ int16_t tbb[PLV][PT], tbs[PLV][PT];
-  __shared__ int32_t ws[4][PNW];
-  __shared__ int32_t wmin[2][PNW];
+  __shared__ int16_t tbb[RLV][RWG], tbs[RLV][RWG];
+  __shared__ uint16_t l_cb[RWG], l_tb[RWG];  // each lane's first node / terminal production (tile-local)
+  __shared__ int32_t ws[4][RNW];
+  __shared__ int32_t wmin[2][RNW];
   const Seg& s = r.s;
   const uint32_t tile = blockIdx.x, tid = threadIdx.x;
-  uint16_t* const bpl = dyn;
-  uint16_t* const spl = bpl + r.mbpush * PT;
-  uint16_t* const s_par = spl + r.mspush * PT;
+  const int bcap = r.bcap, scap = r.scap;
+  uint16_t* const bst = dyn + tid;
+  uint8_t* const sst = reinterpret_cast<uint8_t*>(dyn + bcap * RWG) + tid;
+  uint16_t* const s_par = dyn + bcap * RWG + (scap * RWG + 1) / 2;
   uint16_t* const s_tpar = s_par + r.ncap;
-  uint8_t* const s_prod = reinterpret_cast<uint8_t*>(s_tpar + r.ncap);
-  const TileOf t = tile_of(s, tile);
+  uint8_t* const s_prod = reinterpret_cast<uint8_t*>(s_tpar + r.tcap);
+  const uint32_t seg = s.tile_seg[tile];
   const TileInfo ti = r.info[tile];
-  const SegBase in = r.sb[t.seg];
+  const SegBase in = r.sb[seg];
 
-  // pair id from pass A (NO_PAIR past the input's end); a position without
-  // a pair rejected the input in pass A and gets the empty record here
-  const uint32_t id = r.pid[(uint64_t)tile * PT + tid];
-  // the counts stay in registers; fields picked by a loop index are read
-  // from the record itself (a few KB of table, cache-resident)
-  const uint4* rp = s.rec + 4 * (id == NO_PAIR ? (uint32_t)s.npairs : id);
-  const uint4 r0 = rp[0];
-  const uint16_t* rbr = reinterpret_cast<const uint16_t*>(rp) + 4;  // bracket symbols
-  const uint8_t* rb = reinterpret_cast<const uint8_t*>(rp);         // 24 productions, 32 parents, 40 owners
-  const int32_t nbpop = hb(r0, 0), nbpush = hb(r0, 1), np = hb(r0, 2), nt = hb(r0, 3);
-  const int32_t nspop = hb(r0, 4), nspush = hb(r0, 5);
-  const uint32_t tmask = hb(r0, 6);
-
-  // exclusive scans: slots, terminal productions, bracket depth, slot depth
-  int32_t v[4] = {np, nt, nbpush - nbpop, nspush - nspop}, tot[4];
-  block_scan4(v, tot, ws);
-  const int32_t slot0 = v[0], term0 = v[1], bD = v[2], sD = v[3];
-  const int32_t tile_np = tot[0], tile_nt = tot[1];
-  const int32_t cbase = slot0 - term0;  // tile-local compact index of the position's first node
-  const int32_t m_b = bD - nbpop, m_s = sD - nspop;
-  tbb[0][tid] = (int16_t)m_b;
-  tbs[0][tid] = (int16_t)m_s;
-  #pragma unroll 1
-  for (int e = 0; e < nbpush; ++e) bpl[e * PT + tid] = rbr[nbpop + e];
-  #pragma unroll 1
-  for (int e = 0; e < nspush; ++e) spl[e * PT + tid] = (uint16_t)(cbase + rb[40 + e]);
-  const int32_t bsuf = suffix_min_after(m_b, wmin[0]);
-  const int32_t ssuf = suffix_min_after(m_s, wmin[1]);
-  // level 0 and the push lists are visible: suffix_min_after synchronised
+  // phase 0: records of the lane's positions, lane sums and offsets
+  const uint2 pv = *reinterpret_cast<const uint2*>(r.pid + (uint64_t)tile * PT + (uint64_t)tid * APL);
+  const uint32_t ids[APL] = {pv.x & 0xFFFFu, pv.x >> 16, pv.y & 0xFFFFu, pv.y >> 16};
+  const uint8_t* rb[APL];
+  uint4 h[APL];
+  int32_t np = 0, nt = 0, bd = 0, sd = 0;
 #pragma unroll
-  for (int k = 1; k < PLV; ++k) {
-    const int h = 1 << (k - 1);
+  for (int k = 0; k < APL; ++k) {
+    const uint4* rp = s.rec + 4 * (ids[k] == NO_PAIR ? (uint32_t)s.npairs : ids[k]);
+    rb[k] = reinterpret_cast<const uint8_t*>(rp);
+    h[k] = rp[0];
+    np += (h[k].x >> 16) & 0xFF;
+    nt += h[k].x >> 24;
+    bd += (int32_t)((h[k].x >> 8) & 0xFF) - (int32_t)(h[k].x & 0xFF);
+    sd += (int32_t)((h[k].y >> 8) & 0xFF) - (int32_t)(h[k].y & 0xFF);
+  }
+  int32_t v[4] = {np, nt, bd, sd}, tot[4];
+  block_scan4(v, tot, ws);
+  const int32_t slot0 = v[0], term0 = v[1], bS = v[2], sS = v[3];
+  const int32_t tile_np = tot[0], tile_nt = tot[1];
+  const int32_t cb0 = slot0 - term0;  // tile-local compact index of the lane's first node
+  l_cb[tid] = (uint16_t)cb0;
+  l_tb[tid] = (uint16_t)term0;
+
+  // phase 1: the lane's positions in order
+  const int64_t c_off = ti.p_off - ti.t_off;  // compact index of the tile's first node (all inputs)
+  const int64_t c_in = in.p_off - in.t_off;   // ... of the input's first node
+  int bsp = 0, bun = 0, ssp = 0, sun = 0;
+  int32_t bdep = 0, bmin = 0, sdep = 0, smin = 0;
+  int32_t slot = slot0, tinc = term0;
+  bool bad = false;
+#pragma unroll
+  for (int k = 0; k < APL; ++k) {
+    const uint32_t hx = h[k].x, hy = h[k].y;
+    const int nbpop = hx & 0xFF, nbpush = (hx >> 8) & 0xFF, npk = (hx >> 16) & 0xFF, ntk = hx >> 24;
+    const int nspush = (hy >> 8) & 0xFF;
+    const uint32_t tmask = (hy >> 16) & 0xFF;
+    const uint16_t* rbr = reinterpret_cast<const uint16_t*>(rb[k]) + 4;
+#pragma unroll 1
+    for (int e = 0; e < nbpop; ++e) {
+      const uint16_t sym = rbr[e];
+      if (bsp > 0) {
+        bad |= bst[--bsp * RWG] != sym;
+      } else {
+        bst[(bcap - 1 - bun) * RWG] = sym;
+        ++bun;
+      }
+    }
+    bdep -= nbpop;
+    bmin = bdep < bmin ? bdep : bmin;
+#pragma unroll 1
+    for (int e = 0; e < nbpush; ++e) bst[bsp++ * RWG] = rbr[nbpop + e];
+    bdep += nbpush;
+    const int32_t cbk = slot - tinc;  // tile-local compact index of the position's first node
+    int32_t tseen = 0;
+#pragma unroll 1
+    for (int j = 0; j < npk; ++j) {
+      const uint32_t pr = rb[k][24 + j];
+      const bool term = (tmask >> j) & 1u;
+      const int32_t c = cbk + j - tseen;  // node (tile-local compact index)
+      const int32_t tl = tinc + tseen;    // terminal production (tile-local)
+      tseen += term;
+      if (!r.tree) {
+        s_prod[slot + j] = (uint8_t)pr;
+      } else if (!term) {
+        s_prod[c] = (uint8_t)pr;
+      }
+      const uint32_t code = rb[k][32 + j];
+      int32_t par = -1;
+      if (code & 0x80u) {
+        if (ssp > 0) {
+          par = cb0 + sst[--ssp * RWG];
+        } else {
+          sst[(scap - 1 - sun) * RWG] = term ? (uint8_t)(TERM_SLOT | (tl - term0)) : (uint8_t)(c - cb0);
+          ++sun;
+        }
+        --sdep;
+        smin = sdep < smin ? sdep : smin;
+      } else {
+        par = cbk + (int32_t)code;
+      }
+      if (r.tree && par >= 0) {
+        if (term)
+          s_tpar[tl] = (uint16_t)par;
+        else
+          s_par[c] = (uint16_t)par;
+      }
+    }
+#pragma unroll 1
+    for (int e = 0; e < nspush; ++e) sst[ssp++ * RWG] = (uint8_t)(cbk - cb0 + rb[k][40 + e]);
+    sdep += nspush;
+    slot += npk;
+    tinc += ntk;
+  }
+
+  // phase 2: lane minima, suffix minima, sparse tables
+  const int32_t lbm = bS + bmin, lsm = sS + smin;
+  tbb[0][tid] = (int16_t)lbm;
+  tbs[0][tid] = (int16_t)lsm;
+  const int32_t bsuf = suffix_min_after(lbm, wmin[0]);
+  const int32_t ssuf = suffix_min_after(lsm, wmin[1]);
+  // level 0, the stacks and l_cb / l_tb are visible: suffix_min_after synchronised
+#pragma unroll
+  for (int k = 1; k < RLV; ++k) {
+    const int hh = 1 << (k - 1);
     int16_t a = tbb[k - 1][tid], c = tbs[k - 1][tid];
-    if ((int)tid >= h) {
-      const int16_t a2 = tbb[k - 1][tid - h], c2 = tbs[k - 1][tid - h];
+    if ((int)tid >= hh) {
+      const int16_t a2 = tbb[k - 1][tid - hh], c2 = tbs[k - 1][tid - hh];
       a = a2 < a ? a2 : a;
       c = c2 < c ? c2 : c;
     }
@@ -476,85 +561,69 @@ __global__ __launch_bounds__(PT) void seg_resolve(Resolve r) {
     __syncthreads();
   }
 
-  bool bad = false;
-  const int64_t c_off = ti.p_off - ti.t_off;  // compact index of the tile's first node (all inputs)
-  const int64_t c_in = in.p_off - in.t_off;   // ... of the input's first node
-  // brackets: pops (depth falling, so the matching position only moves back)
+  // pops left over by this lane (depth falling, so the matching lane only
+  // moves back); entries left open at the tile end
   int l = (int)tid - 1;
-  #pragma unroll 1
-  for (int e = 0; e < nbpop; ++e) {
-    const int32_t L = bD - e;
-    const uint32_t sym = rbr[e];
-    l = prev_below(tbb, l, L);
+#pragma unroll 1
+  for (int u = 0; u < bun; ++u) {
+    const int32_t D = bS - u;
+    const uint16_t sym = bst[(bcap - 1 - u) * RWG];
+    l = prev_below(tbb, l, D);
     if (l >= 0)
-      bad |= bpl[(L - tbb[0][l] - 1) * PT + l] != sym;
+      bad |= dyn[(D - tbb[0][l] - 1) * RWG + l] != sym;
     else
-      r.ub_list[ti.ub_off + (-L)] = (uint16_t)sym;
+      r.ub_list[ti.ub_off + (-D)] = sym;
   }
-  #pragma unroll 1
-  for (int e = 0; e < nbpush; ++e) {
-    const int32_t lv = m_b + 1 + e;
-    if (lv <= bsuf) r.ob_list[ti.ob_off + (lv - ti.bmin - 1)] = rbr[nbpop + e];
-  }
-  // productions and their parents
-  l = (int)tid - 1;
-  int32_t tseen = 0;
-  #pragma unroll 1
-  for (int j = 0; j < np; ++j) {
-    const uint32_t pr = rb[24 + j];
-    const bool term = (tmask >> j) & 1u;
-    const int32_t c = cbase + j - tseen;   // tile-local compact index (nodes)
-    const int32_t tl = term0 + tseen;      // tile-local terminal production
-    tseen += term;
-    if (!r.tree) {
-      s_prod[slot0 + j] = (uint8_t)pr;
-      continue;
-    }
-    if (!term) s_prod[c] = (uint8_t)pr;
-    const uint32_t code = rb[32 + j];
-    int32_t par;
-    if (code & 0x80u) {
-      const int32_t L = sD - (int32_t)(code & 0x7Fu);
-      l = prev_below(tbs, l, L);
-      if (l < 0) {
-        r.us_list[ti.us_off + (-L)] = term ? (TOKEN_TARGET | (ti.t_off + tl)) : (c_off + c);
-        continue;
-      }
-      par = spl[(L - tbs[0][l] - 1) * PT + l];
-    } else {
-      par = cbase + (int32_t)code;
-    }
-    if (term)
-      s_tpar[tl] = (uint16_t)par;
-    else
-      s_par[c] = (uint16_t)par;
+#pragma unroll 1
+  for (int e = 0; e < bsp; ++e) {
+    const int32_t lv = lbm + 1 + e;
+    if (lv <= bsuf) r.ob_list[ti.ob_off + (lv - ti.bmin - 1)] = bst[e * RWG];
   }
   if (r.tree) {
-    #pragma unroll 1
-    for (int e = 0; e < nspush; ++e) {
-      const int32_t lv = m_s + 1 + e;
-      if (lv <= ssuf) r.os_list[ti.os_off + (lv - ti.smin - 1)] = c_off + spl[e * PT + tid];
+    const uint8_t* const sall = reinterpret_cast<const uint8_t*>(dyn + bcap * RWG);
+    l = (int)tid - 1;
+#pragma unroll 1
+    for (int u = 0; u < sun; ++u) {
+      const int32_t D = sS - u;
+      const uint32_t tg = sst[(scap - 1 - u) * RWG];
+      l = prev_below(tbs, l, D);
+      const bool term = tg & TERM_SLOT;
+      const int32_t idx = (term ? term0 : cb0) + (int32_t)(tg & 0x7Fu);
+      if (l >= 0) {
+        const uint16_t pc = (uint16_t)(l_cb[l] + sall[(D - tbs[0][l] - 1) * RWG + l]);
+        if (term)
+          s_tpar[idx] = pc;
+        else
+          s_par[idx] = pc;
+      } else {
+        r.us_list[ti.us_off + (-D)] = term ? (TOKEN_TARGET | (ti.t_off + idx)) : (c_off + idx);
+      }
+    }
+#pragma unroll 1
+    for (int e = 0; e < ssp; ++e) {
+      const int32_t lv = lsm + 1 + e;
+      if (lv <= ssuf) r.os_list[ti.os_off + (lv - ti.smin - 1)] = c_off + cb0 + sst[e * RWG];
     }
   }
-  if (bad) atomicOr(s.flags + t.seg, BAD_MATCH);
-  if (tid == 0 && ti.bstart + ti.bmin < in.bstart) atomicOr(s.flags + t.seg, BAD_BDEPTH);
-  if (tid == 0 && ti.sstart + ti.smin < in.sstart - 1) atomicOr(s.flags + t.seg, BAD_SDEPTH);
+  if (bad) atomicOr(s.flags + seg, BAD_MATCH);
+  if (tid == 0 && ti.bstart + ti.bmin < in.bstart) atomicOr(s.flags + seg, BAD_BDEPTH);
+  if (tid == 0 && ti.sstart + ti.smin < in.sstart - 1) atomicOr(s.flags + seg, BAD_SDEPTH);
 
   // coalesced write-out; entries whose parent lies before the tile are
   // written again by far_resolve, which runs after this kernel
   __syncthreads();
   if (r.tree) {
     const int32_t nn = tile_np - tile_nt;
-    #pragma unroll 1
-    for (int32_t i = tid; i < nn; i += PT) {
+#pragma unroll 1
+    for (int32_t i = tid; i < nn; i += RWG) {
       r.tree_prod[c_off + i] = s_prod[i];
       r.tree_par[c_off + i] = c_off + s_par[i] - c_in;
     }
-    #pragma unroll 1
-    for (int32_t i = tid; i < tile_nt; i += PT) r.tok_par[ti.t_off + i] = c_off + s_tpar[i] - c_in;
+#pragma unroll 1
+    for (int32_t i = tid; i < tile_nt; i += RWG) r.tok_par[ti.t_off + i] = c_off + s_tpar[i] - c_in;
   } else {
-    #pragma unroll 1
-    for (int32_t i = tid; i < tile_np; i += PT) r.deriv[ti.p_off + i] = s_prod[i];
+#pragma unroll 1
+    for (int32_t i = tid; i < tile_np; i += RWG) r.deriv[ti.p_off + i] = s_prod[i];
   }
 }
 
@@ -736,10 +805,13 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
                           const uint32_t* h_skip, uint64_t count, bool tree, hipStream_t st) {
   const GrammarDesc* g = c.g;
   if (count == 0 || count > 0xFFFFFFFEull) return fail(ALPACC_ERR_ARG, "batch of %llu inputs", (unsigned long long)count);
-  const int ncap = PT * g->max_prods;
-  const size_t lds_b = (size_t)(g->max_bpush + g->max_spush) * PT * 2 + (size_t)ncap * 5;
-  // tile-local depths are int16 and tile-local compact indices u16
-  if ((int64_t)PT * 8 > 0x7FFF || ncap > 0xFFFF || lds_b > 96 * 1024)
+  const int ncap = PT * g->max_prods, tcap = PT * g->max_nt;
+  const int bcap = APL * (g->max_bpop + g->max_bpush), scap = APL * (g->max_spop + g->max_spush);
+  const size_t lds_b = (size_t)bcap * RWG * 2 + ((size_t)scap * RWG + 1) / 2 * 2 + (size_t)(ncap + tcap) * 2 +
+                       (size_t)ncap + 16;
+  // tile-local depths are int16, compact indices u16, lane-relative slot
+  // owners 7 bits
+  if ((int64_t)PT * 8 > 0x7FFF || ncap > 0xFFFF || APL * g->max_prods > 0x7F || lds_b > 96 * 1024)
     return fail(ALPACC_ERR_LIMIT, "grammar windows too large for the parser tiles");
   // tiles per input: framed positions = tokens + 2; inputs skipped get none
   uint64_t ntiles = 0, positions = 0;
@@ -820,7 +892,7 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
   if (ntiles) {
     hipLaunchKernelGGL(tile_map, dim3((uint32_t)((ntiles + 255) / 256)), dim3(256), 0, st, s.tile_first, s.count,
                        s.ntiles, const_cast<uint32_t*>(s.tile_seg));
-    HIP_TRY(tm.begin(st, "pair_profile"));
+    HIP_TRY(tm.begin(st, "seg_profile"));
     hipLaunchKernelGGL(seg_profile, dim3((uint32_t)ntiles), dim3(AWG), 0, st, s, rec,
                        reinterpret_cast<uint16_t*>(base + off_pid));
     HIP_TRY(tm.end(st));
@@ -855,7 +927,7 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
       return r;
     HIP_TRY(tm.end(st));
     Resolve rs{s,         reinterpret_cast<const uint16_t*>(base + off_pid),
-               info,      sb,       g->max_bpush,  g->max_spush, ncap,
+               info,      sb,       bcap,  scap, ncap, tcap,
                ob_list,   ub_list,   os_list,  us_list,       c.tr_prod.as<uint8_t>(),
                c.tr_parent.as<int64_t>(), c.tr_tokpar.as<int64_t>(), tree ? nullptr : c.ps_d.as<uint8_t>(),
                tree ? 1 : 0};
@@ -864,8 +936,8 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
                                   (int)lds_b));
       c.resolve_lds_set = lds_b;
     }
-    HIP_TRY(tm.begin(st, "tile_resolve"));
-    hipLaunchKernelGGL(seg_resolve, dim3((uint32_t)ntiles), dim3(PT), lds_b, st, rs);
+    HIP_TRY(tm.begin(st, "seg_resolve"));
+    hipLaunchKernelGGL(seg_resolve, dim3((uint32_t)ntiles), dim3(RWG), lds_b, st, rs);
     HIP_TRY(tm.end(st));
     Far fr{s, info, sb, tb, ts, ob_list, ub_list, os_list, us_list, c.tr_parent.as<int64_t>(),
            c.tr_tokpar.as<int64_t>(), tree ? 1 : 0};

@@ -130,7 +130,7 @@ def device_tables(g: Grammar) -> dict:
     # Window-local compact index: production index minus the terminal
     # productions before it (nodes of the compact tree).
     rec = []
-    eff = dict(bpop=0, bpush=0, spop=0, spush=0)
+    eff = dict(bpop=0, bpush=0, spop=0, spush=0, nt=0)
     for i, (s0, s1, p0, p1) in enumerate(p.spans):
         st, ext = [], []
         ok = True
@@ -171,6 +171,7 @@ def device_tables(g: Grammar) -> dict:
         eff["bpush"] = max(eff["bpush"], len(st))
         eff["spop"] = max(eff["spop"], spop)
         eff["spush"] = max(eff["spush"], len(slots))
+        eff["nt"] = max(eff["nt"], terms)
         br = ext + st + [0] * (8 - len(ext) - len(st))
         rec += [len(ext), len(st), len(prods), terms, spop, len(slots), tmask, 0]
         rec += [b for v in br for b in (v & 0xFF, v >> 8)]
@@ -197,6 +198,7 @@ def device_tables(g: Grammar) -> dict:
         arity=p.arity, prod_is_term=[1 if t is not None else 0 for t in p.prod_terminal],
         profile=profile, max_slot_items=max_slot_items, pair_rec=rec, prod_info=prod_info,
         max_bpop=eff["bpop"], max_bpush=eff["bpush"], max_spop=eff["spop"], max_spush=eff["spush"],
+        max_nt=eff["nt"],
     )
 
 
@@ -241,6 +243,7 @@ def emit(g: Grammar) -> str:
         f"  {max(1, p.max_brackets)}, {max(1, p.max_prods)}, {d['max_slot_items']},",
         f"  {ns}::PAIR_REC, {ns}::PROD_INFO,",
         f"  {max(1, d['max_bpop'])}, {max(1, d['max_bpush'])}, {max(1, d['max_spop'])}, {max(1, d['max_spush'])},",
+        f"  {max(1, d['max_nt'])},",
         "};",
         "}  // namespace alpacc",
         "",

@@ -63,8 +63,8 @@ def kernel_bytes(name: str, n: int, T: int, N: int) -> int:
     return {
         "lex_fold": n,                 # input
         "lex_emit": n + 11 * T,        # input + (terminal, start, length) per token
-        "pair_profile": T,             # tokens
-        "tile_resolve": 8 * T + 9 * N,  # token parents + (production, parent) per node
+        "seg_profile": T,              # tokens
+        "seg_resolve": 8 * T + 9 * N,  # token parents + (production, parent) per node
     }.get(name, 0)
 
 
