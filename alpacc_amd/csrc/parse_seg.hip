@@ -89,19 +89,17 @@ __device__ __forceinline__ TileOf tile_of(const Seg& s, uint32_t tile) {
   return t;
 }
 
-// Pair id of framed position p, or NO_PAIR.
-__device__ __forceinline__ uint32_t pair_at(const Seg& s, const TileOf& t, int64_t p) {
+// Pair id of framed position p, or NO_PAIR (32-bit positions: an input has
+// fewer than 2^31 tokens).
+__device__ __forceinline__ uint32_t pair_at(const Seg& s, const TileOf& t, int64_t p64) {
+  const int32_t p = (int32_t)p64, m = (int32_t)t.m;
   uint32_t code = 0;
   const int w = s.q + s.k;
   for (int j = 0; j < w; ++j) {
-    const int64_t f = p - s.q + j;
+    const int32_t f = p - s.q + j;
     uint32_t v;
-    if (f < 0 || f >= t.m)
-      v = s.empty_t;
-    else if (f == 0)
-      v = s.start_t;
-    else if (f == t.m - 1)
-      v = s.end_t;
+    if (f <= 0 || f >= m - 1)
+      v = f == 0 ? (uint32_t)s.start_t : (f == m - 1 ? (uint32_t)s.end_t : (uint32_t)s.empty_t);
     else
       v = t.tk[f - 1];
     if (v >= (uint32_t)s.base) return NO_PAIR;
@@ -146,6 +144,25 @@ struct SegBase {
 __device__ __forceinline__ uint32_t hb(const uint4& r0, int i) {
   return ((i < 4 ? r0.x : r0.y) >> (8 * (i & 3))) & 0xFFu;
 }
+// Fields of a 64-byte pair record held in registers as r0..r3 (the layout
+// emit_hip.py writes): bracket symbol j (0..7), production j, parent code j
+// (0..7), owner e (0..23).
+__device__ __forceinline__ uint32_t rec_br(const uint4& r0, const uint4& r1, int j) {
+  const uint64_t w = j < 4 ? (((uint64_t)r0.w << 32) | r0.z) : (((uint64_t)r1.y << 32) | r1.x);
+  return (uint32_t)(w >> (16 * (j & 3))) & 0xFFFFu;
+}
+__device__ __forceinline__ uint32_t rec_prod(const uint4& r1, int j) {
+  return (uint32_t)((((uint64_t)r1.w << 32) | r1.z) >> (8 * j)) & 0xFFu;
+}
+__device__ __forceinline__ uint32_t rec_parent(const uint4& r2, int j) {
+  return (uint32_t)((((uint64_t)r2.y << 32) | r2.x) >> (8 * j)) & 0xFFu;
+}
+__device__ __forceinline__ uint32_t rec_owner(const uint4& r2, const uint4& r3, int e) {
+  const uint64_t w = e < 8 ? (((uint64_t)r2.w << 32) | r2.z)
+                           : (e < 16 ? (((uint64_t)r3.y << 32) | r3.x) : (((uint64_t)r3.w << 32) | r3.z));
+  return (uint32_t)(w >> (8 * (e & 7))) & 0xFFu;
+}
+
 // ---------------------------------------------------------------- pass A
 
 // Both tile passes: APL consecutive positions per lane, PT / APL lanes.
@@ -445,17 +462,16 @@ __global__ __launch_bounds__(RWG) void seg_resolve(Resolve r) {
   const TileInfo ti = r.info[tile];
   const SegBase in = r.sb[seg];
 
-  // phase 0: records of the lane's positions, lane sums and offsets
+  // phase 0: record heads of the lane's positions, lane sums and offsets
   const uint2 pv = *reinterpret_cast<const uint2*>(r.pid + (uint64_t)tile * PT + (uint64_t)tid * APL);
   const uint32_t ids[APL] = {pv.x & 0xFFFFu, pv.x >> 16, pv.y & 0xFFFFu, pv.y >> 16};
-  const uint8_t* rb[APL];
+  const uint4* rp[APL];
   uint4 h[APL];
   int32_t np = 0, nt = 0, bd = 0, sd = 0;
 #pragma unroll
   for (int k = 0; k < APL; ++k) {
-    const uint4* rp = s.rec + 4 * (ids[k] == NO_PAIR ? (uint32_t)s.npairs : ids[k]);
-    rb[k] = reinterpret_cast<const uint8_t*>(rp);
-    h[k] = rp[0];
+    rp[k] = s.rec + 4 * (ids[k] == NO_PAIR ? (uint32_t)s.npairs : ids[k]);
+    h[k] = rp[k][0];
     np += (h[k].x >> 16) & 0xFF;
     nt += h[k].x >> 24;
     bd += (int32_t)((h[k].x >> 8) & 0xFF) - (int32_t)(h[k].x & 0xFF);
@@ -476,16 +492,23 @@ __global__ __launch_bounds__(RWG) void seg_resolve(Resolve r) {
   int32_t bdep = 0, bmin = 0, sdep = 0, smin = 0;
   int32_t slot = slot0, tinc = term0;
   bool bad = false;
+  // the rest of each record is held in registers while its position runs:
+  // one 48-byte load at the position's start (its address waits for the
+  // walk's state, so the compiler does not hoist the four loads together)
 #pragma unroll
   for (int k = 0; k < APL; ++k) {
-    const uint32_t hx = h[k].x, hy = h[k].y;
+    uintptr_t ra = reinterpret_cast<uintptr_t>(rp[k]);
+    asm volatile("" : "+v"(ra) : "v"(slot));
+    const uint4* q = reinterpret_cast<const uint4*>(ra);
+    const uint4 r1 = q[1], r2 = q[2], r3 = q[3];
+    const uint4 r0 = h[k];
+    const uint32_t hx = r0.x, hy = r0.y;
     const int nbpop = hx & 0xFF, nbpush = (hx >> 8) & 0xFF, npk = (hx >> 16) & 0xFF, ntk = hx >> 24;
     const int nspush = (hy >> 8) & 0xFF;
     const uint32_t tmask = (hy >> 16) & 0xFF;
-    const uint16_t* rbr = reinterpret_cast<const uint16_t*>(rb[k]) + 4;
-#pragma unroll 1
+#pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
     for (int e = 0; e < nbpop; ++e) {
-      const uint16_t sym = rbr[e];
+      const uint16_t sym = (uint16_t)rec_br(r0, r1, e);
       if (bsp > 0) {
         bad |= bst[--bsp * RWG] != sym;
       } else {
@@ -495,14 +518,14 @@ __global__ __launch_bounds__(RWG) void seg_resolve(Resolve r) {
     }
     bdep -= nbpop;
     bmin = bdep < bmin ? bdep : bmin;
-#pragma unroll 1
-    for (int e = 0; e < nbpush; ++e) bst[bsp++ * RWG] = rbr[nbpop + e];
+#pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
+    for (int e = 0; e < nbpush; ++e) bst[bsp++ * RWG] = (uint16_t)rec_br(r0, r1, nbpop + e);
     bdep += nbpush;
     const int32_t cbk = slot - tinc;  // tile-local compact index of the position's first node
     int32_t tseen = 0;
-#pragma unroll 1
+#pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
     for (int j = 0; j < npk; ++j) {
-      const uint32_t pr = rb[k][24 + j];
+      const uint32_t pr = rec_prod(r1, j);
       const bool term = (tmask >> j) & 1u;
       const int32_t c = cbk + j - tseen;  // node (tile-local compact index)
       const int32_t tl = tinc + tseen;    // terminal production (tile-local)
@@ -512,7 +535,7 @@ __global__ __launch_bounds__(RWG) void seg_resolve(Resolve r) {
       } else if (!term) {
         s_prod[c] = (uint8_t)pr;
       }
-      const uint32_t code = rb[k][32 + j];
+      const uint32_t code = rec_parent(r2, j);
       int32_t par = -1;
       if (code & 0x80u) {
         if (ssp > 0) {
@@ -533,8 +556,8 @@ __global__ __launch_bounds__(RWG) void seg_resolve(Resolve r) {
           s_par[c] = (uint16_t)par;
       }
     }
-#pragma unroll 1
-    for (int e = 0; e < nspush; ++e) sst[ssp++ * RWG] = (uint8_t)(cbk - cb0 + rb[k][40 + e]);
+#pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
+    for (int e = 0; e < nspush; ++e) sst[ssp++ * RWG] = (uint8_t)(cbk - cb0 + rec_owner(r2, r3, e));
     sdep += nspush;
     slot += npk;
     tinc += ntk;
@@ -564,7 +587,7 @@ __global__ __launch_bounds__(RWG) void seg_resolve(Resolve r) {
   // pops left over by this lane (depth falling, so the matching lane only
   // moves back); entries left open at the tile end
   int l = (int)tid - 1;
-#pragma unroll 1
+#pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
   for (int u = 0; u < bun; ++u) {
     const int32_t D = bS - u;
     const uint16_t sym = bst[(bcap - 1 - u) * RWG];
@@ -574,7 +597,7 @@ __global__ __launch_bounds__(RWG) void seg_resolve(Resolve r) {
     else
       r.ub_list[ti.ub_off + (-D)] = sym;
   }
-#pragma unroll 1
+#pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
   for (int e = 0; e < bsp; ++e) {
     const int32_t lv = lbm + 1 + e;
     if (lv <= bsuf) r.ob_list[ti.ob_off + (lv - ti.bmin - 1)] = bst[e * RWG];
@@ -582,7 +605,7 @@ __global__ __launch_bounds__(RWG) void seg_resolve(Resolve r) {
   if (r.tree) {
     const uint8_t* const sall = reinterpret_cast<const uint8_t*>(dyn + bcap * RWG);
     l = (int)tid - 1;
-#pragma unroll 1
+#pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
     for (int u = 0; u < sun; ++u) {
       const int32_t D = sS - u;
       const uint32_t tg = sst[(scap - 1 - u) * RWG];
@@ -599,7 +622,7 @@ __global__ __launch_bounds__(RWG) void seg_resolve(Resolve r) {
         r.us_list[ti.us_off + (-D)] = term ? (TOKEN_TARGET | (ti.t_off + idx)) : (c_off + idx);
       }
     }
-#pragma unroll 1
+#pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
     for (int e = 0; e < ssp; ++e) {
       const int32_t lv = lsm + 1 + e;
       if (lv <= ssuf) r.os_list[ti.os_off + (lv - ti.smin - 1)] = c_off + cb0 + sst[e * RWG];
@@ -614,15 +637,15 @@ __global__ __launch_bounds__(RWG) void seg_resolve(Resolve r) {
   __syncthreads();
   if (r.tree) {
     const int32_t nn = tile_np - tile_nt;
-#pragma unroll 1
+#pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
     for (int32_t i = tid; i < nn; i += RWG) {
       r.tree_prod[c_off + i] = s_prod[i];
       r.tree_par[c_off + i] = c_off + s_par[i] - c_in;
     }
-#pragma unroll 1
+#pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
     for (int32_t i = tid; i < tile_nt; i += RWG) r.tok_par[ti.t_off + i] = c_off + s_tpar[i] - c_in;
   } else {
-#pragma unroll 1
+#pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
     for (int32_t i = tid; i < tile_np; i += RWG) r.deriv[ti.p_off + i] = s_prod[i];
   }
 }
