@@ -75,6 +75,7 @@ int lex_batch_on_device(Context& c, int mode, const uint8_t* d_in, const uint64_
   p.endo_init = c.d_endo_init;
   const bool lex_ids = mode == ALPACC_MODE_LEXER;
   p.delta = lex_ids ? c.d_delta_lex : c.d_delta_both;
+  p.token = lex_ids ? c.d_token_lex : c.d_token_both;
   p.nstate = g->nstate;
   p.nendo = g->nendo;
   p.nclass = g->nclass;

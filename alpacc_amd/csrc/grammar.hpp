@@ -16,8 +16,10 @@ struct GrammarDesc {
   const uint16_t* step;          // [nendo * nclass]  e then class c
   const uint16_t* compose;       // [nendo * nendo]   a then b
   const uint8_t* endo_init;      // [nendo]           state the initial state maps to
-  const uint16_t* delta_lex;     // [nstate * 256]    next | produce<<8 | token(s)<<9
-  const uint16_t* delta_both;    // same, combined-mode terminal ids
+  const uint16_t* delta_lex;     // [nstate * 256]    next | produce<<8 | kept<<9
+  const uint16_t* delta_both;    // same (kept: the token state s accepts is not ignored)
+  const uint8_t* token_lex;      // [nstate]          terminal state s accepts
+  const uint8_t* token_both;     // same, combined-mode terminal ids
   int ignore_lex, ignore_both;   // -1 when the grammar has no ignore terminal
   int dead_lex, dead_both;       // terminal id of "no token"
   int terminal_bytes_lex, terminal_bytes_both, index_bytes, length_bytes;

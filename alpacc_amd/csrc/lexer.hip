@@ -19,7 +19,7 @@
 //                    the first tile of every input;
 //   L3 lex_emit      every lane starts from the DFA state
 //                    ENDO_INIT[tile prefix . lane prefix] and walks its bytes
-//                    through the LDS token DFA (next | produce | token),
+//                    through the LDS token DFA (next | produce | kept),
 //                    counting kept tokens and its last boundary; a block scan
 //                    plus a decoupled lookback over (kept-token count, last
 //                    boundary) -- sum and max, so a whole wave inspects 64
@@ -215,11 +215,13 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
   uint32_t* s_start = reinterpret_cast<uint32_t*>(lds + delta_bytes);  // [STAGE] token start
   uint16_t* s_end = reinterpret_cast<uint16_t*>(s_start + STAGE);       // [STAGE] end, tile-relative
   uint8_t* s_term = reinterpret_cast<uint8_t*>(s_end + STAGE);          // [STAGE] terminal
+  __shared__ uint8_t s_token[256];                                       // terminal each state accepts
   __shared__ uint32_t s_tile;
   __shared__ uint32_t s_wave_cnt[NW], s_wave_last[NW];
   __shared__ uint32_t s_pre_cnt, s_pre_last;
   const uint32_t tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
   for (int i = tid; i < p.nstate * 256; i += WG) s_delta[i] = p.delta[i];
+  for (int i = tid; i < p.nstate; i += WG) s_token[i] = p.token[i];
   const int ignore = p.ignore;
 
   for (;;) {
@@ -246,7 +248,7 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
     auto step1 = [&](uint32_t byte, uint32_t j) {
       const uint32_t x = s_delta[s * 256 + byte];
       bm[j >> 5] |= ((x >> 8) & 1u) << (j & 31);
-      cnt += (x & 0x100u) && (int)(x >> 9) != ignore;
+      cnt += (x >> 9) & 1u;
       s = x & 0xFFu;
     };
     if (full) {
@@ -263,7 +265,7 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
     uint32_t fin_t = 0;
     bool fin_kept = false;
     if (owns_end) {
-      fin_t = s_delta[s * 256] >> 9;
+      fin_t = s_token[s];
       if ((int)fin_t == p.dead) {
         atomicOr(p.seg_flags + ts.seg, LEX_REJECT_FINAL);
       } else if ((int)fin_t != ignore) {
@@ -363,19 +365,16 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
         uint32_t st = s0, k = ecnt, start = start0;
         auto step2 = [&](uint32_t byte, uint32_t j) {
           const uint32_t x = s_delta[st * 256 + byte];
-          if (x & 0x100u) {
-            const uint32_t t = x >> 9;
-            if ((int)t != ignore) {
-              const uint32_t i = k - lo;
-              if (i < STAGE) {
-                s_term[i] = (uint8_t)t;
-                s_start[i] = start;
-                s_end[i] = (uint16_t)(voff + j);
-              }
-              ++k;
+          if (x & 0x200u) {  // a kept token closes before byte j
+            const uint32_t i = k - lo;
+            if (i < STAGE) {
+              s_term[i] = s_token[st];
+              s_start[i] = start;
+              s_end[i] = (uint16_t)(voff + j);
             }
-            start = vbase + j;
+            ++k;
           }
+          start = (x & 0x100u) ? vbase + j : start;
           st = x & 0xFFu;
         };
         if (full) {

@@ -31,7 +31,8 @@ struct LexParams {
   const uint16_t* step;
   const uint16_t* compose;
   const uint8_t* endo_init;
-  const uint16_t* delta;
+  const uint16_t* delta;  // next | produce<<8 | kept<<9
+  const uint8_t* token;   // [nstate] terminal each state accepts
   int nstate, nendo, nclass;
   int ignore, dead;
   uint64_t max_len;

@@ -145,8 +145,8 @@ __device__ __forceinline__ uint32_t hb(const uint4& r0, int i) {
   return ((i < 4 ? r0.x : r0.y) >> (8 * (i & 3))) & 0xFFu;
 }
 // Fields of a 64-byte pair record held in registers as r0..r3 (the layout
-// emit_hip.py writes): bracket symbol j (0..7), production j, parent code j
-// (0..7), owner e (0..23).
+// emit_hip.py writes): bracket symbol j (pops 0..3, pushes 4..7),
+// production j, parent code j (0..7), owner e (0..23).
 __device__ __forceinline__ uint32_t rec_br(const uint4& r0, const uint4& r1, int j) {
   const uint64_t w = j < 4 ? (((uint64_t)r0.w << 32) | r0.z) : (((uint64_t)r1.y << 32) | r1.x);
   return (uint32_t)(w >> (16 * (j & 3))) & 0xFFFFu;
@@ -433,6 +433,10 @@ __device__ __forceinline__ int32_t suffix_min_after(int32_t v, int32_t* wtot) {
   return ex;
 }
 
+// MBPOP..MSPUSH bound the window effects of the grammar (bracket pops and
+// pushes, productions, surviving child slots); the host picks the smallest
+// instantiation that covers them.
+//
 // Every lane runs its APL positions' window effects in order on two short
 // stacks in LDS -- bracket symbols, and open child slots as lane-relative
 // owner indices -- interleaved by lane (entry e of lane t at e * RWG + t, so
@@ -441,6 +445,7 @@ __device__ __forceinline__ int32_t suffix_min_after(int32_t v, int32_t* wtot) {
 // consecutive levels just above its minimum, so a pop left over by a later
 // lane at tile-relative level D reads entry D - min - 1 of the last earlier
 // lane whose minimum is below D (a sparse table over the lanes' minima).
+template <int MBPOP, int MBPUSH, int MNP, int MSPUSH>
 __global__ __launch_bounds__(RWG) void seg_resolve(Resolve r) {
   // dynamic LDS: bracket stacks u16 [bcap][RWG], slot stacks u8 [scap][RWG],
   // staged parents of nodes u16 [ncap] and of terminal productions u16
@@ -506,58 +511,67 @@ __global__ __launch_bounds__(RWG) void seg_resolve(Resolve r) {
     const int nbpop = hx & 0xFF, nbpush = (hx >> 8) & 0xFF, npk = (hx >> 16) & 0xFF, ntk = hx >> 24;
     const int nspush = (hy >> 8) & 0xFF;
     const uint32_t tmask = (hy >> 16) & 0xFF;
-#pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
-    for (int e = 0; e < nbpop; ++e) {
-      const uint16_t sym = (uint16_t)rec_br(r0, r1, e);
-      if (bsp > 0) {
-        bad |= bst[--bsp * RWG] != sym;
-      } else {
-        bst[(bcap - 1 - bun) * RWG] = sym;
-        ++bun;
+    // the loops run to the grammar's maxima with constant field offsets;
+    // iterations past the window's own counts do nothing
+#pragma unroll
+    for (int e = 0; e < MBPOP; ++e) {
+      if (e < nbpop) {
+        const uint16_t sym = (uint16_t)rec_br(r0, r1, e);
+        if (bsp > 0) {
+          bad |= bst[--bsp * RWG] != sym;
+        } else {
+          bst[(bcap - 1 - bun) * RWG] = sym;
+          ++bun;
+        }
       }
     }
     bdep -= nbpop;
     bmin = bdep < bmin ? bdep : bmin;
-#pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
-    for (int e = 0; e < nbpush; ++e) bst[bsp++ * RWG] = (uint16_t)rec_br(r0, r1, nbpop + e);
+#pragma unroll
+    for (int e = 0; e < MBPUSH; ++e)
+      if (e < nbpush) bst[(bsp + e) * RWG] = (uint16_t)rec_br(r0, r1, 4 + e);
+    bsp += nbpush;
     bdep += nbpush;
     const int32_t cbk = slot - tinc;  // tile-local compact index of the position's first node
     int32_t tseen = 0;
-#pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
-    for (int j = 0; j < npk; ++j) {
-      const uint32_t pr = rec_prod(r1, j);
-      const bool term = (tmask >> j) & 1u;
-      const int32_t c = cbk + j - tseen;  // node (tile-local compact index)
-      const int32_t tl = tinc + tseen;    // terminal production (tile-local)
-      tseen += term;
-      if (!r.tree) {
-        s_prod[slot + j] = (uint8_t)pr;
-      } else if (!term) {
-        s_prod[c] = (uint8_t)pr;
-      }
-      const uint32_t code = rec_parent(r2, j);
-      int32_t par = -1;
-      if (code & 0x80u) {
-        if (ssp > 0) {
-          par = cb0 + sst[--ssp * RWG];
-        } else {
-          sst[(scap - 1 - sun) * RWG] = term ? (uint8_t)(TERM_SLOT | (tl - term0)) : (uint8_t)(c - cb0);
-          ++sun;
+#pragma unroll
+    for (int j = 0; j < MNP; ++j) {
+      if (j < npk) {
+        const uint32_t pr = rec_prod(r1, j);
+        const bool term = (tmask >> j) & 1u;
+        const int32_t c = cbk + j - tseen;  // node (tile-local compact index)
+        const int32_t tl = tinc + tseen;    // terminal production (tile-local)
+        tseen += term;
+        if (!r.tree) {
+          s_prod[slot + j] = (uint8_t)pr;
+        } else if (!term) {
+          s_prod[c] = (uint8_t)pr;
         }
-        --sdep;
-        smin = sdep < smin ? sdep : smin;
-      } else {
-        par = cbk + (int32_t)code;
-      }
-      if (r.tree && par >= 0) {
-        if (term)
-          s_tpar[tl] = (uint16_t)par;
-        else
-          s_par[c] = (uint16_t)par;
+        const uint32_t code = rec_parent(r2, j);
+        int32_t par = cbk + (int32_t)(code & 0x7Fu);
+        if (code & 0x80u) {
+          if (ssp > 0) {
+            par = cb0 + sst[--ssp * RWG];
+          } else {
+            sst[(scap - 1 - sun) * RWG] = term ? (uint8_t)(TERM_SLOT | (tl - term0)) : (uint8_t)(c - cb0);
+            ++sun;
+            par = -1;
+          }
+          --sdep;
+          smin = sdep < smin ? sdep : smin;
+        }
+        if (r.tree && par >= 0) {
+          if (term)
+            s_tpar[tl] = (uint16_t)par;
+          else
+            s_par[c] = (uint16_t)par;
+        }
       }
     }
-#pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
-    for (int e = 0; e < nspush; ++e) sst[ssp++ * RWG] = (uint8_t)(cbk - cb0 + rec_owner(r2, r3, e));
+#pragma unroll
+    for (int e = 0; e < MSPUSH; ++e)
+      if (e < nspush) sst[(ssp + e) * RWG] = (uint8_t)(cbk - cb0 + rec_owner(r2, r3, e));
+    ssp += nspush;
     sdep += nspush;
     slot += npk;
     tinc += ntk;
@@ -954,13 +968,20 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
                ob_list,   ub_list,   os_list,  us_list,       c.tr_prod.as<uint8_t>(),
                c.tr_parent.as<int64_t>(), c.tr_tokpar.as<int64_t>(), tree ? nullptr : c.ps_d.as<uint8_t>(),
                tree ? 1 : 0};
+    void (*kern)(Resolve);
+    if (g->max_bpop <= 2 && g->max_bpush <= 3 && g->max_prods <= 4 && g->max_spush <= 3)
+      kern = seg_resolve<2, 3, 4, 3>;
+    else if (g->max_bpop <= 3 && g->max_bpush <= 4 && g->max_prods <= 4 && g->max_spush <= 4)
+      kern = seg_resolve<3, 4, 4, 4>;
+    else
+      kern = seg_resolve<4, 4, 8, 24>;
     if (!c.resolve_lds_set || c.resolve_lds_set < lds_b) {
-      HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(seg_resolve), hipFuncAttributeMaxDynamicSharedMemorySize,
+      HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)lds_b));
       c.resolve_lds_set = lds_b;
     }
     HIP_TRY(tm.begin(st, "seg_resolve"));
-    hipLaunchKernelGGL(seg_resolve, dim3((uint32_t)ntiles), dim3(RWG), lds_b, st, rs);
+    hipLaunchKernelGGL(kern, dim3((uint32_t)ntiles), dim3(RWG), lds_b, st, rs);
     HIP_TRY(tm.end(st));
     Far fr{s, info, sb, tb, ts, ob_list, ub_list, os_list, us_list, c.tr_parent.as<int64_t>(),
            c.tr_tokpar.as<int64_t>(), tree ? 1 : 0};

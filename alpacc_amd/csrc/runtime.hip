@@ -138,6 +138,8 @@ int alpacc_hip_open(const char* grammar, int device, alpacc_hip_ctx** out) {
       (r = upload(&c.d_endo_init, g->endo_init, (size_t)g->nendo)) ||
       (r = upload(&c.d_delta_lex, g->delta_lex, (size_t)g->nstate * 256)) ||
       (r = upload(&c.d_delta_both, g->delta_both, (size_t)g->nstate * 256)) ||
+      (r = upload(&c.d_token_lex, g->token_lex, (size_t)g->nstate)) ||
+      (r = upload(&c.d_token_both, g->token_both, (size_t)g->nstate)) ||
       (r = upload(&c.d_window, g->window, (size_t)g->nwindow)) ||
       (r = upload(&c.d_spans, g->spans, (size_t)g->npairs * 4)) ||
       (r = upload(&c.d_profile, g->profile, (size_t)g->npairs * 8)) ||
@@ -164,7 +166,7 @@ void alpacc_hip_close(alpacc_hip_ctx* h) {
   if (!h) return;
   Context& c = h->c;
   (void)hipSetDevice(c.device);
-  void* ptrs[] = {c.d_class, c.d_step, c.d_compose, c.d_endo_init, c.d_delta_lex, c.d_delta_both, c.d_window,
+  void* ptrs[] = {c.d_class, c.d_step, c.d_compose, c.d_endo_init, c.d_delta_lex, c.d_delta_both, c.d_token_lex, c.d_token_both, c.d_window,
                   c.d_spans, c.d_profile, c.d_pair_rec, c.d_prod_info, c.d_stacks, c.d_productions, c.d_arity, c.d_isterm};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);

@@ -67,6 +67,8 @@ struct Context {
   uint8_t* d_endo_init = nullptr;
   uint16_t* d_delta_lex = nullptr;
   uint16_t* d_delta_both = nullptr;
+  uint8_t* d_token_lex = nullptr;
+  uint8_t* d_token_both = nullptr;
   uint16_t* d_window = nullptr;
   int32_t* d_spans = nullptr;
   uint8_t* d_profile = nullptr;

@@ -8,8 +8,10 @@ The GPU path does not use the oracle's packed-endomorphism format.  It needs:
 * ``COMPOSE[a * NENDO + b]`` -- "a then b" for the scans (u16, L2-resident);
 * ``ENDO_INIT[e]`` -- the DFA state e maps the initial state to;
 * ``DELTA_*[s * 256 + b]`` -- the token DFA itself, u16 packed as
-  ``next (8 bits) | produce << 8 | token(s) << 9`` for the two terminal
-  numberings (lexer-only, combined), staged in LDS for the second pass;
+  ``next (8 bits) | produce << 8 | kept << 9`` (kept: the closed token is
+  not ignored) and ``TOKEN_*[s]`` the terminal state s accepts, for the two
+  terminal numberings (lexer-only, combined), staged in LDS for the lexer's
+  last pass;
 * the LLP pair table as a dense window index (``WINDOW[key] -> pair id``,
   0xffff = no pair) over the padded q+k window, pair spans into the bracket
   and production strings (brackets as u16: symbol | left << 15), arities and
@@ -59,16 +61,28 @@ def device_tables(g: Grammar) -> dict:
     endo_init = [endos.funcs[e][1] for e in range(ne)]
 
     def delta(lt) -> List[int]:
+        # next | produce << 8 | kept << 9: a boundary before the byte, and
+        # whether the token it closes (the one state s accepts) is kept
+        out = []
+        ign = lt.ignore_id
+        for s in range(nstate):
+            tok = dfa.token[s] if s else None
+            t = lt.term_id[tok] if tok is not None else lt.dead_id
+            for b in range(256):
+                nxt = dfa.delta[s][b] if s else 0
+                p = 1 if (s and dfa.produce[s][b]) else 0
+                out.append(nxt | (p << 8) | ((p if t != ign else 0) << 9))
+        return out
+
+    def tokens(lt) -> List[int]:
+        # the terminal state s accepts (the token a boundary after it closes)
         out = []
         for s in range(nstate):
             tok = dfa.token[s] if s else None
             t = lt.term_id[tok] if tok is not None else lt.dead_id
-            if t > 127:
-                raise EmitError(f"{g.name}: terminal id {t} exceeds the 7-bit token field")
-            for b in range(256):
-                nxt = dfa.delta[s][b] if s else 0
-                p = 1 if (s and dfa.produce[s][b]) else 0
-                out.append(nxt | (p << 8) | (t << 9))
+            if t > 255:
+                raise EmitError(f"{g.name}: terminal id {t} exceeds u8")
+            out.append(t)
         return out
 
     p = g.parser
@@ -122,7 +136,7 @@ def device_tables(g: Grammar) -> dict:
     # top), slot pops reaching below it and surviving child slots.
     #   0 bracket pops   1 bracket pushes   2 productions   3 terminal productions
     #   4 slot pops      5 slot pushes      6 terminal-production mask   7 0
-    #   8..23   u16 bracket symbols: the pops, then the pushes
+    #   8..15   u16 bracket symbols of the pops, 16..23 of the pushes
     #   24..31  productions
     #   32..39  parent of production j: 0x80 | e for the window's e-th slot
     #           pop, else the window-local compact index of its parent
@@ -144,8 +158,8 @@ def device_tables(g: Grammar) -> dict:
         prods = list(p.productions[p0:p1])
         if len(prods) > 8:
             raise EmitError(f"{g.name}: a window has more than 8 productions")
-        if len(ext) + len(st) > 8:
-            raise EmitError(f"{g.name}: a window keeps more than 8 brackets")
+        if len(ext) > 4 or len(st) > 4:
+            raise EmitError(f"{g.name}: a window keeps more than 4 bracket pops or 4 pushes")
         compact, terms, tmask = [], 0, 0
         for j, prod in enumerate(prods):
             compact.append(j - terms)
@@ -172,7 +186,7 @@ def device_tables(g: Grammar) -> dict:
         eff["spop"] = max(eff["spop"], spop)
         eff["spush"] = max(eff["spush"], len(slots))
         eff["nt"] = max(eff["nt"], terms)
-        br = ext + st + [0] * (8 - len(ext) - len(st))
+        br = ext + [0] * (4 - len(ext)) + st + [0] * (4 - len(st))
         rec += [len(ext), len(st), len(prods), terms, spop, len(slots), tmask, 0]
         rec += [b for v in br for b in (v & 0xFF, v >> 8)]
         rec += prods + [0] * (8 - len(prods))
@@ -190,7 +204,7 @@ def device_tables(g: Grammar) -> dict:
                           for (_, _, p0, p1) in p.spans] + [1])
     return dict(
         nstate=nstate, nendo=ne, nclass=nc, byte_class=classes, step=step, compose=compose,
-        endo_init=endo_init, delta_lex=delta(lo), delta_both=delta(lb),
+        endo_init=endo_init, delta_lex=delta(lo), delta_both=delta(lb), token_lex=tokens(lo), token_both=tokens(lb),
         ignore_lex=-1 if lo.ignore_id is None else lo.ignore_id,
         ignore_both=-1 if lb.ignore_id is None else lb.ignore_id,
         dead_lex=lo.dead_id, dead_both=lb.dead_id,
@@ -217,6 +231,8 @@ def emit(g: Grammar) -> str:
         _arr("uint8_t", "ENDO_INIT", d["endo_init"]),
         _arr("uint16_t", "DELTA_LEX", d["delta_lex"]),
         _arr("uint16_t", "DELTA_BOTH", d["delta_both"]),
+        _arr("uint8_t", "TOKEN_LEX", d["token_lex"]),
+        _arr("uint8_t", "TOKEN_BOTH", d["token_both"]),
         _arr("uint16_t", "WINDOW", d["window"]),
         _arr("int32_t", "SPANS", d["spans"]),
         _arr("uint8_t", "PROFILE", d["profile"]),
@@ -232,7 +248,7 @@ def emit(g: Grammar) -> str:
         f'  "{g.name}", {g.q}, {g.k},',
         f"  {d['nstate']}, {d['nendo']}, {d['nclass']},",
         f"  {ns}::BYTE_CLASS, {ns}::STEP, {ns}::COMPOSE, {ns}::ENDO_INIT,",
-        f"  {ns}::DELTA_LEX, {ns}::DELTA_BOTH,",
+        f"  {ns}::DELTA_LEX, {ns}::DELTA_BOTH, {ns}::TOKEN_LEX, {ns}::TOKEN_BOTH,",
         f"  {d['ignore_lex']}, {d['ignore_both']}, {d['dead_lex']}, {d['dead_both']},",
         f"  {g.terminal_bytes_lex}, {g.terminal_bytes_both}, {g.index_bytes}, {g.length_bytes},",
         f"  {p.empty_t}, {p.start_t}, {p.end_t}, {d['base']},",
