@@ -19,16 +19,22 @@
 //                    the first tile of every input;
 //   L3 lex_emit      every lane starts from the DFA state
 //                    ENDO_INIT[tile prefix . lane prefix] and walks its bytes
-//                    through the LDS token DFA (next | produce | kept),
-//                    counting kept tokens and its last boundary; a block scan
+//                    through the LDS token DFA (rows of next-row address |
+//                    produce | kept, and the state's terminal), keeping its
+//                    boundaries and kept-token closes as 128-bit masks and
+//                    the terminal of each kept token in LDS; a block scan
 //                    plus a decoupled lookback over (kept-token count, last
 //                    boundary) -- sum and max, so a whole wave inspects 64
 //                    predecessor tiles per probe -- gives the output slot and
 //                    the start of every lane's first token (never before its
-//                    input's first byte); a second walk stages (terminal,
-//                    start, end) in LDS by slot, and the workgroup writes the
+//                    input's first byte); each lane then visits its kept
+//                    tokens from the masks (end = the close, start = the last
+//                    boundary before it), stages (terminal, start, end) in
+//                    LDS by slot, and the workgroup writes the
 //                    structure-of-arrays output with consecutive lanes on
-//                    consecutive slots.
+//                    consecutive slots.  A tile with a lane of more kept
+//                    tokens than the LDS holds terminals for re-walks its
+//                    bytes instead.
 #include "device.hpp"
 #include "lexer.hpp"
 
@@ -51,7 +57,6 @@ struct TileSpan {
   uint64_t seg_hi;
   uint32_t seg;
   uint32_t first;   // the input's first tile
-  uint32_t full;    // TILE bytes of the input
   uint32_t shift;   // byte misalignment of the tile start (the same for every lane)
 };
 
@@ -64,12 +69,11 @@ __device__ __forceinline__ TileSpan tile_span(const LexParams& p, uint32_t tile)
   t.seg_lo = p.seg_off[seg];
   t.seg_hi = p.seg_off[seg + 1];
   t.tbase = t.seg_lo + (uint64_t)(tile - tf) * TILE;
-  t.full = t.tbase + TILE <= t.seg_hi;
   t.shift = (uint32_t)((uintptr_t)(p.in + t.tbase) & 3u);
   return t;
 }
 
-// The dwords holding the lane's IPT bytes of a full tile: a[0] holds its
+// The dwords holding the lane's IPT bytes (all in its input): a[0] holds its
 // first byte at byte `shift`, a[IPT / 4] is read only when the tile is
 // misaligned (it holds the lane's last byte), so no byte outside the dwords
 // of the lane's own bytes is touched.
@@ -113,10 +117,12 @@ __global__ __launch_bounds__(WG) void lex_fold(LexParams p) {
   for (uint32_t tile = blockIdx.x; tile < p.ntiles; tile += gridDim.x) {
     const TileSpan ts = tile_span(p, tile);
     const uint64_t base = ts.tbase + (uint64_t)tid * IPT;
+    const uint32_t valid = base >= ts.seg_hi ? 0u : (uint32_t)min((uint64_t)IPT, ts.seg_hi - base);
     uint32_t e = 0;  // identity
-    // full tiles: fully unrolled (the byte registers are indexed with
-    // constants); an input's last tile reads its bytes one by one
-    if (ts.full) {
+    // lanes with all IPT bytes: fully unrolled (the byte registers are
+    // indexed with constants); the lane holding an input's last bytes reads
+    // them one by one
+    if (valid == IPT) {
       uint32_t a[IPT / 4 + 1];
       load_lane(p, base, ts.shift, a);
 #pragma unroll
@@ -126,7 +132,6 @@ __global__ __launch_bounds__(WG) void lex_fold(LexParams p) {
         for (int b = 0; b < 4; ++b) e = s_step[e * nclass + s_class[(wk >> (8 * b)) & 0xFFu]];
       }
     } else {
-      const uint32_t valid = base >= ts.seg_hi ? 0u : (uint32_t)min((uint64_t)IPT, ts.seg_hi - base);
       for (uint32_t j = 0; j < valid; ++j) e = s_step[e * nclass + s_class[p.in[base + j]]];
     }
     uint32_t inc = e;
@@ -201,6 +206,33 @@ __global__ __launch_bounds__(1024) void lex_tile_scan(LexParams p) {
 // ------------------------------------------------------------------- L3
 
 constexpr uint32_t STAGE = 2048;  // kept tokens staged in LDS per round
+constexpr uint32_t ROW = 257;     // LDS row of a DFA state: 256 transitions, then its terminal
+constexpr uint32_t ROWB = ROW * 2;
+
+// Transition entries of the LDS rows (u16).  SCALED (at most 31 states):
+// bits 0..13 the byte offset of the next state's row, bit 14 produce, bit
+// 15 kept; otherwise the next state in bits 0..7, produce bit 8, kept bit 9.
+// Rows are addressed by their LDS address (lds0 + state * ROWB, lds0 the
+// address of the dynamic LDS).
+template <bool SCALED>
+struct RowCode {
+  static constexpr uint32_t PROD = SCALED ? 14 : 8, KEPT = SCALED ? 15 : 9;
+  __device__ static uint16_t encode(uint32_t d, uint32_t lds0) {
+    const uint32_t next = d & 0xFFu, pr = (d >> 8) & 1u, kp = (d >> 9) & 1u;
+    return (uint16_t)((SCALED ? lds0 + next * ROWB : next) | pr << PROD | kp << KEPT);
+  }
+  __device__ static uint32_t next_row(uint32_t x, uint32_t lds0) {
+    return SCALED ? (x & 0x3FFFu) : lds0 + (x & 0xFFu) * ROWB;
+  }
+};
+
+typedef __attribute__((address_space(3))) uint8_t lds_u8;
+typedef __attribute__((address_space(3))) uint16_t lds_u16;
+__device__ __forceinline__ uint32_t lds_addr(const void* p) {
+  return (uint32_t)(uintptr_t)(const lds_u8*)p;
+}
+__device__ __forceinline__ uint32_t lds_load16(uint32_t a) { return *(const lds_u16*)a; }
+__device__ __forceinline__ void lds_store8(uint32_t a, uint32_t v) { *(lds_u8*)a = (uint8_t)v; }
 
 // DFA state of the lane's first byte.
 __device__ __forceinline__ uint32_t lane_state(const LexParams& p, uint32_t tile, uint32_t tid) {
@@ -208,71 +240,108 @@ __device__ __forceinline__ uint32_t lane_state(const LexParams& p, uint32_t tile
   return p.endo_init[pre];
 }
 
+// Workgroup words of lex_emit, kept after its dynamic LDS arrays (the
+// kernel has no static LDS, so the DFA rows start at LDS address 0 and a
+// row offset is an LDS address).
+struct EmitWords {
+  uint32_t tile, pre_cnt, pre_last, pad;
+  uint32_t wave_cnt[NW], wave_last[NW], wave_over[NW];
+};
+
+// LDS layout of lex_emit: DFA rows, the staged slots, the lanes' kept-token
+// terminals (rcap rows of WG bytes and one sink byte), the workgroup words.
+struct EmitLayout {
+  uint32_t start, end, term, lterm, words, total;
+  __host__ __device__ EmitLayout(uint32_t nstate, uint32_t rcap) {
+    start = (nstate * ROWB + 15u) & ~15u;
+    end = start + 4 * STAGE;
+    term = end + 2 * STAGE;
+    lterm = term + STAGE;
+    words = (lterm + rcap * WG + 1 + 15u) & ~15u;
+    total = words + (uint32_t)sizeof(EmitWords);
+  }
+};
+
+template <bool SCALED>
 __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
+  using RC = RowCode<SCALED>;
   extern __shared__ __align__(16) uint8_t lds[];
-  uint16_t* s_delta = reinterpret_cast<uint16_t*>(lds);  // nstate*256
-  const uint32_t delta_bytes = ((uint32_t)p.nstate * 512u + 15u) & ~15u;
-  uint32_t* s_start = reinterpret_cast<uint32_t*>(lds + delta_bytes);  // [STAGE] token start
-  uint16_t* s_end = reinterpret_cast<uint16_t*>(s_start + STAGE);       // [STAGE] end, tile-relative
-  uint8_t* s_term = reinterpret_cast<uint8_t*>(s_end + STAGE);          // [STAGE] terminal
-  __shared__ uint8_t s_token[256];                                       // terminal each state accepts
-  __shared__ uint32_t s_tile;
-  __shared__ uint32_t s_wave_cnt[NW], s_wave_last[NW];
-  __shared__ uint32_t s_pre_cnt, s_pre_last;
+  const uint32_t rcap = p.rcap;
+  const EmitLayout L((uint32_t)p.nstate, rcap);
+  uint16_t* s_rows = reinterpret_cast<uint16_t*>(lds);              // [nstate][ROW]
+  uint32_t* s_start = reinterpret_cast<uint32_t*>(lds + L.start);   // [STAGE] token start
+  uint16_t* s_end = reinterpret_cast<uint16_t*>(lds + L.end);       // [STAGE] end, tile-relative
+  uint8_t* s_term = lds + L.term;                                   // [STAGE] terminal
+  EmitWords& w = *reinterpret_cast<EmitWords*>(lds + L.words);
   const uint32_t tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
-  for (int i = tid; i < p.nstate * 256; i += WG) s_delta[i] = p.delta[i];
-  for (int i = tid; i < p.nstate; i += WG) s_token[i] = p.token[i];
+  const uint32_t lds0 = lds_addr(lds);
+  for (uint32_t i = tid; i < (uint32_t)p.nstate * ROW; i += WG) {
+    const uint32_t s = i / ROW, b = i - s * ROW;
+    s_rows[i] = b == 256 ? (uint16_t)p.token[s] : RC::encode(p.delta[s * 256 + b], lds0);
+  }
   const int ignore = p.ignore;
+  // s_lterm[r][lane] is the byte at LDS address lterm0 + r * WG + lane
+  const uint32_t lterm0 = lds0 + L.lterm, sink = lterm0 + rcap * WG;
 
   for (;;) {
-    if (tid == 0) s_tile = atomicAdd(p.ticket, 1u);
+    if (tid == 0) w.tile = atomicAdd(p.ticket, 1u);
     __syncthreads();
-    const uint32_t tile = s_tile;
+    const uint32_t tile = w.tile;
     if (tile >= p.ntiles) break;
     const TileSpan ts = tile_span(p, tile);
     const uint64_t tbase = ts.tbase, n = ts.seg_hi;
     const uint64_t base = tbase + (uint64_t)tid * IPT;
     const uint32_t valid = base >= n ? 0u : (base + IPT <= n ? (uint32_t)IPT : (uint32_t)(n - base));
-    const bool full = ts.full;
-    uint32_t a[IPT / 4 + 1];
-    if (full) load_lane(p, base, ts.shift, a);
-    const uint32_t s0 = lane_state(p, tile, tid);
+    const uint32_t row0 = lds0 + lane_state(p, tile, tid) * ROWB;
 
-    // walk 1: kept tokens closed in the lane and its last boundary.  Only
-    // an input's last tile is partial: full tiles walk the registers fully
-    // unrolled with no bounds checks, a partial one reads its bytes one by
-    // one.
-    // boundaries as bits (shift counts are inline constants, positions
-    // above 64 would not be)
-    uint32_t s = s0, cnt = 0, bm[IPT / 32] = {0, 0, 0, 0};
-    auto step1 = [&](uint32_t byte, uint32_t j) {
-      const uint32_t x = s_delta[s * 256 + byte];
-      bm[j >> 5] |= ((x >> 8) & 1u) << (j & 31);
-      cnt += (x >> 9) & 1u;
-      s = x & 0xFFu;
+    // walk 1: the lane's boundaries (bm) and kept-token closes (km) as bit
+    // masks, the terminal of its r-th kept token in s_lterm[r][lane] (every
+    // byte writes the slot of the next kept token; writes past rcap tokens
+    // land on one sink byte).  Lanes with all IPT bytes walk the registers
+    // fully unrolled; a lane holding an input's last bytes reads them one by
+    // one in 32-byte groups.
+    uint32_t row = row0, waddr = lterm0 + tid, bm[IPT / 32] = {0, 0, 0, 0}, km[IPT / 32] = {0, 0, 0, 0};
+    auto step1 = [&](uint32_t byte, uint32_t& mb, uint32_t& mk, uint32_t bit) {
+      const uint32_t x = lds_load16(row + 2 * byte);
+      const uint32_t t = lds_load16(row + 512);
+      const uint32_t kp = (x >> RC::KEPT) & 1u;
+      mb |= ((x >> RC::PROD) & 1u) << bit;
+      mk |= kp << bit;
+      lds_store8(waddr < sink ? waddr : sink, t);
+      waddr += kp << 8;
+      asm volatile("" : "+v"(waddr));  // a running address, not rebuilt from a count
+      row = RC::next_row(x, lds0);
     };
-    if (full) {
+    if (valid == IPT) {
+      uint32_t a[IPT / 4 + 1];
+      load_lane(p, base, ts.shift, a);
 #pragma unroll
       for (int k = 0; k < IPT / 4; ++k) {
-        const uint32_t wk = lane_word(a, k, ts.shift, s);
+        const uint32_t wk = lane_word(a, k, ts.shift, row);
 #pragma unroll
-        for (int b = 0; b < 4; ++b) step1((wk >> (8 * b)) & 0xFFu, 4 * k + b);
+        for (int b = 0; b < 4; ++b) step1((wk >> (8 * b)) & 0xFFu, bm[k >> 3], km[k >> 3], (4 * k + b) & 31);
       }
     } else {
-      for (uint32_t j = 0; j < valid; ++j) step1(p.in[base + j], j);
+#pragma unroll
+      for (int q = 0; q < IPT / 32; ++q)
+        for (uint32_t j = 32 * q; j < valid && j < 32u * q + 32; ++j) step1(p.in[base + j], bm[q], km[q], j & 31);
     }
+    uint32_t cnt = 0;  // kept tokens closed inside the lane
+#pragma unroll
+    for (int q = 0; q < IPT / 32; ++q) cnt += __builtin_popcount(km[q]);
+    const bool wave_over = __ballot(cnt > rcap) != 0;
     const bool owns_end = valid > 0 && base + valid == n;
     uint32_t fin_t = 0;
     bool fin_kept = false;
     if (owns_end) {
-      fin_t = s_token[s];
+      fin_t = lds_load16(row + 512);
       if ((int)fin_t == p.dead) {
         atomicOr(p.seg_flags + ts.seg, LEX_REJECT_FINAL);
       } else if ((int)fin_t != ignore) {
         fin_kept = true;
-        ++cnt;
       }
     }
+    cnt += fin_kept;
     uint32_t last = 0;
 #pragma unroll
     for (int q = 0; q < IPT / 32; ++q)
@@ -282,18 +351,21 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
     uint32_t icnt = cnt, ilast = last;
     wave_scan_add_max(icnt, ilast);
     if (lane == WAVE - 1) {
-      s_wave_cnt[wid] = icnt;
-      s_wave_last[wid] = ilast;
+      w.wave_cnt[wid] = icnt;
+      w.wave_last[wid] = ilast;
+      w.wave_over[wid] = wave_over;
     }
     __syncthreads();
     uint32_t wc = 0, wl = 0, tc = 0, tl = 0;
+    bool over = false;
     for (uint32_t i = 0; i < NW; ++i) {
       if (i < wid) {
-        wc += s_wave_cnt[i];
-        wl = wl > s_wave_last[i] ? wl : s_wave_last[i];
+        wc += w.wave_cnt[i];
+        wl = wl > w.wave_last[i] ? wl : w.wave_last[i];
       }
-      tc += s_wave_cnt[i];
-      tl = tl > s_wave_last[i] ? tl : s_wave_last[i];
+      tc += w.wave_cnt[i];
+      tl = tl > w.wave_last[i] ? tl : w.wave_last[i];
+      over |= w.wave_over[i] != 0;
     }
     const uint32_t pc = __shfl_up(icnt, 1, WAVE), pl = __shfl_up(ilast, 1, WAVE);
     const uint32_t ecnt = wc + (lane ? pc : 0u);
@@ -339,56 +411,69 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
         }
       }
       if (lane == 0) {
-        s_pre_cnt = xc;
-        s_pre_last = xl;
+        w.pre_cnt = xc;
+        w.pre_last = xl;
       }
     }
     __syncthreads();
-    const uint64_t slot0 = s_pre_cnt;  // output slot of the tile's first kept token
-    uint32_t start0 = s_pre_last > elast ? s_pre_last : elast;
+    const uint64_t slot0 = w.pre_cnt;  // output slot of the tile's first kept token
+    uint32_t start0 = w.pre_last > elast ? w.pre_last : elast;
     start0 = start0 > (uint32_t)ts.seg_lo ? start0 : (uint32_t)ts.seg_lo;  // an input's first token starts at its first byte
     if (tid == 0 && ts.first) p.seg_tok[ts.seg] = slot0;
     if (tid == 0 && tile == p.ntiles - 1) p.seg_tok[p.nseg] = slot0 + tc;
     bool overflow = false;
     const uint32_t lane_off = tid * IPT;
 
-    // walk 2, per round of STAGE slots: the lanes owning slots of the round
-    // re-walk and stage (terminal, start, end); then the workgroup writes the
-    // round out with consecutive lanes on consecutive slots
+    // per round of STAGE slots, the lanes owning slots of the round stage
+    // (terminal, start, end) by slot; then the workgroup writes the round
+    // out with consecutive lanes on consecutive slots
     for (uint32_t lo = 0; lo < tc; lo += STAGE) {
       const uint32_t hi = lo + STAGE;
       if (cnt > 0 && ecnt < hi && ecnt + cnt > lo) {
-        // the lane's offsets are opaque here, so the 128 positions are not
-        // kept live in registers across the rounds
-        uint32_t vbase = (uint32_t)base, voff = lane_off;
-        asm volatile("" : "+v"(vbase), "+v"(voff));
-        uint32_t st = s0, k = ecnt, start = start0;
-        auto step2 = [&](uint32_t byte, uint32_t j) {
-          const uint32_t x = s_delta[st * 256 + byte];
-          if (x & 0x200u) {  // a kept token closes before byte j
-            const uint32_t i = k - lo;
-            if (i < STAGE) {
-              s_term[i] = s_token[st];
-              s_start[i] = start;
-              s_end[i] = (uint16_t)(voff + j);
+        const uint32_t d = ecnt - lo;  // staging index of the lane's first token (mod 2^32)
+        uint32_t k = 0, start = start0;
+        if (!over) {
+          // kept token k closes before byte j (bit j of km) and starts at the
+          // lane's last boundary before j, else where the lane's first token starts
+#pragma unroll
+          for (int q = 0; q < IPT / 32; ++q) {
+            uint32_t m = km[q];
+            const uint32_t pm = bm[q], qb = (uint32_t)base + 32 * q;
+            while (m) {
+              const uint32_t jb = __builtin_ctz(m);
+              m &= m - 1;
+              const uint32_t below = pm & ((1u << jb) - 1u);
+              const uint32_t i = d + k;
+              if (i < STAGE) {
+                s_term[i] = lds[L.lterm + k * WG + tid];
+                s_start[i] = below ? qb + 31 - __builtin_clz(below) : start;
+                s_end[i] = (uint16_t)(lane_off + 32 * q + jb);
+              }
+              ++k;
             }
-            ++k;
-          }
-          start = (x & 0x100u) ? vbase + j : start;
-          st = x & 0xFFu;
-        };
-        if (full) {
-#pragma unroll
-          for (int k = 0; k < IPT / 4; ++k) {
-            const uint32_t wk = lane_word(a, k, ts.shift, st);
-#pragma unroll
-            for (int b = 0; b < 4; ++b) step2((wk >> (8 * b)) & 0xFFu, 4 * k + b);
+            start = pm ? qb + 31 - __builtin_clz(pm) : start;
           }
         } else {
-          for (uint32_t j = 0; j < valid; ++j) step2(p.in[base + j], j);
+          // some lane of the tile closes more than rcap kept tokens: re-walk
+          // the bytes through the DFA
+          uint32_t r = row0;
+          for (uint32_t j = 0; j < valid; ++j) {
+            const uint32_t x = lds_load16(r + 2 * p.in[base + j]);
+            if ((x >> RC::KEPT) & 1u) {
+              const uint32_t i = d + k;
+              if (i < STAGE) {
+                s_term[i] = (uint8_t)lds_load16(r + 512);
+                s_start[i] = start;
+                s_end[i] = (uint16_t)(lane_off + j);
+              }
+              ++k;
+            }
+            start = ((x >> RC::PROD) & 1u) ? (uint32_t)base + j : start;
+            r = RC::next_row(x, lds0);
+          }
         }
         if (fin_kept) {
-          const uint32_t i = k - lo;
+          const uint32_t i = d + k;
           if (i < STAGE) {
             s_term[i] = (uint8_t)fin_t;
             s_start[i] = start;
@@ -432,9 +517,27 @@ hipError_t occupancy_grid(K k, size_t lds, int* grid) {
 }  // namespace
 
 size_t lex_fold_lds(const LexParams& p) { return 256 + (size_t)p.nendo * p.nclass * 2; }
-size_t lex_emit_lds(const LexParams& p) {
-  return (((size_t)p.nstate * 512 + 15) & ~size_t(15)) + (size_t)STAGE * (4 + 2 + 1);
+namespace {
+// Rows hold next-row LDS addresses when those fit in 14 bits: lex_emit has
+// no static LDS, so its dynamic LDS starts at address 0.
+bool emit_scaled(const LexParams& p) {
+  static const bool no_static_lds = [] {
+    hipFuncAttributes fa{};
+    return hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(lex_emit<true>)) == hipSuccess &&
+           fa.sharedSizeBytes == 0;
+  }();
+  return no_static_lds && (uint32_t)p.nstate * ROWB <= 0x4000u;
 }
+// Kept tokens per lane whose terminals walk 1 keeps in LDS: as many as fit
+// in 40 KiB per workgroup (four workgroups per CU), between 16 and IPT.
+uint32_t emit_rcap(const LexParams& p) {
+  const uint32_t budget = 40 * 1024, fixed = EmitLayout((uint32_t)p.nstate, 0).total + 16;
+  const uint32_t r = fixed < budget ? (budget - fixed) / WG : 0;
+  return r < 16 ? 16 : (r > (uint32_t)IPT ? (uint32_t)IPT : r);
+}
+}  // namespace
+
+size_t lex_emit_lds(const LexParams& p) { return EmitLayout((uint32_t)p.nstate, emit_rcap(p)).total; }
 __global__ void tile_map(const uint32_t* tile_first, uint32_t nseg, uint32_t ntiles, uint32_t* tile_seg) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntiles) return;
@@ -453,7 +556,8 @@ __global__ void tile_map(const uint32_t* tile_first, uint32_t nseg, uint32_t nti
 hipError_t lex_grid_size(const LexParams& p, int* fold_grid, int* emit_grid) {
   hipError_t e = occupancy_grid(lex_fold, lex_fold_lds(p), fold_grid);
   if (e != hipSuccess) return e;
-  return occupancy_grid(lex_emit, lex_emit_lds(p), emit_grid);
+  return emit_scaled(p) ? occupancy_grid(lex_emit<true>, lex_emit_lds(p), emit_grid)
+                        : occupancy_grid(lex_emit<false>, lex_emit_lds(p), emit_grid);
 }
 
 hipError_t lex_launch(const LexParams& p, int fold_grid, int emit_grid, hipStream_t st, KernelTimer& tm) {
@@ -467,7 +571,12 @@ hipError_t lex_launch(const LexParams& p, int fold_grid, int emit_grid, hipStrea
   if ((e = tm.end(st)) != hipSuccess || (e = tm.begin(st, "lex_tile_scan")) != hipSuccess) return e;
   hipLaunchKernelGGL(lex_tile_scan, dim3(1), dim3(1024), 0, st, p);
   if ((e = tm.end(st)) != hipSuccess || (e = tm.begin(st, "lex_emit")) != hipSuccess) return e;
-  hipLaunchKernelGGL(lex_emit, dim3(g3), dim3(WG), lex_emit_lds(p), st, p);
+  LexParams q = p;
+  q.rcap = emit_rcap(p);
+  if (emit_scaled(p))
+    hipLaunchKernelGGL(lex_emit<true>, dim3(g3), dim3(WG), lex_emit_lds(p), st, q);
+  else
+    hipLaunchKernelGGL(lex_emit<false>, dim3(g3), dim3(WG), lex_emit_lds(p), st, q);
   if ((e = tm.end(st)) != hipSuccess) return e;
   return hipGetLastError();
 }

@@ -50,6 +50,7 @@ struct LexParams {
   uint64_t* count_status;  // [ntiles] lookback words, zeroed before every launch
   uint32_t* ticket;        // [1] zeroed before every launch
   uint32_t ntiles;
+  uint32_t rcap;  // set by lex_launch: kept tokens per lane lex_emit keeps terminals for
 };
 
 __device__ __forceinline__ void store_length(const LexParams& p, uint64_t slot, uint64_t len) {

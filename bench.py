@@ -180,7 +180,8 @@ def batch_workload(be, count: int, steps: int, seed: int = 3):
     torch.cuda.synchronize()
     plain_s = time.perf_counter() - t0
     del dev
-    return {
+    items = [("json", host[int(offs[i]):int(offs[i + 1])].tobytes()) for i in range(count)]
+    return items, {
         "workload": f"{count} json inputs, lognormal(ln 4096, 1.5) bytes clipped to [64, 4 MiB], seed {seed}",
         "count": count, "bytes": total, "valid": valid, "plain_valid": plain_valid,
         "batched_ms": round(batched_s * 1e3, 3), "batched_GBps": round(total / batched_s / 1e9, 3),
@@ -218,9 +219,11 @@ def serve_workload(rates, seconds: float, seed: int = 4, plain: bool = True):
     from alpacc_amd import hip
     names = ["json", "arithmetic"]
     out = []
+    items = []
     for rate in rates:
         count = max(1, int(rate * seconds))
         gidx, payloads, arrival = serve_trace(count, rate, seed + int(rate))
+        items = [(names[g], x) for g, x in zip(gidx, payloads)]
         total = sum(len(x) for x in payloads)
         modes = ["both"] * count
         row = {"rate_req_s": rate, "requests": count, "bytes": total}
@@ -240,7 +243,36 @@ def serve_workload(rates, seconds: float, seed: int = 4, plain: bool = True):
         if plain:
             row["throughput_vs_plain"] = round(row["served"]["GBps"] / max(row["plain"]["GBps"], 1e-9), 2)
         out.append(row)
-    return out
+    return out, items
+
+
+def _oracle_worker(args):
+    """Seconds and bytes of the sequential oracle over a list of (grammar, payload)."""
+    mode, items = args
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    secs, nbytes = 0.0, 0
+    for grammar, payload in items:
+        secs += pyoracle.time_case(grammar, mode, payload, len(payload))
+        nbytes += len(payload)
+    return secs, nbytes
+
+
+def cpu_parallel_baseline(items, mode: str, label: str, cores: int = 0):
+    """The oracle over `items` ((grammar, payload) pairs) in `cores` worker
+    processes on this host (default: min(16, cpu count)); aggregate GB/s over
+    the wall-clock of the parallel run."""
+    import multiprocessing as mp
+    cores = cores or max(1, min(16, os.cpu_count() or 1))
+    chunks = [(mode, items[i::cores]) for i in range(cores)]
+    with mp.get_context("spawn").Pool(cores) as pool:
+        pool.map(_oracle_worker, [(mode, items[:1])] * cores)  # load the oracle in every worker
+        t0 = time.perf_counter()
+        res = pool.map(_oracle_worker, chunks)
+        wall = time.perf_counter() - t0
+    nbytes = sum(b for _, b in res)
+    return {"value": round(nbytes / wall / 1e9, 4), "unit": "GB/s", "cores": cores, "kind": "port",
+            "sample": f"{label}: {len(items)} inputs, {nbytes} bytes, {cores} processes, {wall:.2f} s wall"}
 
 
 def main(argv=None):
@@ -340,18 +372,26 @@ def main(argv=None):
         "avg_ms": round(savg, 4),
     }
     batched = None
+    batch_items = serve_items = None
     if a.batch_count:
         log(f"batching workload: {a.batch_count} inputs")
-        batched = batch_workload(be, a.batch_count, a.batch_steps)
+        batch_items, batched = batch_workload(be, a.batch_count, a.batch_steps)
         log(f"batched {batched['batched_GBps']} GB/s, plain {batched['plain_GBps']} GB/s, "
             f"x{batched['speedup_vs_plain']}")
     served = None
     if a.serve_rates:
         rates = [float(x) for x in a.serve_rates.split(",") if x]
-        served = serve_workload(rates, a.serve_seconds)
+        served, serve_items = serve_workload(rates, a.serve_seconds)
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline("json", "both", 1, conforming, min(a.cpu_sample, n))
+        if batched:
+            batched["cpu_baseline"] = cpu_parallel_baseline(batch_items, "both", "batching workload")
+        if served:
+            served_items = serve_items[:max(1, len(serve_items) // 5)]  # a bounded sample of the trace
+            cpu_srv = cpu_parallel_baseline(served_items, "both", "first fifth of the serving trace at the highest rate")
+            for row in served:
+                row["cpu_baseline"] = cpu_srv
     path_bytes = n + 19 * T + 9 * N  # input + tokens + tree (combined-mode outputs)
     out = {
         "metric": "lex+parse input GB/s (1 GiB, %HBM peak); batched GB/s; served p50/p99 ms",
