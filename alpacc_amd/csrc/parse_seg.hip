@@ -384,13 +384,19 @@ struct Resolve {
 
 // Last lane l <= from whose minimum (tb[0][l]) is below L, or a negative
 // value: tb[k][l] is the minimum over lanes [l - 2^k + 1, l] (clipped at 0)
-// and maximal blocks with minimum >= L are skipped, largest first.
-__device__ __forceinline__ int prev_below(const int16_t (*tb)[RWG], int from, int32_t L) {
+// and maximal blocks with minimum >= L are skipped, largest first.  *min
+// receives the found lane's minimum.
+__device__ __forceinline__ int prev_below(const int16_t (*tb)[RWG], int from, int32_t L, int32_t* min) {
   int l = from;
-  if (l < 0 || tb[0][l] < L) return l;
+  if (l < 0) return l;
+  int32_t m = tb[0][l];
+  if (m >= L) {
 #pragma unroll
-  for (int k = RLV - 1; k >= 0; --k)
-    if (l >= 0 && tb[k][l] >= L) l -= 1 << k;
+    for (int k = RLV - 1; k >= 0; --k)
+      if (l >= 0 && tb[k][l] >= L) l -= 1 << k;
+    m = l >= 0 ? tb[0][l] : 0;
+  }
+  *min = m;
   return l;
 }
 
@@ -605,9 +611,10 @@ __global__ __launch_bounds__(RWG) void seg_resolve(Resolve r) {
   for (int u = 0; u < bun; ++u) {
     const int32_t D = bS - u;
     const uint16_t sym = bst[(bcap - 1 - u) * RWG];
-    l = prev_below(tbb, l, D);
+    int32_t m;
+    l = prev_below(tbb, l, D, &m);
     if (l >= 0)
-      bad |= dyn[(D - tbb[0][l] - 1) * RWG + l] != sym;
+      bad |= dyn[(D - m - 1) * RWG + l] != sym;
     else
       r.ub_list[ti.ub_off + (-D)] = sym;
   }
@@ -623,11 +630,12 @@ __global__ __launch_bounds__(RWG) void seg_resolve(Resolve r) {
     for (int u = 0; u < sun; ++u) {
       const int32_t D = sS - u;
       const uint32_t tg = sst[(scap - 1 - u) * RWG];
-      l = prev_below(tbs, l, D);
+      int32_t m;
+      l = prev_below(tbs, l, D, &m);
       const bool term = tg & TERM_SLOT;
       const int32_t idx = (term ? term0 : cb0) + (int32_t)(tg & 0x7Fu);
       if (l >= 0) {
-        const uint16_t pc = (uint16_t)(l_cb[l] + sall[(D - tbs[0][l] - 1) * RWG + l]);
+        const uint16_t pc = (uint16_t)(l_cb[l] + sall[(D - m - 1) * RWG + l]);
         if (term)
           s_tpar[idx] = pc;
         else

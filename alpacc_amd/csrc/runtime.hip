@@ -36,12 +36,21 @@ int fail(int code, const char* fmt, ...) {
 
 int DevBuf::reserve(size_t want) {
   if (want <= cap) return 0;
+  const size_t old = cap;
   if (ptr) (void)hipFree(ptr);
   ptr = nullptr;
   cap = 0;
-  size_t sz = want < 256 ? 256 : want;
+  // at least double: a workspace that keeps growing (a server's batches)
+  // is reallocated a logarithmic number of times, and hipFree waits for the device
+  size_t sz = want < 2 * old ? 2 * old : want;
+  sz = sz < 256 ? 256 : sz;
   sz = (sz + 255) & ~size_t(255);
   hipError_t e = hipMalloc(&ptr, sz);
+  if (e != hipSuccess && sz > want) {  // no room to double: the exact size
+    (void)hipGetLastError();
+    sz = (want + 255) & ~size_t(255);
+    e = hipMalloc(&ptr, sz);
+  }
   if (e != hipSuccess) {
     ptr = nullptr;
     return fail(ALPACC_ERR_NOMEM, "hipMalloc(%zu): %s", sz, hipGetErrorString(e));
@@ -58,12 +67,19 @@ void DevBuf::release() {
 
 int PinnedBuf::reserve(size_t want) {
   if (want <= cap) return 0;
+  const size_t old = cap;
   if (ptr) (void)hipHostFree(ptr);
   ptr = nullptr;
   cap = 0;
-  size_t sz = want < 4096 ? 4096 : want;
+  size_t sz = want < 2 * old ? 2 * old : want;  // at least double, as DevBuf
+  sz = sz < 4096 ? 4096 : sz;
   sz = (sz + 4095) & ~size_t(4095);
   hipError_t e = hipHostMalloc(&ptr, sz, hipHostMallocDefault);
+  if (e != hipSuccess && sz > want) {
+    (void)hipGetLastError();
+    sz = (want + 4095) & ~size_t(4095);
+    e = hipHostMalloc(&ptr, sz, hipHostMallocDefault);
+  }
   if (e != hipSuccess) {
     ptr = nullptr;
     return fail(ALPACC_ERR_NOMEM, "hipHostMalloc(%zu): %s", sz, hipGetErrorString(e));

@@ -51,6 +51,11 @@ struct Req {
 // tree and the parser's intermediates; a conservative 64 per input byte.
 uint64_t estimate(uint64_t n) { return n * 64 + 4096; }
 
+// Payload bytes a batch takes at most (one larger request is a batch of its
+// own): past a few MiB a batch no longer gains throughput, only latency and
+// workspace growth.
+constexpr uint64_t kBatchBytes = 8ull << 20;
+
 }  // namespace
 
 struct alpacc_hip_server {
@@ -102,11 +107,13 @@ void worker(alpacc_hip_server* s, uint32_t g, alpacc_hip_ctx* ctx) {
       if (mode < 0) return;  // stopping with nothing left for this grammar
       auto& d = s->q[g * 3 + mode];
       batch.clear();
+      uint64_t bytes = 0;
       while (!d.empty() && (s->max_batch == 0 || batch.size() < s->max_batch)) {
         Req* r = d.front();
-        if (!batch.empty() && s->in_use + est + r->est > s->budget) break;
+        if (!batch.empty() && (s->in_use + est + r->est > s->budget || bytes + r->n > kBatchBytes)) break;
         batch.push_back(r);
         est += r->est;
+        bytes += r->n;
         d.pop_front();
       }
       s->in_use += est;

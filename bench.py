@@ -229,8 +229,9 @@ def serve_workload(rates, seconds: float, seed: int = 4, plain: bool = True):
         row = {"rate_req_s": rate, "requests": count, "bytes": total}
         for label, lanes, max_batch in (("served", 2, 0), ("plain", 1, 1)) if plain else (("served", 2, 0),):
             srv = hip.Server(names, lanes=lanes, hbm_budget=64 << 30, max_batch=max_batch)
-            srv.replay(gidx[:min(count, 64)], modes[:min(count, 64)], payloads[:min(count, 64)],
-                       np.zeros(min(count, 64), dtype=np.uint64))  # warmup
+            # warmup: one untimed replay of the trace, so every lane's
+            # workspaces have grown to its batches before the timed replay
+            srv.replay(gidx, modes, payloads, arrival)
             recs, lat = srv.replay(gidx, modes, payloads, arrival)
             st = srv.stats()
             srv.close()

@@ -171,8 +171,9 @@ int alpacc_hip_lex_parse_batch_device(alpacc_hip_ctx *ctx, const uint8_t *d_pack
  * 167-189).  Every grammar gets `lanes` contexts (own stream, buffers and
  * worker thread); a free lane takes every request queued for its grammar in
  * one mode, up to `hbm_budget` bytes of estimated device memory admitted at
- * once (0 = 64 GiB) and `max_batch` requests (0 = no limit), and answers them
- * with one batched launch sequence.  Records are byte-identical to
+ * once (0 = 64 GiB), `max_batch` requests (0 = no limit) and 8 MiB of
+ * payload (a larger request is a batch of its own), and answers them with
+ * one batched launch sequence.  Records are byte-identical to
  * alpacc_hip_run_case's for each request alone. */
 typedef struct alpacc_hip_server alpacc_hip_server;
 
