@@ -458,6 +458,7 @@ __global__ __launch_bounds__(RWG) void seg_resolve(Resolve r) {
   // [tcap] (tile-local compact indices), staged productions u8 [ncap]
   extern __shared__ __align__(16) uint16_t dyn[];
   __shared__ int16_t tbb[RLV][RWG], tbs[RLV][RWG];
+  __shared__ uint32_t pkb[RWG], pks[RWG];  // (lane minimum << 16) | previous smaller lane
   __shared__ uint16_t l_cb[RWG], l_tb[RWG];  // each lane's first node / terminal production (tile-local)
   __shared__ int32_t ws[4][RNW];
   __shared__ int32_t wmin[2][RNW];
@@ -603,18 +604,33 @@ __global__ __launch_bounds__(RWG) void seg_resolve(Resolve r) {
     tbs[k][tid] = c;
     __syncthreads();
   }
+  // every lane's previous strictly smaller lane minimum (-1: none): lanes
+  // between a lane and that one have minima >= its own
+  {
+    int32_t md;
+    const int pb = prev_below(tbb, (int)tid - 1, lbm, &md), ps = prev_below(tbs, (int)tid - 1, lsm, &md);
+    pkb[tid] = (uint32_t)(uint16_t)(int16_t)lbm << 16 | (uint16_t)(int16_t)(pb < 0 ? -1 : pb);
+    pks[tid] = (uint32_t)(uint16_t)(int16_t)lsm << 16 | (uint16_t)(int16_t)(ps < 0 ? -1 : ps);
+  }
+  __syncthreads();
 
-  // pops left over by this lane (depth falling, so the matching lane only
-  // moves back); entries left open at the tile end
+  // pops left over by this lane: a pop at tile-relative level D reads entry
+  // D - min - 1 of the last earlier lane whose minimum is below D.  Depth
+  // falls from pop to pop, so the lane only moves back, jumping from a lane
+  // to its previous smaller one; the current lane's packed word stays in a
+  // register.  Then the entries left open at the tile end.
   int l = (int)tid - 1;
+  uint32_t w = l >= 0 ? pkb[l] : 0u;
 #pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
   for (int u = 0; u < bun; ++u) {
     const int32_t D = bS - u;
     const uint16_t sym = bst[(bcap - 1 - u) * RWG];
-    int32_t m;
-    l = prev_below(tbb, l, D, &m);
+    while (l >= 0 && (int32_t)(int16_t)(w >> 16) >= D) {
+      l = (int16_t)(w & 0xFFFFu);
+      if (l >= 0) w = pkb[l];
+    }
     if (l >= 0)
-      bad |= dyn[(D - m - 1) * RWG + l] != sym;
+      bad |= dyn[(D - (int32_t)(int16_t)(w >> 16) - 1) * RWG + l] != sym;
     else
       r.ub_list[ti.ub_off + (-D)] = sym;
   }
@@ -626,16 +642,19 @@ __global__ __launch_bounds__(RWG) void seg_resolve(Resolve r) {
   if (r.tree) {
     const uint8_t* const sall = reinterpret_cast<const uint8_t*>(dyn + bcap * RWG);
     l = (int)tid - 1;
+    w = l >= 0 ? pks[l] : 0u;
 #pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
     for (int u = 0; u < sun; ++u) {
       const int32_t D = sS - u;
       const uint32_t tg = sst[(scap - 1 - u) * RWG];
-      int32_t m;
-      l = prev_below(tbs, l, D, &m);
+      while (l >= 0 && (int32_t)(int16_t)(w >> 16) >= D) {
+        l = (int16_t)(w & 0xFFFFu);
+        if (l >= 0) w = pks[l];
+      }
       const bool term = tg & TERM_SLOT;
       const int32_t idx = (term ? term0 : cb0) + (int32_t)(tg & 0x7Fu);
       if (l >= 0) {
-        const uint16_t pc = (uint16_t)(l_cb[l] + sall[(D - m - 1) * RWG + l]);
+        const uint16_t pc = (uint16_t)(l_cb[l] + sall[(D - (int32_t)(int16_t)(w >> 16) - 1) * RWG + l]);
         if (term)
           s_tpar[idx] = pc;
         else

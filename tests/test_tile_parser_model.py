@@ -177,12 +177,26 @@ def run(g, d, strings, tree):
                 if l >= 0 and tb[kk][l] >= Lv: l -= 1 << kk
             return l
         def sufmin(a, l): return min(a[l + 1:]) if l + 1 < RWG else 0x7FFFFFFF
+        # previous strictly smaller lane minimum (-1: none), from the sparse table
+        psb = [max(-1, prev_below(tbb, x - 1, mbl[x])) for x in range(RWG)]
+        pss = [max(-1, prev_below(tbs, x - 1, msl[x])) for x in range(RWG)]
+        for x in range(RWG):
+            for ps_, m_ in ((psb, mbl), (pss, msl)):
+                assert ps_[x] < x and (ps_[x] < 0 or m_[ps_[x]] < m_[x])
+                assert all(m_[y] >= m_[x] for y in range(max(ps_[x], -1) + 1, x))
+        def jump(ps_, m_, l, D):
+            # last lane <= l whose minimum is below D: lanes between l and its
+            # previous smaller lane have minima >= the minimum at l
+            while l >= 0 and m_[l] >= D:
+                l = ps_[l]
+            return l
         for ln in range(RWG):
             X = L[ln]
             l = ln - 1
             for u in range(X['bun']):
                 D = X['bS'] - u; sym = BST[ln][bcap - 1 - u]
-                l = prev_below(tbb, l, D)
+                l = jump(psb, mbl, l, D)
+                assert l == max(-1, prev_below(tbb, ln - 1, D))
                 if l >= 0:
                     idx = D - tbb[0][l] - 1; assert 0 <= idx < L[l]['bsp'], ('bst idx', idx)
                     if BST[l][idx] != sym: flags[sg] |= BAD_MATCH
@@ -199,7 +213,8 @@ def run(g, d, strings, tree):
                 l = ln - 1
                 for u in range(X['sun']):
                     D = X['sS'] - u; tg = SST[ln][scap - 1 - u]
-                    l = prev_below(tbs, l, D)
+                    l = jump(pss, msl, l, D)
+                    assert l == max(-1, prev_below(tbs, ln - 1, D))
                     term = tg & 0x80; idx = (X['term0'] if term else X['cb0']) + (tg & 0x7F)
                     if l >= 0:
                         e_ = D - tbs[0][l] - 1; assert 0 <= e_ < L[l]['ssp'], ('sst idx', e_)
