@@ -361,7 +361,7 @@ __global__ __launch_bounds__(256) void tile_downsweep(Seg s, const TileRec* rec,
 
 constexpr int RWG = AWG;          // lanes per tile
 constexpr int RNW = RWG / WAVE;   // waves per tile
-constexpr int RLV = 8;            // sparse-table levels: blocks of 1..128 lanes
+constexpr int WLV = 6;            // in-wave sparse-table levels: blocks of 1..32 lanes
 constexpr uint8_t TERM_SLOT = 0x80u;
 
 struct Resolve {
@@ -382,22 +382,25 @@ struct Resolve {
   int tree;
 };
 
-// Last lane l <= from whose minimum (tb[0][l]) is below L, or a negative
-// value: tb[k][l] is the minimum over lanes [l - 2^k + 1, l] (clipped at 0)
-// and maximal blocks with minimum >= L are skipped, largest first.  *min
-// receives the found lane's minimum.
-__device__ __forceinline__ int prev_below(const int16_t (*tb)[RWG], int from, int32_t L, int32_t* min) {
-  int l = from;
-  if (l < 0) return l;
-  int32_t m = tb[0][l];
-  if (m >= L) {
+// Last lane l <= from whose minimum (tb[0][l]) is below L, or -1.  tb[k][l]
+// is the minimum over lanes [max(w, l - 2^k + 1), l], w the first lane of
+// l's wave, pw[l] the minimum over [w, l] and wmin[v] the minimum of wave v:
+// the search descends inside from's wave when it holds such a lane, else
+// inside the nearest earlier wave that does, from its last lane.
+__device__ __forceinline__ int prev_below(const int16_t (*tb)[RWG], const int16_t* pw, const int32_t* wmin, int from,
+                                          int32_t L) {
+  auto descend = [&](int l) {  // a lane below L lies in [first lane of l's wave, l]
 #pragma unroll
-    for (int k = RLV - 1; k >= 0; --k)
-      if (l >= 0 && tb[k][l] >= L) l -= 1 << k;
-    m = l >= 0 ? tb[0][l] : 0;
-  }
-  *min = m;
-  return l;
+    for (int k = WLV - 1; k >= 0; --k)
+      if (tb[k][l] >= L) l -= 1 << k;
+    return l;
+  };
+  if (from < 0) return -1;
+  if (tb[0][from] < L) return from;
+  if (pw[from] < L) return descend(from);
+  for (int v = from / WAVE - 1; v >= 0; --v)
+    if (wmin[v] < L) return descend(v * WAVE + WAVE - 1);
+  return -1;
 }
 
 // Block-wide exclusive scans of four int32 values.
@@ -422,23 +425,6 @@ __device__ __forceinline__ void block_scan4(int32_t* v, int32_t* tot, int32_t (*
   }
 }
 
-// Exclusive minimum over the later lanes of the workgroup.
-__device__ __forceinline__ int32_t suffix_min_after(int32_t v, int32_t* wtot) {
-  const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  int32_t inc = v;
-#pragma unroll
-  for (int d = 1; d < WAVE; d <<= 1) {
-    const int32_t o = __shfl_down(inc, d, WAVE);
-    if (lane + d < (uint32_t)WAVE) inc = o < inc ? o : inc;
-  }
-  if (lane == 0) wtot[wid] = inc;
-  __syncthreads();
-  int32_t ex = __shfl_down(inc, 1, WAVE);
-  if (lane == WAVE - 1) ex = 0x7FFFFFFF;
-  for (uint32_t w = wid + 1; w < (uint32_t)RNW; ++w) ex = wtot[w] < ex ? wtot[w] : ex;
-  return ex;
-}
-
 // MBPOP..MSPUSH bound the window effects of the grammar (bracket pops and
 // pushes, productions, surviving child slots); the host picks the smallest
 // instantiation that covers them.
@@ -457,13 +443,13 @@ __global__ __launch_bounds__(RWG) void seg_resolve(Resolve r) {
   // staged parents of nodes u16 [ncap] and of terminal productions u16
   // [tcap] (tile-local compact indices), staged productions u8 [ncap]
   extern __shared__ __align__(16) uint16_t dyn[];
-  __shared__ int16_t tbb[RLV][RWG], tbs[RLV][RWG];
+  __shared__ int16_t tbb[WLV][RWG], tbs[WLV][RWG], pwb[RWG], pws[RWG];
   __shared__ uint32_t pkb[RWG], pks[RWG];  // (lane minimum << 16) | previous smaller lane
   __shared__ uint16_t l_cb[RWG], l_tb[RWG];  // each lane's first node / terminal production (tile-local)
   __shared__ int32_t ws[4][RNW];
   __shared__ int32_t wmin[2][RNW];
   const Seg& s = r.s;
-  const uint32_t tile = blockIdx.x, tid = threadIdx.x;
+  const uint32_t tile = blockIdx.x, tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
   const int bcap = r.bcap, scap = r.scap;
   uint16_t* const bst = dyn + tid;
   uint8_t* const sst = reinterpret_cast<uint8_t*>(dyn + bcap * RWG) + tid;
@@ -584,33 +570,54 @@ __global__ __launch_bounds__(RWG) void seg_resolve(Resolve r) {
     tinc += ntk;
   }
 
-  // phase 2: lane minima, suffix minima, sparse tables
+  // phase 2: lane minima; in-wave sparse tables and prefix / suffix minima
+  // by shuffles, wave minima in LDS; then every lane's previous strictly
+  // smaller lane minimum (-1: none) -- lanes between a lane and that one
+  // have minima >= its own
   const int32_t lbm = bS + bmin, lsm = sS + smin;
+  int32_t pb = lbm, ps = lsm, sb = lbm, ss = lsm;
   tbb[0][tid] = (int16_t)lbm;
   tbs[0][tid] = (int16_t)lsm;
-  const int32_t bsuf = suffix_min_after(lbm, wmin[0]);
-  const int32_t ssuf = suffix_min_after(lsm, wmin[1]);
-  // level 0, the stacks and l_cb / l_tb are visible: suffix_min_after synchronised
 #pragma unroll
-  for (int k = 1; k < RLV; ++k) {
+  for (int k = 1; k <= WLV; ++k) {  // the last step makes pb / ps the in-wave prefix minima
     const int hh = 1 << (k - 1);
-    int16_t a = tbb[k - 1][tid], c = tbs[k - 1][tid];
-    if ((int)tid >= hh) {
-      const int16_t a2 = tbb[k - 1][tid - hh], c2 = tbs[k - 1][tid - hh];
-      a = a2 < a ? a2 : a;
-      c = c2 < c ? c2 : c;
+    const int32_t a2 = __shfl_up(pb, hh, WAVE), c2 = __shfl_up(ps, hh, WAVE);
+    if ((int)lane >= hh) {
+      pb = a2 < pb ? a2 : pb;
+      ps = c2 < ps ? c2 : ps;
     }
-    tbb[k][tid] = a;
-    tbs[k][tid] = c;
-    __syncthreads();
+    if (k < WLV) {
+      tbb[k][tid] = (int16_t)pb;
+      tbs[k][tid] = (int16_t)ps;
+    }
   }
-  // every lane's previous strictly smaller lane minimum (-1: none): lanes
-  // between a lane and that one have minima >= its own
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) {  // in-wave inclusive suffix minima
+    const int32_t a2 = __shfl_down(sb, d, WAVE), c2 = __shfl_down(ss, d, WAVE);
+    if (lane + d < (uint32_t)WAVE) {
+      sb = a2 < sb ? a2 : sb;
+      ss = c2 < ss ? c2 : ss;
+    }
+  }
+  pwb[tid] = (int16_t)pb;
+  pws[tid] = (int16_t)ps;
+  if (lane == WAVE - 1) {
+    wmin[0][wid] = pb;
+    wmin[1][wid] = ps;
+  }
+  // the stacks, l_cb / l_tb, the tables and the wave minima are visible after this
+  __syncthreads();
+  int32_t bsuf = __shfl_down(sb, 1, WAVE), ssuf = __shfl_down(ss, 1, WAVE);
+  if (lane == WAVE - 1) bsuf = ssuf = 0x7FFFFFFF;
+  for (uint32_t v = wid + 1; v < (uint32_t)RNW; ++v) {
+    bsuf = wmin[0][v] < bsuf ? wmin[0][v] : bsuf;
+    ssuf = wmin[1][v] < ssuf ? wmin[1][v] : ssuf;
+  }
   {
-    int32_t md;
-    const int pb = prev_below(tbb, (int)tid - 1, lbm, &md), ps = prev_below(tbs, (int)tid - 1, lsm, &md);
-    pkb[tid] = (uint32_t)(uint16_t)(int16_t)lbm << 16 | (uint16_t)(int16_t)(pb < 0 ? -1 : pb);
-    pks[tid] = (uint32_t)(uint16_t)(int16_t)lsm << 16 | (uint16_t)(int16_t)(ps < 0 ? -1 : ps);
+    const int pbl = prev_below(tbb, pwb, wmin[0], (int)tid - 1, lbm);
+    const int psl = prev_below(tbs, pws, wmin[1], (int)tid - 1, lsm);
+    pkb[tid] = (uint32_t)(uint16_t)(int16_t)lbm << 16 | (uint16_t)(int16_t)pbl;
+    pks[tid] = (uint32_t)(uint16_t)(int16_t)lsm << 16 | (uint16_t)(int16_t)psl;
   }
   __syncthreads();
 

@@ -177,9 +177,35 @@ def run(g, d, strings, tree):
                 if l >= 0 and tb[kk][l] >= Lv: l -= 1 << kk
             return l
         def sufmin(a, l): return min(a[l + 1:]) if l + 1 < RWG else 0x7FFFFFFF
-        # previous strictly smaller lane minimum (-1: none), from the sparse table
-        psb = [max(-1, prev_below(tbb, x - 1, mbl[x])) for x in range(RWG)]
-        pss = [max(-1, prev_below(tbs, x - 1, msl[x])) for x in range(RWG)]
+        # previous strictly smaller lane minimum (-1: none), as the kernel finds
+        # it: in-wave sparse tables (tile = 4 waves of WM lanes here), in-wave
+        # prefix minima and wave minima
+        WM = max(1, RWG // 4); WL = (WM - 1).bit_length(); assert RWG % WM == 0
+        def wtab(a):
+            tb = [list(a)]
+            for kk in range(1, WL):
+                h = 1 << (kk - 1)
+                tb.append([min(tb[kk - 1][x], tb[kk - 1][x - h]) if x % WM >= h else tb[kk - 1][x] for x in range(RWG)])
+            pw = [min(a[x - x % WM:x + 1]) for x in range(RWG)]
+            wm = [min(a[v * WM:(v + 1) * WM]) for v in range(RWG // WM)]
+            return tb, pw, wm
+        def wprev(T, fr, Lv):
+            tb, pw, wm = T
+            def descend(l):
+                for kk in range(WL - 1, -1, -1):
+                    assert 0 <= l < RWG
+                    if tb[kk][l] >= Lv: l -= 1 << kk
+                assert 0 <= l and tb[0][l] < Lv
+                return l
+            if fr < 0: return -1
+            if tb[0][fr] < Lv: return fr
+            if pw[fr] < Lv: return descend(fr)
+            for v in range(fr // WM - 1, -1, -1):
+                if wm[v] < Lv: return descend(v * WM + WM - 1)
+            return -1
+        wtb, wts = wtab(mbl), wtab(msl)
+        psb = [wprev(wtb, x - 1, mbl[x]) for x in range(RWG)]
+        pss = [wprev(wts, x - 1, msl[x]) for x in range(RWG)]
         for x in range(RWG):
             for ps_, m_ in ((psb, mbl), (pss, msl)):
                 assert ps_[x] < x and (ps_[x] < 0 or m_[ps_[x]] < m_[x])
@@ -330,7 +356,7 @@ def _cases(gname, g, rng):
     return strs, [x for x in lexed if x[0] is not None]
 
 
-@pytest.mark.parametrize("pt,apl", [(8, 2), (16, 4), (12, 1)])
+@pytest.mark.parametrize("pt,apl", [(8, 2), (16, 4), (12, 1), (64, 2)])
 @pytest.mark.parametrize("gname", GRAMMARS)
 def test_tile_model_matches_oracle(gname, pt, apl):
     configure(pt, apl)
