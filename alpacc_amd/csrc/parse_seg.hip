@@ -108,6 +108,44 @@ __device__ __forceinline__ uint32_t pair_at(const Seg& s, const TileOf& t, int64
   return s.window[code];
 }
 
+constexpr int MAXW = 8;  // widest window (q + k) the register path of pass A covers
+
+// Pair ids of framed positions p0 .. p0 + n - 1 (NO_PAIR past the input's
+// end): the framed tokens all of them need are loaded first, one
+// independent load each, then every window code is formed in registers.
+// W bounds the window width (q + k <= W).
+template <int W, int N>
+__device__ __forceinline__ void window_ids(const Seg& s, const TileOf& t, int32_t p0, uint32_t* ids) {
+  const int w = s.q + s.k;
+  const int32_t m = (int32_t)t.m;
+  uint32_t v[N + W - 1];
+#pragma unroll
+  for (int i = 0; i < N + W - 1; ++i) {
+    const int32_t f = p0 - s.q + i;
+    uint32_t x = (uint32_t)s.empty_t;
+    if (f == 0)
+      x = (uint32_t)s.start_t;
+    else if (f == m - 1)
+      x = (uint32_t)s.end_t;
+    else if (i < N + w - 1 && f > 0 && f < m - 1)
+      x = t.tk[f - 1];
+    v[i] = x;
+  }
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+    uint32_t code = 0;
+    bool ok = p0 + r < m;
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      if (j < w) {
+        ok &= v[r + j] < (uint32_t)s.base;
+        code = code * (uint32_t)s.base + v[r + j];
+      }
+    }
+    ids[r] = ok ? (uint32_t)s.window[code] : NO_PAIR;
+  }
+}
+
 // (sum, min-prefix) monoid: a then b.
 struct SM {
   int32_t s, m;
@@ -178,12 +216,17 @@ __global__ __launch_bounds__(AWG) void seg_profile(Seg s, TileRec* rec, uint16_t
   SM b{0, 0}, sl{0, 0};
   bool bad = false;
   uint32_t ids[APL];
+  if (s.q + s.k <= MAXW) {
+    window_ids<MAXW, APL>(s, t, (int32_t)p0, ids);
+  } else {
+#pragma unroll
+    for (int r = 0; r < APL; ++r) ids[r] = p0 + r < t.m ? pair_at(s, t, p0 + r) : NO_PAIR;
+  }
 #pragma unroll
   for (int r = 0; r < APL; ++r) {
     const int64_t p = p0 + r;
-    uint32_t id = NO_PAIR;
+    const uint32_t id = ids[r];
     if (p < t.m) {
-      id = pair_at(s, t, p);
       if (id == NO_PAIR) {
         bad = true;
       } else {
