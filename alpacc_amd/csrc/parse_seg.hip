@@ -684,11 +684,12 @@ __global__ __launch_bounds__(RWG) void seg_resolve(Resolve r) {
     else
       r.ub_list[ti.ub_off + (-D)] = sym;
   }
+  // entry e sits at level lbm + 1 + e and stays open iff no later lane
+  // falls below it: the first min(bsp, bsuf - lbm) entries
+  const int64_t broom = (int64_t)bsuf - lbm;
+  const int nob = broom < bsp ? (int)broom : bsp;
 #pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
-  for (int e = 0; e < bsp; ++e) {
-    const int32_t lv = lbm + 1 + e;
-    if (lv <= bsuf) r.ob_list[ti.ob_off + (lv - ti.bmin - 1)] = bst[e * RWG];
-  }
+  for (int e = 0; e < nob; ++e) r.ob_list[ti.ob_off + (lbm + e - ti.bmin)] = bst[e * RWG];
   if (r.tree) {
     const uint8_t* const sall = reinterpret_cast<const uint8_t*>(dyn + bcap * RWG);
     l = (int)tid - 1;
@@ -713,11 +714,10 @@ __global__ __launch_bounds__(RWG) void seg_resolve(Resolve r) {
         r.us_list[ti.us_off + (-D)] = term ? (TOKEN_TARGET | (ti.t_off + idx)) : (c_off + idx);
       }
     }
+    const int64_t sroom = (int64_t)ssuf - lsm;
+    const int nos = sroom < ssp ? (int)sroom : ssp;
 #pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
-    for (int e = 0; e < ssp; ++e) {
-      const int32_t lv = lsm + 1 + e;
-      if (lv <= ssuf) r.os_list[ti.os_off + (lv - ti.smin - 1)] = c_off + cb0 + sst[e * RWG];
-    }
+    for (int e = 0; e < nos; ++e) r.os_list[ti.os_off + (lsm + e - ti.smin)] = c_off + cb0 + sst[e * RWG];
   }
   if (bad) atomicOr(s.flags + seg, BAD_MATCH);
   if (tid == 0 && ti.bstart + ti.bmin < in.bstart) atomicOr(s.flags + seg, BAD_BDEPTH);
@@ -792,15 +792,17 @@ struct Far {
   int tree;
 };
 
+// One wave per tile: a tile has only a few pops left over.
 __global__ __launch_bounds__(256) void far_resolve(Far f) {
-  const uint32_t tile = blockIdx.x;
+  const uint32_t tile = blockIdx.x * (256 / WAVE) + threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
+  if (tile >= f.s.ntiles) return;
   const TileInfo ti = f.info[tile];
   const uint32_t seg = f.s.tile_seg[tile];
   const int64_t first = f.s.tile_first[seg];
   const SegBase in = f.sb[seg];
   const int64_t c_in = in.p_off - in.t_off;
   bool bad = false, bad_root = false;
-  for (int u = threadIdx.x; u < -ti.bmin; u += 256) {
+  for (int u = lane; u < -ti.bmin; u += WAVE) {
     const int32_t D = (int32_t)ti.bstart - u;  // depth before the pop
     const uint16_t b = f.ub_list[ti.ub_off + u];
     const int64_t t2 = tile_prev_less(f.tb, tile, D);
@@ -812,7 +814,7 @@ __global__ __launch_bounds__(256) void far_resolve(Far f) {
     const uint16_t push = f.ob_list[o.ob_off + (D - (int32_t)o.bstart - o.bmin - 1)];
     bad |= push != b;
   }
-  for (int u = threadIdx.x; f.tree && u < -ti.smin; u += 256) {
+  for (int u = lane; f.tree && u < -ti.smin; u += WAVE) {
     const int32_t D = (int32_t)ti.sstart - u;  // open child slots before the pop
     const int64_t target = f.us_list[ti.us_off + u];
     const int64_t t2 = tile_prev_less(f.ts, tile, D);
@@ -1063,7 +1065,7 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
     Far fr{s, info, sb, tb, ts, ob_list, ub_list, os_list, us_list, c.tr_parent.as<int64_t>(),
            c.tr_tokpar.as<int64_t>(), tree ? 1 : 0};
     HIP_TRY(tm.begin(st, "far_resolve"));
-    hipLaunchKernelGGL(far_resolve, dim3((uint32_t)ntiles), dim3(256), 0, st, fr);
+    hipLaunchKernelGGL(far_resolve, dim3((uint32_t)((ntiles + 3) / 4)), dim3(256), 0, st, fr);
     HIP_TRY(tm.end(st));
   }
   hipLaunchKernelGGL(seg_finish, dim3((uint32_t)((count + 255) / 256)), dim3(256), 0, st, s, info, sb, sout,

@@ -106,7 +106,7 @@ def read_pmc(csv_dir: str, counter: str):
                 if row.get("Counter_Name") != counter:
                     continue
                 name = row["Kernel_Name"].replace("(anonymous namespace)::", "")
-                short = name.split("(")[0].split("::")[-1]
+                short = name.split("(")[0].split("::")[-1].split("<")[0]  # template arguments dropped
                 acc.setdefault(short, []).append(float(row["Counter_Value"]))
     return {k: (sum(v) / len(v), max(v)) for k, v in acc.items()}
 
