@@ -153,13 +153,12 @@ struct SM {
 __device__ __forceinline__ SM sm_cat(SM a, SM b) { return {a.s + b.s, a.m < a.s + b.m ? a.m : a.s + b.m}; }
 
 __device__ __forceinline__ SM wave_scan_sm(SM v) {
-  const uint32_t lane = lane_id();
-#pragma unroll
-  for (int d = 1; d < WAVE; d <<= 1) {
-    SM o{__shfl_up(v.s, d, WAVE), __shfl_up(v.m, d, WAVE)};
-    if (lane >= (uint32_t)d) v = sm_cat(o, v);
-  }
-  return v;
+  // identity: sum 0, minimum above any prefix depth of a tile
+  const U32x2 r = dpp_scan(U32x2{(uint32_t)v.s, (uint32_t)v.m}, U32x2{0u, 0x3FFFFFFFu}, Mov2{}, [](U32x2 a, U32x2 b) {
+    const SM c = sm_cat(SM{(int32_t)a.a, (int32_t)a.b}, SM{(int32_t)b.a, (int32_t)b.b});
+    return U32x2{(uint32_t)c.s, (uint32_t)c.m};
+  });
+  return SM{(int32_t)r.a, (int32_t)r.b};
 }
 
 struct TileRec {
@@ -634,13 +633,11 @@ __global__ __launch_bounds__(RWG) void seg_resolve(Resolve r) {
       tbs[k][tid] = (int16_t)ps;
     }
   }
-#pragma unroll
-  for (int d = 1; d < WAVE; d <<= 1) {  // in-wave inclusive suffix minima
-    const int32_t a2 = __shfl_down(sb, d, WAVE), c2 = __shfl_down(ss, d, WAVE);
-    if (lane + d < (uint32_t)WAVE) {
-      sb = a2 < sb ? a2 : sb;
-      ss = c2 < ss ? c2 : ss;
-    }
+  {  // in-wave inclusive suffix minima: a DPP prefix scan over the lanes in reverse
+    const int rl = WAVE - 1 - (int)lane;
+    const int32_t rb = wave_scan_min32(__shfl(sb, rl, WAVE)), rs = wave_scan_min32(__shfl(ss, rl, WAVE));
+    sb = __shfl(rb, rl, WAVE);
+    ss = __shfl(rs, rl, WAVE);
   }
   pwb[tid] = (int16_t)pb;
   pws[tid] = (int16_t)ps;
