@@ -249,7 +249,7 @@ struct EmitWords {
 };
 
 // LDS layout of lex_emit: DFA rows, the staged slots, the lanes' kept-token
-// terminals (rcap rows of WG bytes and one sink byte), the workgroup words.
+// terminals (rcap rows of WG bytes and four sink rows), the workgroup words.
 struct EmitLayout {
   uint32_t start, end, term, lterm, words, total;
   __host__ __device__ EmitLayout(uint32_t nstate, uint32_t rcap) {
@@ -257,7 +257,7 @@ struct EmitLayout {
     end = start + 4 * STAGE;
     term = end + 2 * STAGE;
     lterm = term + STAGE;
-    words = (lterm + rcap * WG + 1 + 15u) & ~15u;
+    words = (lterm + (rcap + 4) * WG + 15u) & ~15u;
     total = words + (uint32_t)sizeof(EmitWords);
   }
 };
@@ -280,8 +280,9 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
     s_rows[i] = b == 256 ? (uint16_t)p.token[s] : RC::encode(p.delta[s * 256 + b], lds0);
   }
   const int ignore = p.ignore;
-  // s_lterm[r][lane] is the byte at LDS address lterm0 + r * WG + lane
-  const uint32_t lterm0 = lds0 + L.lterm, sink = lterm0 + rcap * WG;
+  // s_lterm[r][lane] is the byte at LDS address lterm0 + r * WG + lane; a
+  // lane's writes past rcap tokens go to its own byte of the sink rows
+  const uint32_t lterm0 = lds0 + L.lterm, sink = lterm0 + rcap * WG + tid;
 
   for (;;) {
     if (tid == 0) w.tile = atomicAdd(p.ticket, 1u);
@@ -296,18 +297,19 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
 
     // walk 1: the lane's boundaries (bm) and kept-token closes (km) as bit
     // masks, the terminal of its r-th kept token in s_lterm[r][lane] (every
-    // byte writes the slot of the next kept token; writes past rcap tokens
-    // land on one sink byte).  Lanes with all IPT bytes walk the registers
-    // fully unrolled; a lane holding an input's last bytes reads them one by
-    // one in 32-byte groups.
+    // byte writes the slot of the next kept token).  Lanes with all IPT
+    // bytes walk the registers fully unrolled, clamping their write address
+    // to the sink once per 4 bytes (so at most 3 rows past it); a lane
+    // holding an input's last bytes reads them one by one in 32-byte groups.
     uint32_t row = row0, waddr = lterm0 + tid, bm[IPT / 32] = {0, 0, 0, 0}, km[IPT / 32] = {0, 0, 0, 0};
-    auto step1 = [&](uint32_t byte, uint32_t& mb, uint32_t& mk, uint32_t bit) {
-      const uint32_t x = lds_load16(row + 2 * byte);
+    // off2 = 2 * byte: the transition's offset in the row
+    auto step1 = [&](uint32_t off2, uint32_t& mb, uint32_t& mk, uint32_t bit) {
+      const uint32_t x = lds_load16(row + off2);
       const uint32_t t = lds_load16(row + 512);
       const uint32_t kp = (x >> RC::KEPT) & 1u;
       mb |= ((x >> RC::PROD) & 1u) << bit;
       mk |= kp << bit;
-      lds_store8(waddr < sink ? waddr : sink, t);
+      lds_store8(waddr, t);
       waddr += kp << 8;
       asm volatile("" : "+v"(waddr));  // a running address, not rebuilt from a count
       row = RC::next_row(x, lds0);
@@ -318,13 +320,22 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
 #pragma unroll
       for (int k = 0; k < IPT / 4; ++k) {
         const uint32_t wk = lane_word(a, k, ts.shift, row);
-#pragma unroll
-        for (int b = 0; b < 4; ++b) step1((wk >> (8 * b)) & 0xFFu, bm[k >> 3], km[k >> 3], (4 * k + b) & 31);
+        // doubled bytes 0 and 2, 1 and 3 in the halves of two words
+        uint32_t e02 = (wk << 1) & 0x01FE01FEu, e13 = (wk >> 7) & 0x01FE01FEu;
+        asm volatile("" : "+v"(e02), "+v"(e13));  // kept paired, not re-extracted per byte
+        waddr = waddr < sink ? waddr : sink;
+        step1(e02 & 0xFFFFu, bm[k >> 3], km[k >> 3], (4 * k) & 31);
+        step1(e13 & 0xFFFFu, bm[k >> 3], km[k >> 3], (4 * k + 1) & 31);
+        step1(e02 >> 16, bm[k >> 3], km[k >> 3], (4 * k + 2) & 31);
+        step1(e13 >> 16, bm[k >> 3], km[k >> 3], (4 * k + 3) & 31);
       }
     } else {
 #pragma unroll
       for (int q = 0; q < IPT / 32; ++q)
-        for (uint32_t j = 32 * q; j < valid && j < 32u * q + 32; ++j) step1(p.in[base + j], bm[q], km[q], j & 31);
+        for (uint32_t j = 32 * q; j < valid && j < 32u * q + 32; ++j) {
+          waddr = waddr < sink ? waddr : sink;
+          step1(2u * p.in[base + j], bm[q], km[q], j & 31);
+        }
     }
     uint32_t cnt = 0;  // kept tokens closed inside the lane
 #pragma unroll
