@@ -205,7 +205,7 @@ __global__ __launch_bounds__(1024) void lex_tile_scan(LexParams p) {
 
 // ------------------------------------------------------------------- L3
 
-constexpr uint32_t STAGE = 2048;  // kept tokens staged in LDS per round
+constexpr uint32_t STAGE = 2176;  // kept tokens staged in LDS per round (JSON tiles: two rounds)
 constexpr uint32_t ROW = 257;     // LDS row of a DFA state: 256 transitions, then its terminal
 constexpr uint32_t ROWB = ROW * 2;
 
@@ -382,8 +382,10 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
     const uint32_t ecnt = wc + (lane ? pc : 0u);
     const uint32_t elast = lane ? (wl > pl ? wl : pl) : wl;
 
-    // decoupled lookback by wave 0: 64 predecessor tiles per probe
-    if (wid == 0) {
+    // decoupled lookback by the last wave (its lanes' tokens mostly fall in
+    // a later round), 64 predecessor tiles per probe, while the other waves
+    // stage the first round; its results are read after the round's barrier
+    if (wid == NW - 1) {
       const uint64_t agg = (uint64_t)tc | ((uint64_t)tl << 31);
       uint32_t xc = 0, xl = 0;
       if (tile == 0) {
@@ -426,23 +428,24 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
         w.pre_last = xl;
       }
     }
-    __syncthreads();
-    const uint64_t slot0 = w.pre_cnt;  // output slot of the tile's first kept token
-    uint32_t start0 = w.pre_last > elast ? w.pre_last : elast;
-    start0 = start0 > (uint32_t)ts.seg_lo ? start0 : (uint32_t)ts.seg_lo;  // an input's first token starts at its first byte
-    if (tid == 0 && ts.first) p.seg_tok[ts.seg] = slot0;
-    if (tid == 0 && tile == p.ntiles - 1) p.seg_tok[p.nseg] = slot0 + tc;
+    // where the lane's first token starts: the last boundary of an earlier
+    // lane, else (NO_START) the tile's carried start, known after the lookback
+    constexpr uint32_t NO_START = 0xFFFFFFFFu;
+    const uint32_t start_in = elast ? elast : NO_START;
+    uint64_t slot0 = 0;
+    uint32_t carried = 0;
     bool overflow = false;
     const uint32_t lane_off = tid * IPT;
 
     // per round of STAGE slots, the lanes owning slots of the round stage
     // (terminal, start, end) by slot; then the workgroup writes the round
     // out with consecutive lanes on consecutive slots
-    for (uint32_t lo = 0; lo < tc; lo += STAGE) {
+    uint32_t lo = 0;
+    do {
       const uint32_t hi = lo + STAGE;
       if (cnt > 0 && ecnt < hi && ecnt + cnt > lo) {
         const uint32_t d = ecnt - lo;  // staging index of the lane's first token (mod 2^32)
-        uint32_t k = 0, start = start0;
+        uint32_t k = 0, start = start_in;
         if (!over) {
           // kept token k closes before byte j (bit j of km) and starts at the
           // lane's last boundary before j, else where the lane's first token starts
@@ -493,11 +496,18 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
         }
       }
       __syncthreads();
+      if (lo == 0) {
+        slot0 = w.pre_cnt;  // output slot of the tile's first kept token
+        // the tile's carried start: the last boundary before it, never before its input's first byte
+        carried = w.pre_last > (uint32_t)ts.seg_lo ? w.pre_last : (uint32_t)ts.seg_lo;
+        if (tid == 0 && ts.first) p.seg_tok[ts.seg] = slot0;
+        if (tid == 0 && tile == p.ntiles - 1) p.seg_tok[p.nseg] = slot0 + tc;
+      }
       const uint32_t m = tc - lo < STAGE ? tc - lo : STAGE;
       const uint64_t max_len = p.max_len;
       for (uint32_t i = tid; i < m; i += WG) {
         const uint64_t slot = slot0 + lo + i;
-        const uint32_t b = s_start[i];
+        const uint32_t b0 = s_start[i], b = b0 == NO_START ? carried : b0;
         const uint64_t len = tbase + s_end[i] - b;
         overflow |= len > max_len;
         p.terminals[slot] = s_term[i];
@@ -505,7 +515,8 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
         store_length(p, slot, len);
       }
       __syncthreads();
-    }
+      lo += STAGE;
+    } while (lo < tc);
     if (overflow) atomicOr(p.seg_flags + ts.seg, LEX_REJECT_OVERFLOW);
   }
 }
