@@ -56,9 +56,12 @@ int lex_batch_on_device(Context& c, int mode, const uint8_t* d_in, const uint64_
 
   const int lb = g->length_bytes;
   const uint64_t cap = total + 1;
-  // scan state: count_status u64[ntiles] | tile_endo u32[ntiles] | tile_prefix u16[ntiles] | lane_prefix
+  // scan state: count_status u64[ntiles] | tile_endo u32[ntiles] | tile_prefix u16[ntiles] |
+  // tile_blk u32[2 * scan blocks] | lane_prefix
+  const size_t nblk = ((size_t)ntiles + 255) / 256;
   const size_t st_bytes = (size_t)ntiles * 8, te_off = st_bytes, tp_off = te_off + (size_t)ntiles * 4,
-               lp_off = (tp_off + (size_t)ntiles * 2 + 255) & ~size_t(255);
+               tb_off = (tp_off + (size_t)ntiles * 2 + 15) & ~size_t(15),
+               lp_off = (tb_off + nblk * 8 + 255) & ~size_t(255);
   if ((r = c.lx_term.reserve(cap)) || (r = c.lx_start.reserve(cap * 8)) || (r = c.lx_len.reserve(cap * lb)) ||
       (r = c.lx_status.reserve(lp_off + (size_t)ntiles * LEX_WG * 2 + 64)))
     return r;
@@ -93,6 +96,7 @@ int lex_batch_on_device(Context& c, int mode, const uint8_t* d_in, const uint64_
   p.count_status = reinterpret_cast<uint64_t*>(sb);
   p.tile_endo = reinterpret_cast<uint32_t*>(sb + te_off);
   p.tile_prefix = reinterpret_cast<uint16_t*>(sb + tp_off);
+  p.tile_blk = reinterpret_cast<uint32_t*>(sb + tb_off);
   p.lane_prefix = reinterpret_cast<uint16_t*>(sb + lp_off);
   p.ntiles = (uint32_t)ntiles;
   if (!c.lex_grid) HIP_TRY(lex_grid_size(p, &c.lex_grid, &c.lex_grid2));

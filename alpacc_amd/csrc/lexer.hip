@@ -15,8 +15,8 @@
 //                    (compose table in L2) gives each lane its exclusive
 //                    prefix inside the tile (u16, written out) and the tile
 //                    its aggregate;
-//   L2 lex_tile_scan one workgroup scans the tile aggregates, restarting at
-//                    the first tile of every input;
+//   L2 lex_tile_scan three short launches scan the tile aggregates,
+//                    restarting at the first tile of every input;
 //   L3 lex_emit      every lane starts from the DFA state
 //                    ENDO_INIT[tile prefix . lane prefix] and walks its bytes
 //                    through the LDS token DFA (rows of next-row address |
@@ -154,53 +154,96 @@ __global__ __launch_bounds__(WG) void lex_fold(LexParams p) {
 
 // ------------------------------------------------------------------- L2
 
-// Exclusive scan of the tile endomorphisms by one workgroup of 1024 lanes,
-// restarting at the first tile of every input: each lane folds a contiguous
-// run into (restarts inside, endomorphism since the last restart), a wave
-// scan and a sequential pass over the 16 wave totals join the runs with
-// (f1, a) . (f2, b) = (f1 | f2, f2 ? b : a then b), then each lane rewrites
-// its run.
-__global__ __launch_bounds__(1024) void lex_tile_scan(LexParams p) {
-  __shared__ uint32_t s_wave[16], s_wflag[16];
-  const uint32_t tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const uint16_t* __restrict__ compose = p.compose;
-  const uint32_t nendo = p.nendo;
-  auto comp = [&](uint32_t a, uint32_t b) -> uint32_t { return compose[a * nendo + b]; };
-  auto first = [&](uint32_t t) -> bool { return p.tile_first[p.tile_seg[t]] == t; };
-  const uint32_t nt = p.ntiles;
-  const uint32_t per = (nt + 1023) / 1024;
-  const uint32_t lo = tid * per < nt ? tid * per : nt, hi = lo + per < nt ? lo + per : nt;
-  uint32_t run = 0, rf = 0;
-  for (uint32_t t = lo; t < hi; ++t) {
-    if (first(t)) {
-      run = 0;
-      rf = 1;
-    }
-    run = comp(run, p.tile_endo[t]);
-  }
-  uint32_t inc = run, f = rf;
+// Exclusive scan of the tile endomorphisms, restarting at the first tile of
+// every input, with the segmented operator (f1, a) . (f2, b) = (f1 | f2,
+// f2 ? b : a then b), in three short launches (one tile per thread):
+//   S1 per block of 256 tiles: the tile's prefix inside its block, with a
+//      flag when a restart lies between the block start and the tile
+//      (written over tile_endo), and the block aggregate;
+//   S2 one workgroup: each block's carry (exclusive over the blocks);
+//   S3 each tile: prefix = flagged ? local : carry then local.
+constexpr int TSB = 256;  // tiles per scan block
+
+__device__ __forceinline__ void seg_comp_scan(const uint16_t* __restrict__ compose, uint32_t nendo, uint32_t& f,
+                                              uint32_t& e) {
+  const uint32_t lane = threadIdx.x & (WAVE - 1);
 #pragma unroll
   for (int d = 1; d < WAVE; d <<= 1) {
-    const uint32_t o = __shfl_up(inc, d, WAVE), of = __shfl_up(f, d, WAVE);
+    const uint32_t of = __shfl_up(f, d, WAVE), oe = __shfl_up(e, d, WAVE);
     if (lane >= (uint32_t)d) {
-      inc = f ? inc : comp(o, inc);
+      e = f ? e : compose[oe * nendo + e];
       f |= of;
     }
   }
-  if (lane == 63) {
-    s_wave[wid] = inc;
-    s_wflag[wid] = f;
+}
+
+// Exclusive segmented scan of (f, e) over a workgroup of TSB threads;
+// *tot receives the inclusive aggregate of the whole group.
+__device__ __forceinline__ void seg_comp_block(const uint16_t* __restrict__ compose, uint32_t nendo, uint32_t f,
+                                               uint32_t e, uint32_t& xf, uint32_t& xe, uint32_t& tf, uint32_t& te) {
+  __shared__ uint32_t s_f[TSB / WAVE], s_e[TSB / WAVE];
+  const uint32_t lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+  uint32_t fi = f, ei = e;
+  seg_comp_scan(compose, nendo, fi, ei);
+  if (lane == WAVE - 1) {
+    s_f[wid] = fi;
+    s_e[wid] = ei;
   }
   __syncthreads();
-  uint32_t wpre = 0;
-  for (uint32_t i = 0; i < wid; ++i) wpre = s_wflag[i] ? s_wave[i] : comp(wpre, s_wave[i]);
-  const uint32_t prev = __shfl_up(inc, 1, WAVE), pf = __shfl_up(f, 1, WAVE);
-  uint32_t acc = lane == 0 ? wpre : (pf ? prev : comp(wpre, prev));
-  for (uint32_t t = lo; t < hi; ++t) {
-    if (first(t)) acc = 0;
-    p.tile_prefix[t] = (uint16_t)acc;
-    acc = comp(acc, p.tile_endo[t]);
+  uint32_t wf = 0, we = 0;  // exclusive over the earlier waves
+  for (uint32_t w = 0; w < wid; ++w) {
+    we = s_f[w] ? s_e[w] : compose[we * nendo + s_e[w]];
+    wf |= s_f[w];
   }
+  uint32_t lf = __shfl_up(fi, 1, WAVE), le = __shfl_up(ei, 1, WAVE);
+  if (lane == 0) lf = le = 0;
+  xe = lf ? le : compose[we * nendo + le];
+  xf = wf | lf;
+  tf = 0;
+  te = 0;
+  for (uint32_t w = 0; w < TSB / WAVE; ++w) {
+    te = s_f[w] ? s_e[w] : compose[te * nendo + s_e[w]];
+    tf |= s_f[w];
+  }
+}
+
+__global__ __launch_bounds__(TSB) void lex_tile_scan1(LexParams p) {
+  const uint32_t t = blockIdx.x * TSB + threadIdx.x, nt = p.ntiles;
+  const bool ok = t < nt;
+  const uint32_t f = ok && p.tile_first[p.tile_seg[t]] == t ? 1u : 0u;
+  const uint32_t e = ok ? p.tile_endo[t] : 0u;
+  uint32_t xf, xe, tf, te;
+  seg_comp_block(p.compose, p.nendo, f, e, xf, xe, tf, te);
+  // the tile's prefix inside the block: identity at a restart
+  if (ok) p.tile_endo[t] = f ? 0x80000000u : ((xf ? 0x80000000u : 0u) | xe);
+  if (threadIdx.x == 0) p.tile_blk[blockIdx.x] = (tf ? 0x80000000u : 0u) | te;
+}
+
+__global__ __launch_bounds__(TSB) void lex_tile_scan2(LexParams p, uint32_t nblk) {
+  __shared__ uint32_t s_cf, s_ce;
+  if (threadIdx.x == 0) s_cf = s_ce = 0;
+  __syncthreads();
+  for (uint32_t b0 = 0; b0 < nblk; b0 += TSB) {
+    const uint32_t b = b0 + threadIdx.x;
+    const uint32_t w = b < nblk ? p.tile_blk[b] : 0u;
+    uint32_t xf, xe, tf, te;
+    seg_comp_block(p.compose, p.nendo, w >> 31, w & 0x7FFFFFFFu, xf, xe, tf, te);
+    const uint32_t cf = s_cf, ce = s_ce;
+    if (b < nblk) p.tile_blk[nblk + b] = xf ? xe : p.compose[ce * p.nendo + xe];  // carry into block b
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      s_ce = tf ? te : p.compose[ce * p.nendo + te];
+      s_cf = cf | tf;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(TSB) void lex_tile_scan3(LexParams p, uint32_t nblk) {
+  const uint32_t t = blockIdx.x * TSB + threadIdx.x;
+  if (t >= p.ntiles) return;
+  const uint32_t w = p.tile_endo[t], carry = p.tile_blk[nblk + t / TSB];
+  p.tile_prefix[t] = (uint16_t)((w >> 31) ? (w & 0x7FFFFFFFu) : p.compose[carry * p.nendo + w]);
 }
 
 // ------------------------------------------------------------------- L3
@@ -591,7 +634,10 @@ hipError_t lex_launch(const LexParams& p, int fold_grid, int emit_grid, hipStrea
   if ((e = tm.begin(st, "lex_fold")) != hipSuccess) return e;
   hipLaunchKernelGGL(lex_fold, dim3(g1), dim3(WG), lex_fold_lds(p), st, p);
   if ((e = tm.end(st)) != hipSuccess || (e = tm.begin(st, "lex_tile_scan")) != hipSuccess) return e;
-  hipLaunchKernelGGL(lex_tile_scan, dim3(1), dim3(1024), 0, st, p);
+  const uint32_t nblk = (nt + TSB - 1) / TSB;
+  hipLaunchKernelGGL(lex_tile_scan1, dim3(nblk), dim3(TSB), 0, st, p);
+  hipLaunchKernelGGL(lex_tile_scan2, dim3(1), dim3(TSB), 0, st, p, nblk);
+  hipLaunchKernelGGL(lex_tile_scan3, dim3(nblk), dim3(TSB), 0, st, p, nblk);
   if ((e = tm.end(st)) != hipSuccess || (e = tm.begin(st, "lex_emit")) != hipSuccess) return e;
   LexParams q = p;
   q.rcap = emit_rcap(p);

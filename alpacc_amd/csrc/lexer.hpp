@@ -47,6 +47,7 @@ struct LexParams {
   uint16_t* lane_prefix;   // [ntiles * LEX_WG] exclusive endomorphism of each lane inside its tile
   uint32_t* tile_endo;     // [ntiles] tile aggregates
   uint16_t* tile_prefix;   // [ntiles] exclusive tile prefixes inside the input
+  uint32_t* tile_blk;      // [2 * ceil(ntiles / 256)] scan blocks' aggregates, then their carries
   uint64_t* count_status;  // [ntiles] lookback words, zeroed before every launch
   uint32_t* ticket;        // [1] zeroed before every launch
   uint32_t ntiles;
