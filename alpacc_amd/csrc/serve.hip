@@ -73,7 +73,6 @@ struct alpacc_hip_server {
   std::unordered_map<uint64_t, std::unique_ptr<Req>> tickets;
   uint64_t next_ticket = 1;
   uint64_t n_batches = 0, n_reqs = 0, biggest = 0;
-  std::string start_error;
 };
 
 namespace {
