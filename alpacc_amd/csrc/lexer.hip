@@ -90,6 +90,16 @@ __device__ __forceinline__ void load_lane(const LexParams& p, uint64_t base, uin
   a[IPT / 4] = shift ? reinterpret_cast<const uint32_t*>(src)[IPT / 4] : 0u;
 }
 
+// The dwords holding the lane's first `valid` bytes (fewer than IPT: the
+// input ends inside the lane), zero beyond them: only dwords holding one of
+// the input's bytes are read.
+__device__ __forceinline__ void load_lane_part(const LexParams& p, uint64_t base, uint32_t shift, uint32_t valid,
+                                               uint32_t* a) {
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(p.in + base - shift);
+#pragma unroll
+  for (int d = 0; d <= IPT / 4; ++d) a[d] = (uint32_t)(4 * d) < valid + shift ? src[d] : 0u;
+}
+
 // Bytes 4k..4k+3 of the lane.  The empty asm makes the realignment wait
 // for `dep` (the walk's state), so the 32 realigned words are formed as the
 // walk reaches them instead of all at once next to the raw ones.
@@ -119,9 +129,9 @@ __global__ __launch_bounds__(WG) void lex_fold(LexParams p) {
     const uint64_t base = ts.tbase + (uint64_t)tid * IPT;
     const uint32_t valid = base >= ts.seg_hi ? 0u : (uint32_t)min((uint64_t)IPT, ts.seg_hi - base);
     uint32_t e = 0;  // identity
-    // lanes with all IPT bytes: fully unrolled (the byte registers are
-    // indexed with constants); the lane holding an input's last bytes reads
-    // them one by one
+    // the lane's bytes fully unrolled from registers (indexed with
+    // constants); in the lane holding an input's last bytes, the positions
+    // past them keep the state
     if (valid == IPT) {
       uint32_t a[IPT / 4 + 1];
       load_lane(p, base, ts.shift, a);
@@ -131,8 +141,18 @@ __global__ __launch_bounds__(WG) void lex_fold(LexParams p) {
 #pragma unroll
         for (int b = 0; b < 4; ++b) e = s_step[e * nclass + s_class[(wk >> (8 * b)) & 0xFFu]];
       }
-    } else {
-      for (uint32_t j = 0; j < valid; ++j) e = s_step[e * nclass + s_class[p.in[base + j]]];
+    } else if (valid > 0) {
+      uint32_t a[IPT / 4 + 1];
+      load_lane_part(p, base, ts.shift, valid, a);
+#pragma unroll
+      for (int k = 0; k < IPT / 4; ++k) {
+        const uint32_t wk = lane_word(a, k, ts.shift, e);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const uint32_t n = s_step[e * nclass + s_class[(wk >> (8 * b)) & 0xFFu]];
+          e = (uint32_t)(4 * k + b) < valid ? n : e;
+        }
+      }
     }
     uint32_t inc = e;
 #pragma unroll
@@ -249,7 +269,10 @@ __global__ __launch_bounds__(TSB) void lex_tile_scan3(LexParams p, uint32_t nblk
 // ------------------------------------------------------------------- L3
 
 constexpr uint32_t STAGE = 2176;  // kept tokens staged in LDS per round (JSON tiles: two rounds)
-constexpr uint32_t ROW = 257;     // LDS row of a DFA state: 256 transitions, then its terminal
+// LDS row of a DFA state: 256 transitions, its terminal, and a transition
+// to itself that produces nothing (the bytes past an input's end)
+constexpr uint32_t ROW = 258;
+constexpr uint32_t STAY2 = 2 * 257;  // offset of that transition in the row
 constexpr uint32_t ROWB = ROW * 2;
 
 // Transition entries of the LDS rows (u16).  SCALED (at most 31 states):
@@ -320,7 +343,7 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
   const uint32_t lds0 = lds_addr(lds);
   for (uint32_t i = tid; i < (uint32_t)p.nstate * ROW; i += WG) {
     const uint32_t s = i / ROW, b = i - s * ROW;
-    s_rows[i] = b == 256 ? (uint16_t)p.token[s] : RC::encode(p.delta[s * 256 + b], lds0);
+    s_rows[i] = b == 256 ? (uint16_t)p.token[s] : RC::encode(b == 257 ? s : p.delta[s * 256 + b], lds0);
   }
   const int ignore = p.ignore;
   // s_lterm[r][lane] is the byte at LDS address lterm0 + r * WG + lane; a
@@ -340,10 +363,11 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
 
     // walk 1: the lane's boundaries (bm) and kept-token closes (km) as bit
     // masks, the terminal of its r-th kept token in s_lterm[r][lane] (every
-    // byte writes the slot of the next kept token).  Lanes with all IPT
-    // bytes walk the registers fully unrolled, clamping their write address
-    // to the sink once per 4 bytes (so at most 3 rows past it); a lane
-    // holding an input's last bytes reads them one by one in 32-byte groups.
+    // byte writes the slot of the next kept token).  The lane's bytes are
+    // walked fully unrolled from registers, with the write address clamped
+    // to the sink once per 4 bytes (so at most 3 rows past it); in the lane
+    // holding an input's last bytes, the positions past them take the row's
+    // stay transition.
     uint32_t row = row0, waddr = lterm0 + tid, bm[IPT / 32] = {0, 0, 0, 0}, km[IPT / 32] = {0, 0, 0, 0};
     // off2 = 2 * byte: the transition's offset in the row
     auto step1 = [&](uint32_t off2, uint32_t& mb, uint32_t& mk, uint32_t bit) {
@@ -372,13 +396,19 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
         step1(e02 >> 16, bm[k >> 3], km[k >> 3], (4 * k + 2) & 31);
         step1(e13 >> 16, bm[k >> 3], km[k >> 3], (4 * k + 3) & 31);
       }
-    } else {
+    } else if (valid > 0) {
+      uint32_t a[IPT / 4 + 1];
+      load_lane_part(p, base, ts.shift, valid, a);
 #pragma unroll
-      for (int q = 0; q < IPT / 32; ++q)
-        for (uint32_t j = 32 * q; j < valid && j < 32u * q + 32; ++j) {
-          waddr = waddr < sink ? waddr : sink;
-          step1(2u * p.in[base + j], bm[q], km[q], j & 31);
+      for (int k = 0; k < IPT / 4; ++k) {
+        const uint32_t wk = lane_word(a, k, ts.shift, row);
+        waddr = waddr < sink ? waddr : sink;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const uint32_t j = 4 * k + b;
+          step1(j < valid ? ((wk >> (8 * b)) & 0xFFu) * 2u : STAY2, bm[k >> 3], km[k >> 3], j & 31);
         }
+      }
     }
     uint32_t cnt = 0;  // kept tokens closed inside the lane
 #pragma unroll
