@@ -29,30 +29,33 @@ int lex_batch_on_device(Context& c, int mode, const uint8_t* d_in, const uint64_
   if (h_off[count] > LEX_MAX_INPUT)
     return fail(ALPACC_ERR_LIMIT, "inputs of %llu bytes exceed the %llu-byte launch limit",
                 (unsigned long long)h_off[count], (unsigned long long)LEX_MAX_INPUT);
-  // every input owns whole tiles, at least one
-  uint64_t ntiles = 0;
+  // lane prefixes keep a flag in bit 15 of their endomorphism ids
+  if (g->nendo > 0x8000) return fail(ALPACC_ERR_LIMIT, "%d lexer endomorphisms exceed 32768", g->nendo);
+  // every input owns whole chunks of LEX_IPT bytes, at least one
+  uint64_t nchunks = 0;
   for (uint64_t i = 0; i < count; ++i) {
     if (h_off[i + 1] < h_off[i]) return fail(ALPACC_ERR_ARG, "offsets decrease at input %llu", (unsigned long long)i);
     const uint64_t n = h_off[i + 1] - h_off[i];
-    ntiles += n ? (n + LEX_TILE - 1) / LEX_TILE : 1;
+    nchunks += n ? (n + LEX_IPT - 1) / LEX_IPT : 1;
   }
-  if (ntiles > 0x7FFFFFFFull) return fail(ALPACC_ERR_LIMIT, "too many lexer tiles");
+  if (nchunks > 0x7FFFFFFFull) return fail(ALPACC_ERR_LIMIT, "too many lexer chunks");
+  const uint64_t ntiles = (nchunks + LEX_WG - 1) / LEX_WG;
   const BatchLayout L(count, ntiles);
   int r;
   if ((r = c.lx_meta.reserve(L.total)) || (r = c.h_meta.reserve(L.total))) return r;
   uint8_t* hm = c.h_meta.as<uint8_t>();
   uint8_t* dm = c.lx_meta.as<uint8_t>();
   uint64_t* h_seg = reinterpret_cast<uint64_t*>(hm + L.off_seg);
-  uint32_t* h_tf = reinterpret_cast<uint32_t*>(hm + L.off_tf);
-  uint32_t t = 0;
+  uint32_t* h_cf = reinterpret_cast<uint32_t*>(hm + L.off_cf);
+  uint32_t k = 0;
   for (uint64_t i = 0; i < count; ++i) {
     h_seg[i] = h_off[i];
-    h_tf[i] = t;
+    h_cf[i] = k;
     const uint64_t n = h_off[i + 1] - h_off[i];
-    t += n ? (uint32_t)((n + LEX_TILE - 1) / LEX_TILE) : 1u;
+    k += n ? (uint32_t)((n + LEX_IPT - 1) / LEX_IPT) : 1u;
   }
   h_seg[count] = h_off[count];
-  h_tf[count] = t;
+  h_cf[count] = k;
 
   const int lb = g->length_bytes;
   const uint64_t cap = total + 1;
@@ -69,7 +72,7 @@ int lex_batch_on_device(Context& c, int mode, const uint8_t* d_in, const uint64_
   p.in = d_in;
   p.in_bytes = h_off[count];
   p.seg_off = reinterpret_cast<const uint64_t*>(dm + L.off_seg);
-  p.tile_first = reinterpret_cast<const uint32_t*>(dm + L.off_tf);
+  p.chunk_first = reinterpret_cast<const uint32_t*>(dm + L.off_cf);
   p.tile_seg = reinterpret_cast<uint32_t*>(dm + L.off_tseg);
   p.nseg = (uint32_t)count;
   p.byte_class = c.d_class;

@@ -7,16 +7,18 @@
 // the state after the last byte is accepting.
 //
 // A launch lexes a batch of inputs packed back to back (a single input is a
-// batch of one).  Every input owns whole tiles of 256 lanes x 128 bytes; a
-// tile's first byte need not be aligned (loads are dword-aligned and bytes
-// are realigned in registers).
+// batch of one).  Every input owns whole chunks of 128 bytes (one lane each,
+// its last chunk partial) and a tile is 256 consecutive chunks, so many
+// small inputs share a tile; a chunk's first byte need not be aligned (loads
+// are dword-aligned and bytes are realigned in registers).  The scans
+// restart at the first chunk of every input.
 //   L1 lex_fold      every lane folds its 128 bytes into one endomorphism
-//                    (LDS tables STEP[e][byte class]); a workgroup scan
-//                    (compose table in L2) gives each lane its exclusive
-//                    prefix inside the tile (u16, written out) and the tile
-//                    its aggregate;
+//                    (LDS tables STEP[e][byte class]); a segmented
+//                    workgroup scan (compose table in L2) gives each lane
+//                    its exclusive prefix inside the tile (u16, written out)
+//                    and the tile its aggregate;
 //   L2 lex_tile_scan three short launches scan the tile aggregates,
-//                    restarting at the first tile of every input;
+//                    restarting at the tiles where an input starts;
 //   L3 lex_emit      every lane starts from the DFA state
 //                    ENDO_INIT[tile prefix . lane prefix] and walks its bytes
 //                    through the LDS token DFA (rows of next-row address |
@@ -48,33 +50,84 @@ typedef uint32_t u32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
 constexpr int WG = LEX_WG;
 constexpr int IPT = LEX_IPT;
 constexpr int NW = WG / WAVE;
-constexpr uint32_t TILE = LEX_TILE;
 
-// A tile, its input and whether its bytes all belong to that input.
+// A tile: the inputs of its first and last chunks and the first of them.
 struct TileSpan {
   uint64_t tbase;   // offset of the tile's first byte in the packed inputs
-  uint64_t seg_lo;  // its input: [seg_lo, seg_hi)
+  uint64_t seg_lo;  // input s_lo: [seg_lo, seg_hi), its first chunk cf
   uint64_t seg_hi;
-  uint32_t seg;
-  uint32_t first;   // the input's first tile
-  uint32_t shift;   // byte misalignment of the tile start (the same for every lane)
+  uint32_t s_lo, s_hi, cf;
+  __device__ bool multi() const { return s_hi != s_lo; }
 };
 
 __device__ __forceinline__ TileSpan tile_span(const LexParams& p, uint32_t tile) {
   TileSpan t;
-  const uint32_t seg = p.tile_seg[tile];
-  const uint32_t tf = p.tile_first[seg];
-  t.seg = seg;
-  t.first = tf == tile;
-  t.seg_lo = p.seg_off[seg];
-  t.seg_hi = p.seg_off[seg + 1];
-  t.tbase = t.seg_lo + (uint64_t)(tile - tf) * TILE;
-  t.shift = (uint32_t)((uintptr_t)(p.in + t.tbase) & 3u);
+  t.s_lo = p.tile_seg[2 * tile];
+  t.s_hi = p.tile_seg[2 * tile + 1];
+  t.cf = p.chunk_first[t.s_lo];
+  t.seg_lo = p.seg_off[t.s_lo];
+  t.seg_hi = p.seg_off[t.s_lo + 1];
+  t.tbase = t.seg_lo + (uint64_t)(tile * WG - t.cf) * IPT;
   return t;
 }
 
+// Chunk offsets (from the tile's first chunk) of the inputs s_lo + d that
+// start inside a multi-input tile, d = 1 .. s_hi - s_lo, into s_ichunk[d]
+// (0xFFFF past them).  Needs a barrier before use.
+__device__ __forceinline__ void tile_inputs(const LexParams& p, const TileSpan& ts, uint32_t tile,
+                                            uint16_t* s_ichunk) {
+  const uint32_t d = threadIdx.x + 1, m = ts.s_hi - ts.s_lo;
+  if (d < (uint32_t)WG) s_ichunk[d] = d <= m ? (uint16_t)(p.chunk_first[ts.s_lo + d] - tile * WG) : (uint16_t)0xFFFFu;
+}
+
+// The same inputs' byte offsets from the tile's first byte.
+__device__ __forceinline__ void input_starts(const LexParams& p, const TileSpan& ts, uint16_t* s_istart) {
+  const uint32_t d = threadIdx.x + 1, m = ts.s_hi - ts.s_lo;
+  if (d < (uint32_t)WG) s_istart[d] = d <= m ? (uint16_t)(p.seg_off[ts.s_lo + d] - ts.tbase) : (uint16_t)0xFFFFu;
+}
+
+// The largest d in [0, 255] with every one of sorted[1..d] <= v (index 0 is
+// never read).
+__device__ __forceinline__ uint32_t count_le(const uint16_t* sorted, uint32_t v) {
+  uint32_t d = 0;
+#pragma unroll
+  for (uint32_t s = WG / 2; s >= 1; s >>= 1) d = sorted[d + s] <= v ? d + s : d;
+  return d;
+}
+
+// A lane's chunk: its input, its first byte and how many bytes it holds.
+struct LaneSpan {
+  uint64_t base, seg_lo, seg_hi;
+  uint32_t seg, valid, shift;
+  bool first;  // the input's first chunk
+};
+
+__device__ __forceinline__ LaneSpan lane_span(const LexParams& p, const TileSpan& ts, uint32_t tile,
+                                              const uint16_t* s_ichunk) {
+  LaneSpan l;
+  const uint32_t chunk = tile * WG + threadIdx.x;
+  uint32_t cf = ts.cf;
+  l.seg = ts.s_lo;
+  l.seg_lo = ts.seg_lo;
+  l.seg_hi = ts.seg_hi;
+  if (ts.multi()) {
+    const uint32_t d = count_le(s_ichunk, threadIdx.x);
+    if (d) {
+      l.seg = ts.s_lo + d;
+      cf = p.chunk_first[l.seg];
+      l.seg_lo = p.seg_off[l.seg];
+      l.seg_hi = p.seg_off[l.seg + 1];
+    }
+  }
+  l.first = chunk == cf;
+  l.base = l.seg_lo + (uint64_t)(chunk - cf) * IPT;
+  l.valid = l.base >= l.seg_hi ? 0u : (uint32_t)min((uint64_t)IPT, l.seg_hi - l.base);
+  l.shift = (uint32_t)((uintptr_t)(p.in + l.base) & 3u);
+  return l;
+}
+
 // The dwords holding the lane's IPT bytes (all in its input): a[0] holds its
-// first byte at byte `shift`, a[IPT / 4] is read only when the tile is
+// first byte at byte `shift`, a[IPT / 4] is read only when the lane is
 // misaligned (it holds the lane's last byte), so no byte outside the dwords
 // of the lane's own bytes is touched.
 __device__ __forceinline__ void load_lane(const LexParams& p, uint64_t base, uint32_t shift, uint32_t* a) {
@@ -109,81 +162,8 @@ __device__ __forceinline__ uint32_t lane_word(const uint32_t* a, int k, uint32_t
   return __builtin_amdgcn_alignbyte(hi, lo, shift);
 }
 
-// ------------------------------------------------------------------- L1
-
-__global__ __launch_bounds__(WG) void lex_fold(LexParams p) {
-  extern __shared__ __align__(16) uint8_t lds[];
-  uint8_t* s_class = lds;
-  uint16_t* s_step = reinterpret_cast<uint16_t*>(lds + 256);
-  __shared__ uint32_t s_wave[NW];
-  const uint32_t tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
-  for (int i = tid; i < 256; i += WG) s_class[i] = p.byte_class[i];
-  for (int i = tid; i < p.nendo * p.nclass; i += WG) s_step[i] = p.step[i];
-  __syncthreads();
-  const uint16_t* __restrict__ compose = p.compose;
-  const uint32_t nendo = p.nendo, nclass = p.nclass;
-  auto comp = [&](uint32_t a, uint32_t b) -> uint32_t { return compose[a * nendo + b]; };
-
-  for (uint32_t tile = blockIdx.x; tile < p.ntiles; tile += gridDim.x) {
-    const TileSpan ts = tile_span(p, tile);
-    const uint64_t base = ts.tbase + (uint64_t)tid * IPT;
-    const uint32_t valid = base >= ts.seg_hi ? 0u : (uint32_t)min((uint64_t)IPT, ts.seg_hi - base);
-    uint32_t e = 0;  // identity
-    // the lane's bytes fully unrolled from registers (indexed with
-    // constants); in the lane holding an input's last bytes, the positions
-    // past them keep the state
-    if (valid == IPT) {
-      uint32_t a[IPT / 4 + 1];
-      load_lane(p, base, ts.shift, a);
-#pragma unroll
-      for (int k = 0; k < IPT / 4; ++k) {
-        const uint32_t wk = lane_word(a, k, ts.shift, e);
-#pragma unroll
-        for (int b = 0; b < 4; ++b) e = s_step[e * nclass + s_class[(wk >> (8 * b)) & 0xFFu]];
-      }
-    } else if (valid > 0) {
-      uint32_t a[IPT / 4 + 1];
-      load_lane_part(p, base, ts.shift, valid, a);
-#pragma unroll
-      for (int k = 0; k < IPT / 4; ++k) {
-        const uint32_t wk = lane_word(a, k, ts.shift, e);
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          const uint32_t n = s_step[e * nclass + s_class[(wk >> (8 * b)) & 0xFFu]];
-          e = (uint32_t)(4 * k + b) < valid ? n : e;
-        }
-      }
-    }
-    uint32_t inc = e;
-#pragma unroll
-    for (int d = 1; d < WAVE; d <<= 1) {
-      uint32_t o = __shfl_up(inc, d, WAVE);
-      if (lane >= (uint32_t)d) inc = comp(o, inc);
-    }
-    if (lane == WAVE - 1) s_wave[wid] = inc;
-    __syncthreads();
-    uint32_t wpre = 0;
-    for (uint32_t i = 0; i < wid; ++i) wpre = comp(wpre, s_wave[i]);
-    uint32_t prev = __shfl_up(inc, 1, WAVE);
-    uint32_t excl = lane == 0 ? wpre : comp(wpre, prev);
-    p.lane_prefix[(uint64_t)tile * WG + tid] = (uint16_t)excl;
-    if (tid == WG - 1) p.tile_endo[tile] = comp(wpre, inc);
-    __syncthreads();
-  }
-}
-
-// ------------------------------------------------------------------- L2
-
-// Exclusive scan of the tile endomorphisms, restarting at the first tile of
-// every input, with the segmented operator (f1, a) . (f2, b) = (f1 | f2,
-// f2 ? b : a then b), in three short launches (one tile per thread):
-//   S1 per block of 256 tiles: the tile's prefix inside its block, with a
-//      flag when a restart lies between the block start and the tile
-//      (written over tile_endo), and the block aggregate;
-//   S2 one workgroup: each block's carry (exclusive over the blocks);
-//   S3 each tile: prefix = flagged ? local : carry then local.
-constexpr int TSB = 256;  // tiles per scan block
-
+// Inclusive segmented scan of (f, e) over a wave, with the operator
+// (f1, a) . (f2, b) = (f1 | f2, f2 ? b : a then b).
 __device__ __forceinline__ void seg_comp_scan(const uint16_t* __restrict__ compose, uint32_t nendo, uint32_t& f,
                                               uint32_t& e) {
   const uint32_t lane = threadIdx.x & (WAVE - 1);
@@ -196,6 +176,101 @@ __device__ __forceinline__ void seg_comp_scan(const uint16_t* __restrict__ compo
     }
   }
 }
+
+// ------------------------------------------------------------------- L1
+
+__global__ __launch_bounds__(WG) void lex_fold(LexParams p) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  uint8_t* s_class = lds;
+  uint16_t* s_step = reinterpret_cast<uint16_t*>(lds + 256);
+  __shared__ uint32_t s_wave[NW];
+  __shared__ uint16_t s_ichunk[WG];
+  const uint32_t tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
+  for (int i = tid; i < 256; i += WG) s_class[i] = p.byte_class[i];
+  for (int i = tid; i < p.nendo * p.nclass; i += WG) s_step[i] = p.step[i];
+  __syncthreads();
+  const uint16_t* __restrict__ compose = p.compose;
+  const uint32_t nendo = p.nendo, nclass = p.nclass;
+  auto comp = [&](uint32_t a, uint32_t b) -> uint32_t { return compose[a * nendo + b]; };
+
+  for (uint32_t tile = blockIdx.x; tile < p.ntiles; tile += gridDim.x) {
+    const TileSpan ts = tile_span(p, tile);
+    if (ts.multi()) {
+      tile_inputs(p, ts, tile, s_ichunk);
+      __syncthreads();
+    }
+    const LaneSpan ls = lane_span(p, ts, tile, s_ichunk);
+    const uint32_t valid = ls.valid;
+    uint32_t e = 0;  // identity
+    // the lane's bytes fully unrolled from registers (indexed with
+    // constants); in the lane holding an input's last bytes, the positions
+    // past them keep the state
+    if (valid == IPT) {
+      uint32_t a[IPT / 4 + 1];
+      load_lane(p, ls.base, ls.shift, a);
+#pragma unroll
+      for (int k = 0; k < IPT / 4; ++k) {
+        const uint32_t wk = lane_word(a, k, ls.shift, e);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) e = s_step[e * nclass + s_class[(wk >> (8 * b)) & 0xFFu]];
+      }
+    } else if (valid > 0) {
+      uint32_t a[IPT / 4 + 1];
+      load_lane_part(p, ls.base, ls.shift, valid, a);
+#pragma unroll
+      for (int k = 0; k < IPT / 4; ++k) {
+        const uint32_t wk = lane_word(a, k, ls.shift, e);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const uint32_t n = s_step[e * nclass + s_class[(wk >> (8 * b)) & 0xFFu]];
+          e = (uint32_t)(4 * k + b) < valid ? n : e;
+        }
+      }
+    }
+    // segmented scan of (restart, endomorphism), restarting at the first
+    // chunk of every input: (f1, a) . (f2, b) = (f1 | f2, f2 ? b : a then b);
+    // a tile where no input starts scans the endomorphisms alone
+    const bool restarts = ts.multi() || ts.cf == tile * WG;
+    uint32_t fi = ls.first, ei = e;
+    if (restarts) {
+      seg_comp_scan(compose, nendo, fi, ei);
+    } else {
+#pragma unroll
+      for (int d = 1; d < WAVE; d <<= 1) {
+        const uint32_t o = __shfl_up(ei, d, WAVE);
+        if (lane >= (uint32_t)d) ei = comp(o, ei);
+      }
+    }
+    if (lane == WAVE - 1) s_wave[wid] = fi << 31 | ei;
+    __syncthreads();
+    uint32_t wf = 0, we = 0;  // over the earlier waves
+    for (uint32_t i = 0; i < wid; ++i) {
+      const uint32_t w = s_wave[i], f = w >> 31, x = w & 0x7FFFFFFFu;
+      we = f ? x : comp(we, x);
+      wf |= f;
+    }
+    const uint32_t lf = restarts ? __shfl_up(fi, 1, WAVE) : 0u, le = __shfl_up(ei, 1, WAVE);
+    const uint32_t xf = lane ? wf | lf : wf, xe = lane ? (lf ? le : comp(we, le)) : we;
+    // the lane's prefix from the tile start, or (bit 15) from its input's
+    // start when that lies inside the tile
+    p.lane_prefix[(uint64_t)tile * WG + tid] = (uint16_t)(ls.first ? 0x8000u : (xf ? 0x8000u | xe : xe));
+    if (tid == WG - 1) p.tile_endo[tile] = (wf | fi) << 31 | (fi ? ei : comp(we, ei));
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------- L2
+
+// Exclusive scan of the tile aggregates (f, e) -- f set when an input starts
+// inside the tile, e the endomorphism from the last such start (else from
+// the tile start) to the tile end -- with the segmented operator, in three
+// short launches (one tile per thread):
+//   S1 per block of 256 tiles: the tile's prefix inside its block, with a
+//      flag when a restart lies between the block start and the tile
+//      (written over tile_endo), and the block aggregate;
+//   S2 one workgroup: each block's carry (exclusive over the blocks);
+//   S3 each tile: prefix = flagged ? local : carry then local.
+constexpr int TSB = 256;  // tiles per scan block
 
 // Exclusive segmented scan of (f, e) over a workgroup of TSB threads;
 // *tot receives the inclusive aggregate of the whole group.
@@ -230,12 +305,11 @@ __device__ __forceinline__ void seg_comp_block(const uint16_t* __restrict__ comp
 __global__ __launch_bounds__(TSB) void lex_tile_scan1(LexParams p) {
   const uint32_t t = blockIdx.x * TSB + threadIdx.x, nt = p.ntiles;
   const bool ok = t < nt;
-  const uint32_t f = ok && p.tile_first[p.tile_seg[t]] == t ? 1u : 0u;
-  const uint32_t e = ok ? p.tile_endo[t] : 0u;
+  const uint32_t a = ok ? p.tile_endo[t] : 0u;
   uint32_t xf, xe, tf, te;
-  seg_comp_block(p.compose, p.nendo, f, e, xf, xe, tf, te);
-  // the tile's prefix inside the block: identity at a restart
-  if (ok) p.tile_endo[t] = f ? 0x80000000u : ((xf ? 0x80000000u : 0u) | xe);
+  seg_comp_block(p.compose, p.nendo, a >> 31, a & 0x7FFFFFFFu, xf, xe, tf, te);
+  // the tile's prefix inside the block
+  if (ok) p.tile_endo[t] = (xf ? 0x80000000u : 0u) | xe;
   if (threadIdx.x == 0) p.tile_blk[blockIdx.x] = (tf ? 0x80000000u : 0u) | te;
 }
 
@@ -302,7 +376,8 @@ __device__ __forceinline__ void lds_store8(uint32_t a, uint32_t v) { *(lds_u8*)a
 
 // DFA state of the lane's first byte.
 __device__ __forceinline__ uint32_t lane_state(const LexParams& p, uint32_t tile, uint32_t tid) {
-  const uint32_t pre = p.compose[(uint32_t)p.tile_prefix[tile] * p.nendo + p.lane_prefix[(uint64_t)tile * WG + tid]];
+  const uint32_t lp = p.lane_prefix[(uint64_t)tile * WG + tid];
+  const uint32_t pre = (lp & 0x8000u) ? lp & 0x7FFFu : p.compose[(uint32_t)p.tile_prefix[tile] * p.nendo + lp];
   return p.endo_init[pre];
 }
 
@@ -338,6 +413,9 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
   uint32_t* s_start = reinterpret_cast<uint32_t*>(lds + L.start);   // [STAGE] token start
   uint16_t* s_end = reinterpret_cast<uint16_t*>(lds + L.end);       // [STAGE] end, tile-relative
   uint8_t* s_term = lds + L.term;                                   // [STAGE] terminal
+  uint16_t* s_ichunk = reinterpret_cast<uint16_t*>(s_term);  // [WG] before the staging
+  // [WG] after walk 1, in its sink rows
+  uint16_t* s_istart = reinterpret_cast<uint16_t*>(lds + L.lterm + rcap * WG);
   EmitWords& w = *reinterpret_cast<EmitWords*>(lds + L.words);
   const uint32_t tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
   const uint32_t lds0 = lds_addr(lds);
@@ -353,12 +431,16 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
   for (;;) {
     if (tid == 0) w.tile = atomicAdd(p.ticket, 1u);
     __syncthreads();
-    const uint32_t tile = w.tile;
+    const uint32_t tile = __builtin_amdgcn_readfirstlane(w.tile);
     if (tile >= p.ntiles) break;
     const TileSpan ts = tile_span(p, tile);
-    const uint64_t tbase = ts.tbase, n = ts.seg_hi;
-    const uint64_t base = tbase + (uint64_t)tid * IPT;
-    const uint32_t valid = base >= n ? 0u : (base + IPT <= n ? (uint32_t)IPT : (uint32_t)(n - base));
+    if (ts.multi()) {
+      tile_inputs(p, ts, tile, s_ichunk);
+      __syncthreads();
+    }
+    const LaneSpan ls = lane_span(p, ts, tile, s_ichunk);
+    const uint64_t tbase = ts.tbase, base = ls.base;
+    const uint32_t valid = ls.valid;
     const uint32_t row0 = lds0 + lane_state(p, tile, tid) * ROWB;
 
     // walk 1: the lane's boundaries (bm) and kept-token closes (km) as bit
@@ -383,10 +465,10 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
     };
     if (valid == IPT) {
       uint32_t a[IPT / 4 + 1];
-      load_lane(p, base, ts.shift, a);
+      load_lane(p, base, ls.shift, a);
 #pragma unroll
       for (int k = 0; k < IPT / 4; ++k) {
-        const uint32_t wk = lane_word(a, k, ts.shift, row);
+        const uint32_t wk = lane_word(a, k, ls.shift, row);
         // doubled bytes 0 and 2, 1 and 3 in the halves of two words
         uint32_t e02 = (wk << 1) & 0x01FE01FEu, e13 = (wk >> 7) & 0x01FE01FEu;
         asm volatile("" : "+v"(e02), "+v"(e13));  // kept paired, not re-extracted per byte
@@ -398,10 +480,10 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
       }
     } else if (valid > 0) {
       uint32_t a[IPT / 4 + 1];
-      load_lane_part(p, base, ts.shift, valid, a);
+      load_lane_part(p, base, ls.shift, valid, a);
 #pragma unroll
       for (int k = 0; k < IPT / 4; ++k) {
-        const uint32_t wk = lane_word(a, k, ts.shift, row);
+        const uint32_t wk = lane_word(a, k, ls.shift, row);
         waddr = waddr < sink ? waddr : sink;
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
@@ -414,13 +496,13 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
 #pragma unroll
     for (int q = 0; q < IPT / 32; ++q) cnt += __builtin_popcount(km[q]);
     const bool wave_over = __ballot(cnt > rcap) != 0;
-    const bool owns_end = valid > 0 && base + valid == n;
+    const bool owns_end = valid > 0 && base + valid == ls.seg_hi;
     uint32_t fin_t = 0;
     bool fin_kept = false;
     if (owns_end) {
       fin_t = lds_load16(row + 512);
       if ((int)fin_t == p.dead) {
-        atomicOr(p.seg_flags + ts.seg, LEX_REJECT_FINAL);
+        atomicOr(p.seg_flags + ls.seg, LEX_REJECT_FINAL);
       } else if ((int)fin_t != ignore) {
         fin_kept = true;
       }
@@ -440,6 +522,10 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
       w.wave_over[wid] = wave_over;
     }
     __syncthreads();
+    // walk 1 is over, its sink rows free until the next tile: the byte
+    // offsets of the inputs starting in the tile, read from the first
+    // round's write-out on
+    if (ts.multi()) input_starts(p, ts, s_istart);
     uint32_t wc = 0, wl = 0, tc = 0, tl = 0;
     bool over = false;
     for (uint32_t i = 0; i < NW; ++i) {
@@ -502,13 +588,15 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
       }
     }
     // where the lane's first token starts: the last boundary of an earlier
-    // lane, else (NO_START) the tile's carried start, known after the lookback
+    // lane, else (NO_START) the last boundary before the tile, known after
+    // the lookback; never before the token's input's first byte (applied at
+    // the write-out, where a boundary of an earlier input gives way to it)
     constexpr uint32_t NO_START = 0xFFFFFFFFu;
     const uint32_t start_in = elast ? elast : NO_START;
     uint64_t slot0 = 0;
     uint32_t carried = 0;
     bool overflow = false;
-    const uint32_t lane_off = tid * IPT;
+    const uint32_t lane_off = (uint32_t)(base - tbase);  // the tile's bytes are contiguous
 
     // per round of STAGE slots, the lanes owning slots of the round stage
     // (terminal, start, end) by slot; then the workgroup writes the round
@@ -570,27 +658,52 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
       }
       __syncthreads();
       if (lo == 0) {
-        slot0 = w.pre_cnt;  // output slot of the tile's first kept token
-        // the tile's carried start: the last boundary before it, never before its input's first byte
-        carried = w.pre_last > (uint32_t)ts.seg_lo ? w.pre_last : (uint32_t)ts.seg_lo;
-        if (tid == 0 && ts.first) p.seg_tok[ts.seg] = slot0;
+        slot0 = w.pre_cnt;     // output slot of the tile's first kept token
+        carried = w.pre_last;  // the last boundary before the tile
+        if (ls.first) p.seg_tok[ls.seg] = slot0 + ecnt;
         if (tid == 0 && tile == p.ntiles - 1) p.seg_tok[p.nseg] = slot0 + tc;
       }
       const uint32_t m = tc - lo < STAGE ? tc - lo : STAGE;
       const uint64_t max_len = p.max_len;
-      for (uint32_t i = tid; i < m; i += WG) {
-        const uint64_t slot = slot0 + lo + i;
-        const uint32_t b0 = s_start[i], b = b0 == NO_START ? carried : b0;
-        const uint64_t len = tbase + s_end[i] - b;
-        overflow |= len > max_len;
-        p.terminals[slot] = s_term[i];
-        p.starts[slot] = (int64_t)(b - ts.seg_lo);
-        store_length(p, slot, len);
+      if (!ts.multi()) {
+        // one input: a carried start never lies before its first byte
+        const uint32_t cs = carried > (uint32_t)ts.seg_lo ? carried : (uint32_t)ts.seg_lo;
+        for (uint32_t i = tid; i < m; i += WG) {
+          const uint64_t slot = slot0 + lo + i;
+          const uint32_t b0 = s_start[i], b = b0 == NO_START ? cs : b0;
+          const uint64_t len = tbase + s_end[i] - b;
+          overflow |= len > max_len;
+          p.terminals[slot] = s_term[i];
+          p.starts[slot] = (int64_t)(b - ts.seg_lo);
+          store_length(p, slot, len);
+        }
+      } else {
+        for (uint32_t i = tid; i < m; i += WG) {
+          const uint64_t slot = slot0 + lo + i;
+          const uint32_t e = s_end[i];
+          // the token's input: the one holding its last byte (before the
+          // tile when it closes at the tile's first byte: the first input)
+          uint32_t seg = ts.s_lo;
+          uint64_t seg_lo = ts.seg_lo;
+          const uint32_t d = e ? count_le(s_istart, e - 1) : 0u;
+          if (d) {
+            seg = ts.s_lo + d;
+            seg_lo = tbase + s_istart[d];
+          }
+          const uint32_t b0 = s_start[i];
+          uint64_t b = b0 == NO_START ? carried : b0;
+          b = b > seg_lo ? b : seg_lo;
+          const uint64_t len = tbase + e - b;
+          if (len > max_len) atomicOr(p.seg_flags + seg, LEX_REJECT_OVERFLOW);
+          p.terminals[slot] = s_term[i];
+          p.starts[slot] = (int64_t)(b - seg_lo);
+          store_length(p, slot, len);
+        }
       }
       __syncthreads();
       lo += STAGE;
     } while (lo < tc);
-    if (overflow) atomicOr(p.seg_flags + ts.seg, LEX_REJECT_OVERFLOW);
+    if (overflow) atomicOr(p.seg_flags + ts.s_lo, LEX_REJECT_OVERFLOW);
   }
 }
 
@@ -647,6 +760,25 @@ __global__ void tile_map(const uint32_t* tile_first, uint32_t nseg, uint32_t nti
   tile_seg[t] = lo;
 }
 
+namespace {
+// tile_seg[2t] and tile_seg[2t + 1]: the inputs of the first and the last
+// chunk of lexer tile t (the last i with chunk_first[i] <= chunk).
+__global__ void lex_tile_map(const uint32_t* chunk_first, uint32_t nseg, uint32_t ntiles, uint32_t* tile_seg) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 2 * ntiles) return;
+  const uint32_t t = i >> 1, nchunks = chunk_first[nseg];
+  const uint32_t c = (i & 1) ? min(t * WG + WG - 1, nchunks - 1) : t * WG;
+  uint32_t lo = 0, hi = nseg;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (chunk_first[mid] <= c)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  tile_seg[i] = lo;
+}
+}  // namespace
 
 hipError_t lex_grid_size(const LexParams& p, int* fold_grid, int* emit_grid) {
   hipError_t e = occupancy_grid(lex_fold, lex_fold_lds(p), fold_grid);
@@ -660,7 +792,7 @@ hipError_t lex_launch(const LexParams& p, int fold_grid, int emit_grid, hipStrea
   int g1 = fold_grid < (int)nt ? fold_grid : (int)nt;
   int g3 = emit_grid < (int)nt ? emit_grid : (int)nt;
   hipError_t e;
-  hipLaunchKernelGGL(tile_map, dim3((nt + 255) / 256), dim3(256), 0, st, p.tile_first, p.nseg, nt, p.tile_seg);
+  hipLaunchKernelGGL(lex_tile_map, dim3((2 * nt + 255) / 256), dim3(256), 0, st, p.chunk_first, p.nseg, nt, p.tile_seg);
   if ((e = tm.begin(st, "lex_fold")) != hipSuccess) return e;
   hipLaunchKernelGGL(lex_fold, dim3(g1), dim3(WG), lex_fold_lds(p), st, p);
   if ((e = tm.end(st)) != hipSuccess || (e = tm.begin(st, "lex_tile_scan")) != hipSuccess) return e;

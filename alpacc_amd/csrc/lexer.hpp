@@ -9,22 +9,24 @@
 namespace alpacc {
 
 constexpr int LEX_WG = 256;   // lanes per workgroup (4 waves)
-constexpr int LEX_IPT = 128;  // input bytes per lane; a tile is 32 KiB
+constexpr int LEX_IPT = 128;  // input bytes per lane (a chunk); a tile is 256 chunks, 32 KiB
 constexpr uint32_t LEX_TILE = (uint32_t)LEX_WG * LEX_IPT;
 
 constexpr uint32_t LEX_REJECT_FINAL = 1u;     // last state not accepting
 constexpr uint32_t LEX_REJECT_OVERFLOW = 2u;  // a kept token is longer than length_t
 
 // A launch lexes `nseg` inputs packed back to back in `in` (a single input
-// is a batch of one).  Every input owns whole tiles -- at least one, so an
-// empty input has a tile with no bytes -- and tile_first[i] is the first
-// tile of input i (tile_first[nseg] = ntiles).
+// is a batch of one).  Every input owns whole chunks of LEX_IPT bytes (its
+// last one partial) -- at least one, so an empty input has a chunk with no
+// bytes -- and chunk_first[i] is the first chunk of input i
+// (chunk_first[nseg] = all chunks).  Tiles are LEX_WG consecutive chunks, so
+// a tile holds up to LEX_WG inputs and its bytes are one contiguous range.
 struct LexParams {
   const uint8_t* in;
-  uint64_t in_bytes;           // bytes of the packed inputs at `in`
-  const uint64_t* seg_off;     // [nseg + 1] byte offset of each input in `in`
-  const uint32_t* tile_first;  // [nseg + 1]
-  uint32_t* tile_seg;          // [ntiles] input of each tile (written by tile_map)
+  uint64_t in_bytes;            // bytes of the packed inputs at `in`
+  const uint64_t* seg_off;      // [nseg + 1] byte offset of each input in `in`
+  const uint32_t* chunk_first;  // [nseg + 1]
+  uint32_t* tile_seg;           // [2 * ntiles] inputs of each tile's first and last chunks
   uint32_t nseg;
   // tables
   const uint8_t* byte_class;
@@ -44,8 +46,10 @@ struct LexParams {
   uint32_t* seg_flags;  // [nseg] LEX_REJECT_* bits, zeroed before the launch
   uint64_t* seg_tok;    // [nseg + 1] first token slot of each input; [nseg] = all tokens
   // scan state
-  uint16_t* lane_prefix;   // [ntiles * LEX_WG] exclusive endomorphism of each lane inside its tile
-  uint32_t* tile_endo;     // [ntiles] tile aggregates
+  // [ntiles * LEX_WG] exclusive endomorphism of each lane inside its tile,
+  // bit 15 set when its input starts inside the tile (at or before the lane)
+  uint16_t* lane_prefix;
+  uint32_t* tile_endo;     // [ntiles] tile aggregates, bit 31 set when an input starts inside the tile
   uint16_t* tile_prefix;   // [ntiles] exclusive tile prefixes inside the input
   uint32_t* tile_blk;      // [2 * ceil(ntiles / 256)] scan blocks' aggregates, then their carries
   uint64_t* count_status;  // [ntiles] lookback words, zeroed before every launch
@@ -63,7 +67,8 @@ __device__ __forceinline__ void store_length(const LexParams& p, uint64_t slot, 
   }
 }
 
-// tile_seg[t] = the last i with tile_first[i] <= t, for t < ntiles.
+// tile_seg[t] = the last i with tile_first[i] <= t, for t < ntiles (the
+// parser's tiles, whole tiles per input).
 __global__ void tile_map(const uint32_t* tile_first, uint32_t nseg, uint32_t ntiles, uint32_t* tile_seg);
 
 size_t lex_fold_lds(const LexParams& p);

@@ -38,19 +38,19 @@ struct PinnedBuf {
 };
 
 // Byte layout of a batch's metadata, the same on the device and in the
-// pinned host mirror: uploaded [seg_off u64 | tile_first u32], read back
-// [seg_tok u64 | seg_flags u32], then (device only) tile_seg u32.
+// pinned host mirror: uploaded [seg_off u64 | chunk_first u32], read back
+// [seg_tok u64 | seg_flags u32], then (device only) tile_seg u32[2 * ntiles].
 struct BatchLayout {
-  size_t off_seg, off_tf, off_tok, off_flags, off_tseg, up_bytes, down_bytes, total;
+  size_t off_seg, off_cf, off_tok, off_flags, off_tseg, up_bytes, down_bytes, total;
   BatchLayout(uint64_t count, uint64_t ntiles) {
     off_seg = 0;
-    off_tf = (count + 1) * 8;
-    up_bytes = off_tf + (count + 1) * 4;
+    off_cf = (count + 1) * 8;
+    up_bytes = off_cf + (count + 1) * 4;
     off_tok = (up_bytes + 7) & ~size_t(7);
     off_flags = off_tok + (count + 1) * 8;
     down_bytes = off_flags + count * 4 - off_tok;
     off_tseg = (off_flags + count * 4 + 255) & ~size_t(255);
-    total = off_tseg + ntiles * 4 + 64;
+    total = off_tseg + ntiles * 8 + 64;
   }
 };
 
