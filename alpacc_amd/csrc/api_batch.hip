@@ -28,8 +28,128 @@ int stage_input(Context& c, const void* host, uint64_t bytes, hipStream_t st, co
 
 namespace {
 
-// Lexer + parser over a batch in device memory: fills c.lb_* and c.pb_res.
+
+// The combined sequence over a batch of `count` inputs at d_in, sized for
+// the capacity `caps` (count = caps.count, padded), enqueued on st.
+int lex_parse_enqueue_caps(Context& c, const uint8_t* d_in, const uint64_t* off, const BatchCaps& caps, hipStream_t st) {
+  int r = lex_batch_on_device(c, ALPACC_MODE_BOTH, d_in, off, caps.count, st, &caps);
+  if (r < 0) return r;
+  return parse_batch_on_device(c, c.lx_term.as<uint8_t>(), c.d_lb_tok, nullptr, nullptr, caps.count, true, st, &caps,
+                               c.d_lb_flags, c.d_lb_live);
+}
+
+uint64_t pow2_at_least(uint64_t x, uint64_t lo) {
+  uint64_t p = lo;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+// Lexer + parser over a batch in device memory through a graph: the
+// sequence is captured once for a capacity (inputs and bytes rounded up to
+// powers of two, growing with the batches seen) and replayed for every
+// batch within it that reads the same inputs pointer on the same stream,
+// while the buffers it uses stay where they were; otherwise it is captured
+// again.  A capture first runs the sequence eagerly (sizing every buffer,
+// producing this batch's results), then records it.  Fills c.lb_* and
+// c.pb_res for the first `count` inputs, like lex_parse_batch.
+int lex_parse_batch_graph(Context& c, const uint8_t* d_in, const uint64_t* off, uint64_t count, hipStream_t st) {
+  if (count == 0 || count > 0x7FFFFFFFull) return fail(ALPACC_ERR_ARG, "batch of %llu inputs", (unsigned long long)count);
+  for (uint64_t i = 0; i < count; ++i)
+    if (off[i + 1] < off[i]) return fail(ALPACC_ERR_ARG, "offsets decrease at input %llu", (unsigned long long)i);
+  const uint64_t bytes = off[count];
+  BatchCaps caps{pow2_at_least(count, 16), pow2_at_least(bytes, 1 << 16), count};
+  if (c.graph) {  // a larger capacity already captured serves this batch too
+    caps.count = caps.count > c.graph_caps.count ? caps.count : c.graph_caps.count;
+    caps.bytes = caps.bytes > c.graph_caps.bytes ? caps.bytes : c.graph_caps.bytes;
+  }
+  std::vector<uint64_t> padded(caps.count + 1);
+  for (uint64_t i = 0; i <= caps.count; ++i) padded[i] = off[i < count ? i : count];
+  const bool timing = c.timer.on;
+  c.timer.on = false;  // no event pairs inside a graph
+  int r = 0;
+  const bool reuse = c.graph && caps.count == c.graph_caps.count && caps.bytes == c.graph_caps.bytes &&
+                     c.graph_in == d_in && c.graph_stream == st && c.graph_gen == c.graph_buffers_gen();
+  auto copy = [&](const Context::Copy& x) {
+    return x.bytes ? hipMemcpyAsync(x.dst, x.src, x.bytes, x.kind, st) : hipSuccess;
+  };
+  Context::GraphState& gs = c.graph_state;
+  if (reuse) {
+    // the batch's metadata (host block), its upload, the graph, the results
+    r = lex_batch_on_device(c, ALPACC_MODE_BOTH, d_in, padded.data(), caps.count, st, &caps, false);
+    if (r >= 0) {
+      hipError_t e = copy(gs.up);
+      if (e == hipSuccess) e = hipGraphLaunch(c.graph, st);
+      if (e == hipSuccess) e = copy(gs.lex_down);
+      if (e == hipSuccess) e = copy(gs.parse_down);
+      if (e == hipSuccess) e = hipStreamSynchronize(st);
+      if (e != hipSuccess) r = fail(ALPACC_ERR_HIP, "graph replay: %s", hipGetErrorString(e));
+      c.graph_replays++;
+      // the results where the captured sequence leaves them
+      c.lb_count = gs.lb_count;
+      c.lb_flags = gs.lb_flags;
+      c.lb_tok = gs.lb_tok;
+      c.d_lb_flags = gs.d_lb_flags;
+      c.d_lb_tok = gs.d_lb_tok;
+      c.d_lb_live = gs.d_lb_live;
+      c.pb_count = gs.pb_count;
+      c.pb_res = gs.pb_res;
+    }
+  } else {
+    if (c.graph) {
+      (void)hipGraphExecDestroy(c.graph);
+      c.graph = nullptr;
+    }
+    // eager run: sizes every buffer and answers this batch
+    r = lex_parse_enqueue_caps(c, d_in, padded.data(), caps, st);
+    if (r >= 0) {
+      hipError_t e = hipStreamSynchronize(st);
+      if (e != hipSuccess) r = fail(ALPACC_ERR_HIP, "batch: %s", hipGetErrorString(e));
+    }
+    if (r >= 0) {
+      const uint64_t gen = c.graph_buffers_gen();
+      hipGraph_t graph = nullptr;
+      hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+      if (e == hipSuccess) {
+        c.capturing = true;  // device work only: the copies run around each launch
+        r = lex_parse_enqueue_caps(c, d_in, padded.data(), caps, st);
+        c.capturing = false;
+        e = hipStreamEndCapture(st, &graph);
+      }
+      if (r >= 0 && e == hipSuccess && c.graph_buffers_gen() != gen) r = fail(ALPACC_ERR_HIP, "buffers moved during capture");
+      if (r >= 0 && e == hipSuccess) e = hipGraphInstantiate(&c.graph, graph, nullptr, nullptr, 0);
+      if (graph) (void)hipGraphDestroy(graph);
+      if (r >= 0 && e != hipSuccess) r = fail(ALPACC_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+      if (r >= 0) {
+        c.graph_caps = caps;
+        c.graph_in = d_in;
+        c.graph_stream = st;
+        c.graph_gen = gen;
+        c.graph_captures++;
+        gs.up = c.cp_meta_up;
+        gs.lex_down = c.cp_lex_down;
+        gs.parse_down = c.cp_parse_down;
+        gs.lb_count = c.lb_count;
+        gs.lb_flags = c.lb_flags;
+        gs.lb_tok = c.lb_tok;
+        gs.d_lb_flags = c.d_lb_flags;
+        gs.d_lb_tok = c.d_lb_tok;
+        gs.d_lb_live = c.d_lb_live;
+        gs.pb_count = c.pb_count;
+        gs.pb_res = c.pb_res;
+      } else if (c.graph) {
+        (void)hipGraphExecDestroy(c.graph);
+        c.graph = nullptr;
+      }
+    }
+  }
+  c.timer.on = timing;
+  return r < 0 ? r : 0;
+}
+
+// Lexer + parser over a batch in device memory: fills c.lb_* and c.pb_res
+// (through the captured graph when the context has graphs on).
 int lex_parse_batch(Context& c, const uint8_t* d_in, const uint64_t* off, uint64_t count, hipStream_t st) {
+  if (c.use_graphs) return lex_parse_batch_graph(c, d_in, off, count, st);
   int r = lex_batch_on_device(c, ALPACC_MODE_BOTH, d_in, off, count, st);
   if (r < 0) return r;
   return parse_batch_on_device(c, c.lx_term.as<uint8_t>(), c.d_lb_tok, c.lb_tok, c.lb_flags, count, true, st);
@@ -236,6 +356,25 @@ int alpacc_hip_run_batch(alpacc_hip_ctx* h, int mode, const uint8_t* packed, con
   }
   *records = out;
   *records_len = len;
+  return 0;
+}
+
+int alpacc_hip_set_graphs(alpacc_hip_ctx* h, int on) {
+  if (!h) return fail(ALPACC_ERR_ARG, "null argument");
+  Context& c = h->c;
+  c.use_graphs = on != 0;
+  if (!c.use_graphs && c.graph) {
+    HIP_TRY(hipSetDevice(c.device));
+    (void)hipGraphExecDestroy(c.graph);
+    c.graph = nullptr;
+  }
+  return 0;
+}
+
+int alpacc_hip_graph_stats(const alpacc_hip_ctx* h, uint64_t* captures, uint64_t* replays) {
+  if (!h || !captures || !replays) return fail(ALPACC_ERR_ARG, "null argument");
+  *captures = h->c.graph_captures;
+  *replays = h->c.graph_replays;
   return 0;
 }
 

@@ -22,10 +22,12 @@ namespace alpacc {
 constexpr uint64_t LEX_MAX_INPUT = (1ull << 31) - 1;
 
 int lex_batch_on_device(Context& c, int mode, const uint8_t* d_in, const uint64_t* h_off, uint64_t count,
-                        hipStream_t st) {
+                        hipStream_t st, const BatchCaps* caps, bool enqueue) {
   const GrammarDesc* g = c.g;
   if (count == 0 || count > 0xFFFFFFFEull) return fail(ALPACC_ERR_ARG, "batch of %llu inputs", (unsigned long long)count);
   const uint64_t total = h_off[count] - h_off[0];
+  if (caps && (count != caps->count || h_off[count] > caps->bytes))
+    return fail(ALPACC_ERR_ARG, "batch beyond the capacity its launch sequence is sized for");
   if (h_off[count] > LEX_MAX_INPUT)
     return fail(ALPACC_ERR_LIMIT, "inputs of %llu bytes exceed the %llu-byte launch limit",
                 (unsigned long long)h_off[count], (unsigned long long)LEX_MAX_INPUT);
@@ -39,7 +41,10 @@ int lex_batch_on_device(Context& c, int mode, const uint8_t* d_in, const uint64_
     nchunks += n ? (n + LEX_IPT - 1) / LEX_IPT : 1;
   }
   if (nchunks > 0x7FFFFFFFull) return fail(ALPACC_ERR_LIMIT, "too many lexer chunks");
-  const uint64_t ntiles = (nchunks + LEX_WG - 1) / LEX_WG;
+  const uint64_t nt_actual = (nchunks + LEX_WG - 1) / LEX_WG;
+  // at capacity: at most bytes / LEX_IPT full chunks plus one partial (or
+  // empty) chunk per input
+  const uint64_t ntiles = caps ? (caps->bytes / LEX_IPT + caps->count + LEX_WG - 1) / LEX_WG : nt_actual;
   const BatchLayout L(count, ntiles);
   int r;
   if ((r = c.lx_meta.reserve(L.total)) || (r = c.h_meta.reserve(L.total))) return r;
@@ -56,9 +61,12 @@ int lex_batch_on_device(Context& c, int mode, const uint8_t* d_in, const uint64_
   }
   h_seg[count] = h_off[count];
   h_cf[count] = k;
+  *reinterpret_cast<uint32_t*>(hm + L.off_nt) = (uint32_t)nt_actual;
+  *reinterpret_cast<uint32_t*>(hm + L.off_live) = (uint32_t)(caps ? caps->live : count);
+  if (!enqueue) return 0;
 
   const int lb = g->length_bytes;
-  const uint64_t cap = total + 1;
+  const uint64_t cap = (caps ? caps->bytes : total) + 1;
   // scan state: count_status u64[ntiles] | tile_endo u32[ntiles] | tile_prefix u16[ntiles] |
   // tile_blk u32[2 * scan blocks] | lane_prefix
   const size_t nblk = ((size_t)ntiles + 255) / 256;
@@ -73,6 +81,7 @@ int lex_batch_on_device(Context& c, int mode, const uint8_t* d_in, const uint64_
   p.in_bytes = h_off[count];
   p.seg_off = reinterpret_cast<const uint64_t*>(dm + L.off_seg);
   p.chunk_first = reinterpret_cast<const uint32_t*>(dm + L.off_cf);
+  p.nt_dev = reinterpret_cast<const uint32_t*>(dm + L.off_nt);
   p.tile_seg = reinterpret_cast<uint32_t*>(dm + L.off_tseg);
   p.nseg = (uint32_t)count;
   p.byte_class = c.d_class;
@@ -103,17 +112,22 @@ int lex_batch_on_device(Context& c, int mode, const uint8_t* d_in, const uint64_
   p.lane_prefix = reinterpret_cast<uint16_t*>(sb + lp_off);
   p.ntiles = (uint32_t)ntiles;
   if (!c.lex_grid) HIP_TRY(lex_grid_size(p, &c.lex_grid, &c.lex_grid2));
-  HIP_TRY(hipMemcpyAsync(dm, hm, L.up_bytes, hipMemcpyHostToDevice, st));
+  c.cp_meta_up = Context::Copy{dm, hm, L.up_bytes, hipMemcpyHostToDevice};
+  if (!c.capturing) HIP_TRY(hipMemcpyAsync(dm, hm, L.up_bytes, hipMemcpyHostToDevice, st));
   if ((r = zero_words(c.small.ptr, 16, st)) || (r = zero_words(c.lx_status.ptr, st_bytes, st)) ||
       (r = zero_words(dm + L.off_flags, count * 4, st)))
     return r;
-  if (c.timing) HIP_TRY(hipEventRecord(c.ev[0], st));
+  const bool timed = c.timing && !caps;
+  if (timed) HIP_TRY(hipEventRecord(c.ev[0], st));
   HIP_TRY(lex_launch(p, c.lex_grid, c.lex_grid2, st, c.timer));
-  if (c.timing) HIP_TRY(hipEventRecord(c.ev[1], st));
-  HIP_TRY(hipMemcpyAsync(hm + L.off_tok, dm + L.off_tok, L.down_bytes, hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipStreamSynchronize(st));
-  HIP_TRY(c.timer.collect());
-  if (c.timing) {
+  if (timed) HIP_TRY(hipEventRecord(c.ev[1], st));
+  c.cp_lex_down = Context::Copy{hm + L.off_tok, dm + L.off_tok, L.down_bytes, hipMemcpyDeviceToHost};
+  if (!c.capturing) HIP_TRY(hipMemcpyAsync(hm + L.off_tok, dm + L.off_tok, L.down_bytes, hipMemcpyDeviceToHost, st));
+  if (!caps) {
+    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(c.timer.collect());
+  }
+  if (timed) {
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, c.ev[0], c.ev[1]));
     c.lex_ms += ms;
@@ -124,6 +138,7 @@ int lex_batch_on_device(Context& c, int mode, const uint8_t* d_in, const uint64_
   c.lb_flags = reinterpret_cast<const uint32_t*>(hm + L.off_flags);
   c.d_lb_tok = reinterpret_cast<const uint64_t*>(dm + L.off_tok);
   c.d_lb_flags = reinterpret_cast<const uint32_t*>(dm + L.off_flags);
+  c.d_lb_live = reinterpret_cast<const uint32_t*>(dm + L.off_live);
   return 0;
 }
 

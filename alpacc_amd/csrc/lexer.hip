@@ -193,7 +193,8 @@ __global__ __launch_bounds__(WG) void lex_fold(LexParams p) {
   const uint32_t nendo = p.nendo, nclass = p.nclass;
   auto comp = [&](uint32_t a, uint32_t b) -> uint32_t { return compose[a * nendo + b]; };
 
-  for (uint32_t tile = blockIdx.x; tile < p.ntiles; tile += gridDim.x) {
+  const uint32_t nt = *p.nt_dev;
+  for (uint32_t tile = blockIdx.x; tile < nt; tile += gridDim.x) {
     const TileSpan ts = tile_span(p, tile);
     if (ts.multi()) {
       tile_inputs(p, ts, tile, s_ichunk);
@@ -303,7 +304,7 @@ __device__ __forceinline__ void seg_comp_block(const uint16_t* __restrict__ comp
 }
 
 __global__ __launch_bounds__(TSB) void lex_tile_scan1(LexParams p) {
-  const uint32_t t = blockIdx.x * TSB + threadIdx.x, nt = p.ntiles;
+  const uint32_t t = blockIdx.x * TSB + threadIdx.x, nt = *p.nt_dev;
   const bool ok = t < nt;
   const uint32_t a = ok ? p.tile_endo[t] : 0u;
   uint32_t xf, xe, tf, te;
@@ -335,7 +336,7 @@ __global__ __launch_bounds__(TSB) void lex_tile_scan2(LexParams p, uint32_t nblk
 
 __global__ __launch_bounds__(TSB) void lex_tile_scan3(LexParams p, uint32_t nblk) {
   const uint32_t t = blockIdx.x * TSB + threadIdx.x;
-  if (t >= p.ntiles) return;
+  if (t >= *p.nt_dev) return;
   const uint32_t w = p.tile_endo[t], carry = p.tile_blk[nblk + t / TSB];
   p.tile_prefix[t] = (uint16_t)((w >> 31) ? (w & 0x7FFFFFFFu) : p.compose[carry * p.nendo + w]);
 }
@@ -427,12 +428,13 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
   // s_lterm[r][lane] is the byte at LDS address lterm0 + r * WG + lane; a
   // lane's writes past rcap tokens go to its own byte of the sink rows
   const uint32_t lterm0 = lds0 + L.lterm, sink = lterm0 + rcap * WG + tid;
+  const uint32_t nt = *p.nt_dev;
 
   for (;;) {
     if (tid == 0) w.tile = atomicAdd(p.ticket, 1u);
     __syncthreads();
     const uint32_t tile = __builtin_amdgcn_readfirstlane(w.tile);
-    if (tile >= p.ntiles) break;
+    if (tile >= nt) break;
     const TileSpan ts = tile_span(p, tile);
     if (ts.multi()) {
       tile_inputs(p, ts, tile, s_ichunk);
@@ -661,7 +663,7 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
         slot0 = w.pre_cnt;     // output slot of the tile's first kept token
         carried = w.pre_last;  // the last boundary before the tile
         if (ls.first) p.seg_tok[ls.seg] = slot0 + ecnt;
-        if (tid == 0 && tile == p.ntiles - 1) p.seg_tok[p.nseg] = slot0 + tc;
+        if (tid == 0 && tile == nt - 1) p.seg_tok[p.nseg] = slot0 + tc;
       }
       const uint32_t m = tc - lo < STAGE ? tc - lo : STAGE;
       const uint64_t max_len = p.max_len;
@@ -746,26 +748,12 @@ uint32_t emit_rcap(const LexParams& p) {
 }  // namespace
 
 size_t lex_emit_lds(const LexParams& p) { return EmitLayout((uint32_t)p.nstate, emit_rcap(p)).total; }
-__global__ void tile_map(const uint32_t* tile_first, uint32_t nseg, uint32_t ntiles, uint32_t* tile_seg) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ntiles) return;
-  uint32_t lo = 0, hi = nseg;
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (tile_first[mid] <= t)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  tile_seg[t] = lo;
-}
-
 namespace {
 // tile_seg[2t] and tile_seg[2t + 1]: the inputs of the first and the last
 // chunk of lexer tile t (the last i with chunk_first[i] <= chunk).
-__global__ void lex_tile_map(const uint32_t* chunk_first, uint32_t nseg, uint32_t ntiles, uint32_t* tile_seg) {
+__global__ void lex_tile_map(const uint32_t* chunk_first, uint32_t nseg, const uint32_t* nt_dev, uint32_t* tile_seg) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= 2 * ntiles) return;
+  if (i >= 2 * *nt_dev) return;
   const uint32_t t = i >> 1, nchunks = chunk_first[nseg];
   const uint32_t c = (i & 1) ? min(t * WG + WG - 1, nchunks - 1) : t * WG;
   uint32_t lo = 0, hi = nseg;
@@ -792,7 +780,8 @@ hipError_t lex_launch(const LexParams& p, int fold_grid, int emit_grid, hipStrea
   int g1 = fold_grid < (int)nt ? fold_grid : (int)nt;
   int g3 = emit_grid < (int)nt ? emit_grid : (int)nt;
   hipError_t e;
-  hipLaunchKernelGGL(lex_tile_map, dim3((2 * nt + 255) / 256), dim3(256), 0, st, p.chunk_first, p.nseg, nt, p.tile_seg);
+  hipLaunchKernelGGL(lex_tile_map, dim3((2 * nt + 255) / 256), dim3(256), 0, st, p.chunk_first, p.nseg, p.nt_dev,
+                     p.tile_seg);
   if ((e = tm.begin(st, "lex_fold")) != hipSuccess) return e;
   hipLaunchKernelGGL(lex_fold, dim3(g1), dim3(WG), lex_fold_lds(p), st, p);
   if ((e = tm.end(st)) != hipSuccess || (e = tm.begin(st, "lex_tile_scan")) != hipSuccess) return e;

@@ -54,6 +54,10 @@ struct LexParams {
   uint32_t* tile_blk;      // [2 * ceil(ntiles / 256)] scan blocks' aggregates, then their carries
   uint64_t* count_status;  // [ntiles] lookback words, zeroed before every launch
   uint32_t* ticket;        // [1] zeroed before every launch
+  // tiles of the launch: *nt_dev in device memory; ntiles is an upper bound
+  // (equal to it unless the launch sequence is sized for a batch capacity)
+  // that sizes the grids and the scan state
+  const uint32_t* nt_dev;
   uint32_t ntiles;
   uint32_t rcap;  // set by lex_launch: kept tokens per lane lex_emit keeps terminals for
 };
@@ -66,10 +70,6 @@ __device__ __forceinline__ void store_length(const LexParams& p, uint64_t slot, 
     default: static_cast<uint64_t*>(p.lengths)[slot] = len; break;
   }
 }
-
-// tile_seg[t] = the last i with tile_first[i] <= t, for t < ntiles (the
-// parser's tiles, whole tiles per input).
-__global__ void tile_map(const uint32_t* tile_first, uint32_t nseg, uint32_t ntiles, uint32_t* tile_seg);
 
 size_t lex_fold_lds(const LexParams& p);
 size_t lex_emit_lds(const LexParams& p);

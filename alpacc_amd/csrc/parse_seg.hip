@@ -34,6 +34,8 @@
 //   F  seg_finish    per input: balance, counts, offsets, verdict.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "../../include/alpacc_hip.h"
 #include "device.hpp"
 #include "lexer.hpp"
@@ -64,6 +66,10 @@ struct Seg {
   const uint64_t* tok_off;     // [count + 1] first token of each input
   const uint32_t* tile_first;  // [count + 1] first tile of each input (inputs not parsed have none)
   const uint32_t* tile_seg;    // [ntiles] input of each tile
+  // tiles of the launch: *nt_dev in device memory; ntiles is an upper bound
+  // (equal to it unless the sequence is sized for a batch capacity) that
+  // sizes the grids and the workspace
+  const uint32_t* nt_dev;
   uint32_t count, ntiles;
   int q, k, base, empty_t, start_t, end_t;
   const uint16_t* window;
@@ -207,8 +213,8 @@ constexpr int APL = 4;
 constexpr int AWG = PT / APL;
 constexpr int ANW = AWG / WAVE;
 
-__global__ __launch_bounds__(AWG) void seg_profile(Seg s, TileRec* rec, uint16_t* pid) {
-  const uint32_t tile = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+__device__ __forceinline__ void profile_tile(const Seg& s, uint32_t tile, uint32_t tid, TileRec* rec, uint16_t* pid) {
+  const uint32_t lane = tid & 63, wid = tid >> 6;
   const TileOf t = tile_of(s, tile);
   const int64_t p0 = (int64_t)t.lt * PT + (int64_t)tid * APL;
   int32_t np = 0, nt = 0;
@@ -276,6 +282,32 @@ __global__ __launch_bounds__(AWG) void seg_profile(Seg s, TileRec* rec, uint16_t
   }
 }
 
+// The thread index, opaque to the compiler: values derived from it are
+// formed inside each tile's iteration instead of held in registers across
+// the whole loop.
+__device__ __forceinline__ uint32_t tid_per_tile() {
+  uint32_t tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  return tid;
+}
+
+// One tile per workgroup: workgroup t takes tile t, or (STRIDE, for a grid
+// sized before the tile count is known) the grid strides over the tiles.
+// The loop is a separate instantiation because values the compiler hoists
+// out of it cost the single-tile form its occupancy.
+template <bool STRIDE>
+__global__ __launch_bounds__(AWG) void seg_profile(Seg s, TileRec* rec, uint16_t* pid) {
+  const uint32_t nt = *s.nt_dev;
+  if (!STRIDE) {
+    if (blockIdx.x < nt) profile_tile(s, blockIdx.x, threadIdx.x, rec, pid);
+    return;
+  }
+  for (uint32_t tile = blockIdx.x; tile < nt; tile += gridDim.x) {
+    profile_tile(s, tile, tid_per_tile(), rec, pid);
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------- pass S
 
 struct ScanOut {
@@ -328,9 +360,9 @@ __device__ __forceinline__ bool is_first(const Seg& s, uint32_t t) { return s.ti
 
 __global__ __launch_bounds__(256) void tile_block_sums(Seg s, const TileRec* rec, int64_t* bsum) {
   int64_t v[NV] = {0, 0, 0, 0, 0, 0, 0, 0};
-  const uint32_t t0 = blockIdx.x * SB + threadIdx.x * (SB / 256);
+  const uint32_t t0 = blockIdx.x * SB + threadIdx.x * (SB / 256), nt = *s.nt_dev;
   for (int i = 0; i < SB / 256; ++i) {
-    if (t0 + i >= s.ntiles) break;
+    if (t0 + i >= nt) break;
     int64_t x[NV];
     tile_terms(rec[t0 + i], is_first(s, t0 + i), x);
     for (int k = 0; k < NV; ++k) v[k] += x[k];
@@ -363,9 +395,9 @@ __global__ __launch_bounds__(256) void tile_block_scan(int64_t* bsum, uint32_t n
 __global__ __launch_bounds__(256) void tile_downsweep(Seg s, const TileRec* rec, const int64_t* bsum, TileInfo* info,
                                                       SegBase* sb, int32_t* mb, int32_t* ms) {
   int64_t v[NV] = {0, 0, 0, 0, 0, 0, 0, 0};
-  const uint32_t t0 = blockIdx.x * SB + threadIdx.x * (SB / 256);
+  const uint32_t t0 = blockIdx.x * SB + threadIdx.x * (SB / 256), nt = *s.nt_dev;
   for (int i = 0; i < SB / 256; ++i) {
-    if (t0 + i >= s.ntiles) break;
+    if (t0 + i >= nt) break;
     int64_t x[NV];
     tile_terms(rec[t0 + i], is_first(s, t0 + i), x);
     for (int k = 0; k < NV; ++k) v[k] += x[k];
@@ -375,7 +407,7 @@ __global__ __launch_bounds__(256) void tile_downsweep(Seg s, const TileRec* rec,
   for (int k = 0; k < NV; ++k) v[k] += bsum[(uint64_t)blockIdx.x * NV + k];
   for (int i = 0; i < SB / 256; ++i) {
     const uint32_t t = t0 + i;
-    if (t >= s.ntiles) break;
+    if (t >= nt) break;
     const TileRec r = rec[t];
     const bool first = is_first(s, t);
     TileInfo ti;
@@ -480,7 +512,7 @@ __device__ __forceinline__ void block_scan4(int32_t* v, int32_t* tot, int32_t (*
 // lane at tile-relative level D reads entry D - min - 1 of the last earlier
 // lane whose minimum is below D (a sparse table over the lanes' minima).
 template <int MBPOP, int MBPUSH, int MNP, int MSPUSH>
-__global__ __launch_bounds__(RWG) void seg_resolve(Resolve r) {
+__device__ __forceinline__ void resolve_tile(const Resolve& r, uint32_t tile, uint32_t tid) {
   // dynamic LDS: bracket stacks u16 [bcap][RWG], slot stacks u8 [scap][RWG],
   // staged parents of nodes u16 [ncap] and of terminal productions u16
   // [tcap] (tile-local compact indices), staged productions u8 [ncap]
@@ -491,7 +523,7 @@ __global__ __launch_bounds__(RWG) void seg_resolve(Resolve r) {
   __shared__ int32_t ws[4][RNW];
   __shared__ int32_t wmin[2][RNW];
   const Seg& s = r.s;
-  const uint32_t tile = blockIdx.x, tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
+  const uint32_t lane = tid & (WAVE - 1), wid = tid / WAVE;
   const int bcap = r.bcap, scap = r.scap;
   uint16_t* const bst = dyn + tid;
   uint8_t* const sst = reinterpret_cast<uint8_t*>(dyn + bcap * RWG) + tid;
@@ -738,6 +770,20 @@ __global__ __launch_bounds__(RWG) void seg_resolve(Resolve r) {
   }
 }
 
+// One tile per workgroup, as seg_profile.
+template <int MBPOP, int MBPUSH, int MNP, int MSPUSH, bool STRIDE>
+__global__ __launch_bounds__(RWG) void seg_resolve(Resolve r) {
+  const uint32_t nt = *r.s.nt_dev;
+  if (!STRIDE) {
+    if (blockIdx.x < nt) resolve_tile<MBPOP, MBPUSH, MNP, MSPUSH>(r, blockIdx.x, threadIdx.x);
+    return;
+  }
+  for (uint32_t tile = blockIdx.x; tile < nt; tile += gridDim.x) {
+    resolve_tile<MBPOP, MBPUSH, MNP, MSPUSH>(r, tile, tid_per_tile());
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------- pass C
 
 constexpr int MT_LEVELS = 8;
@@ -790,9 +836,7 @@ struct Far {
 };
 
 // One wave per tile: a tile has only a few pops left over.
-__global__ __launch_bounds__(256) void far_resolve(Far f) {
-  const uint32_t tile = blockIdx.x * (256 / WAVE) + threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
-  if (tile >= f.s.ntiles) return;
+__device__ __forceinline__ void far_tile(const Far& f, uint32_t tile, uint32_t lane) {
   const TileInfo ti = f.info[tile];
   const uint32_t seg = f.s.tile_seg[tile];
   const int64_t first = f.s.tile_first[seg];
@@ -830,6 +874,13 @@ __global__ __launch_bounds__(256) void far_resolve(Far f) {
   }
   if (bad) atomicOr(f.s.flags + seg, BAD_MATCH);
   if (bad_root) atomicOr(f.s.flags + seg, BAD_ROOT);
+}
+
+// One wave per tile, the grid striding over the launch's tiles.
+__global__ __launch_bounds__(256) void far_resolve(Far f) {
+  const uint32_t nt = *f.s.nt_dev, lane = threadIdx.x & (WAVE - 1);
+  for (uint32_t tile = blockIdx.x * (256 / WAVE) + threadIdx.x / WAVE; tile < nt; tile += gridDim.x * (256 / WAVE))
+    far_tile(f, tile, lane);
 }
 
 __global__ __launch_bounds__(256) void tiles_min_level(const int32_t* in, uint64_t n_in, int32_t* out, uint64_t n_out) {
@@ -886,7 +937,7 @@ __global__ void seg_finish(Seg s, const TileInfo* info, const SegBase* sb, const
   }
   const SegBase in = sb[i];
   int64_t p_end, t_end, b_end, c_end;
-  if (f1 < s.ntiles) {
+  if (f1 < *s.nt_dev) {
     const TileInfo& e = info[f1];
     p_end = e.p_off;
     t_end = e.t_off;
@@ -912,12 +963,76 @@ __global__ void seg_finish(Seg s, const TileInfo* info, const SegBase* sb, const
   res[i] = o;
 }
 
+// tile_seg[t] = the last input i with tile_first[i] <= t, for the launch's
+// tiles.
+__global__ void pseg_map(const uint32_t* tile_first, uint32_t count, const uint32_t* nt_dev, uint32_t* tile_seg) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= *nt_dev) return;
+  uint32_t lo = 0, hi = count;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (tile_first[mid] <= t)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  tile_seg[t] = lo;
+}
+
+// The tiles of a batch whose token counts are known on the device only:
+// input i gets ceil((tokens + 2) / PT) tiles unless skip[i] is set or i is
+// at or past *live (padding); tile_first is their exclusive scan and
+// *nt_dev the total.  One workgroup of PLAN_WG lanes, PLAN_IPT inputs each
+// per round.
+constexpr int PLAN_WG = 1024, PLAN_IPT = 4;
+__global__ __launch_bounds__(PLAN_WG) void parse_plan(const uint64_t* tok_off, const uint32_t* skip,
+                                                      const uint32_t* live, uint32_t count, uint32_t* tile_first,
+                                                      uint32_t* nt_dev) {
+  __shared__ uint32_t s_w[PLAN_WG / WAVE];
+  const uint32_t tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE, n_live = *live;
+  uint32_t carry = 0;
+  for (uint32_t r0 = 0; r0 < count; r0 += PLAN_WG * PLAN_IPT) {
+    uint32_t t[PLAN_IPT], sum = 0;
+#pragma unroll
+    for (int k = 0; k < PLAN_IPT; ++k) {
+      const uint32_t i = r0 + tid * PLAN_IPT + k;
+      const bool on = i < count && i < n_live && !skip[i];
+      t[k] = on ? (uint32_t)((tok_off[i + 1] - tok_off[i] + 2 + PT - 1) / PT) : 0u;
+      sum += t[k];
+    }
+    const uint32_t inc = wave_scan_add32(sum);
+    if (lane == WAVE - 1) s_w[wid] = inc;
+    __syncthreads();
+    uint32_t pre = carry, tot = 0;
+    for (uint32_t w = 0; w < PLAN_WG / WAVE; ++w) {
+      if (w < wid) pre += s_w[w];
+      tot += s_w[w];
+    }
+    pre += inc - sum;
+#pragma unroll
+    for (int k = 0; k < PLAN_IPT; ++k) {
+      const uint32_t i = r0 + tid * PLAN_IPT + k;
+      if (i < count) tile_first[i] = pre;
+      pre += t[k];
+    }
+    carry += tot;
+    __syncthreads();
+  }
+  if (tid == 0) {
+    tile_first[count] = carry;
+    *nt_dev = carry;
+  }
+}
+
 }  // namespace
 
 int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_tok_off, const uint64_t* h_tok_off,
-                          const uint32_t* h_skip, uint64_t count, bool tree, hipStream_t st) {
+                          const uint32_t* h_skip, uint64_t count, bool tree, hipStream_t st, const BatchCaps* caps,
+                          const uint32_t* d_skip, const uint32_t* d_live) {
   const GrammarDesc* g = c.g;
   if (count == 0 || count > 0xFFFFFFFEull) return fail(ALPACC_ERR_ARG, "batch of %llu inputs", (unsigned long long)count);
+  if (caps && (count != caps->count || !d_skip || !d_live))
+    return fail(ALPACC_ERR_ARG, "batch beyond the capacity its launch sequence is sized for");
   const int ncap = PT * g->max_prods, tcap = PT * g->max_nt;
   const int bcap = APL * (g->max_bpop + g->max_bpush), scap = APL * (g->max_spop + g->max_spush);
   const size_t lds_b = (size_t)bcap * RWG * 2 + ((size_t)scap * RWG + 1) / 2 * 2 + (size_t)(ncap + tcap) * 2 +
@@ -926,36 +1041,50 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
   // owners 7 bits
   if ((int64_t)PT * 8 > 0x7FFF || ncap > 0xFFFF || APL * g->max_prods > 0x7F || lds_b > 96 * 1024)
     return fail(ALPACC_ERR_LIMIT, "grammar windows too large for the parser tiles");
-  // tiles per input: framed positions = tokens + 2; inputs skipped get none
+  int r;
+  // tiles per input: framed positions = tokens + 2; inputs skipped get none.
+  // At capacity every input holds at most caps->bytes tokens in all.
   uint64_t ntiles = 0, positions = 0;
-  for (uint64_t i = 0; i < count; ++i) {
-    if (h_skip && h_skip[i]) continue;
-    const uint64_t m = h_tok_off[i + 1] - h_tok_off[i] + 2;
-    ntiles += (m + PT - 1) / PT;
-    positions += m;
+  uint32_t* h_tf = nullptr;
+  if (caps) {
+    positions = caps->bytes + 2 * count;
+    ntiles = (caps->bytes + count * (PT + 1)) / PT + 1;
+    if ((r = c.h_pmeta.reserve(64))) return r;
+  } else {
+    for (uint64_t i = 0; i < count; ++i) {
+      if (h_skip && h_skip[i]) continue;
+      const uint64_t m = h_tok_off[i + 1] - h_tok_off[i] + 2;
+      ntiles += (m + PT - 1) / PT;
+      positions += m;
+    }
+    // host metadata: tile_first u32[count + 1], then the tile count
+    if ((r = c.h_pmeta.reserve((count + 2) * 4 + 64))) return r;
+    h_tf = c.h_pmeta.as<uint32_t>();
+    uint32_t tt = 0;
+    for (uint64_t i = 0; i < count; ++i) {
+      h_tf[i] = tt;
+      if (h_skip && h_skip[i]) continue;
+      tt += (uint32_t)((h_tok_off[i + 1] - h_tok_off[i] + 2 + PT - 1) / PT);
+    }
+    h_tf[count] = tt;
+    h_tf[count + 1] = tt;
   }
-  if (ntiles > 0x7FFFFFFFull || positions * 8 > 0x7FFFFFFFull)
+  // depths grow by at most the pushes of a window per position (at
+  // capacity that bound itself; else 8 per position, the bound the tiles
+  // were laid out for)
+  const uint64_t per_pos = caps ? (uint64_t)std::max(1, std::max(g->max_bpush, g->max_spush)) : 8;
+  if (ntiles > 0x7FFFFFFFull || positions * per_pos + (caps ? count : 0) > 0x7FFFFFFFull)
     return fail(ALPACC_ERR_LIMIT, "batch of %llu framed positions exceeds the parser's 31-bit depths",
                 (unsigned long long)positions);
-  int r;
-  // host metadata: tile_first u32[count + 1]
-  if ((r = c.h_pmeta.reserve((count + 1) * 4 + 64))) return r;
-  uint32_t* h_tf = c.h_pmeta.as<uint32_t>();
-  uint32_t tt = 0;
-  for (uint64_t i = 0; i < count; ++i) {
-    h_tf[i] = tt;
-    if (h_skip && h_skip[i]) continue;
-    tt += (uint32_t)((h_tok_off[i + 1] - h_tok_off[i] + 2 + PT - 1) / PT);
-  }
-  h_tf[count] = tt;
 
-  // device workspace: tile_first | tile_seg | TileRec | TileInfo | SegBase |
-  // mb | ms | trees | block sums | ScanOut | ParseResult | flags
+  // device workspace: tile_first (+ tile count) | tile_seg | TileRec |
+  // TileInfo | SegBase | mb | ms | trees | block sums | ScanOut |
+  // ParseResult | flags | pair ids
   const uint64_t tts = tree_scratch(ntiles ? ntiles : 1);
   const uint32_t nsb = (uint32_t)((ntiles + SB - 1) / SB);
   auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
   const size_t off_tf = 0;
-  const size_t off_ts = al(off_tf + (count + 1) * 4);
+  const size_t off_ts = al(off_tf + (count + 2) * 4);
   const size_t off_rec = al(off_ts + ntiles * 4);
   const size_t off_info = al(off_rec + ntiles * sizeof(TileRec));
   const size_t off_sb = al(off_info + ntiles * sizeof(TileInfo));
@@ -972,11 +1101,13 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
   if ((r = c.ps_a.reserve(total)) || (r = c.h_pres.reserve(count * sizeof(ParseResult) + sizeof(ScanOut) + 64)))
     return r;
   uint8_t* base = c.ps_a.as<uint8_t>();
+  uint32_t* d_tf = reinterpret_cast<uint32_t*>(base + off_tf);
   Seg s{};
   s.tok = d_tok;
   s.tok_off = d_tok_off;
-  s.tile_first = reinterpret_cast<const uint32_t*>(base + off_tf);
+  s.tile_first = d_tf;
   s.tile_seg = reinterpret_cast<const uint32_t*>(base + off_ts);
+  s.nt_dev = d_tf + count + 1;
   s.count = (uint32_t)count;
   s.ntiles = (uint32_t)ntiles;
   s.q = g->q;
@@ -998,31 +1129,55 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
   ScanOut* sout = reinterpret_cast<ScanOut*>(base + off_out);
   ParseResult* res = reinterpret_cast<ParseResult*>(base + off_res);
 
-  HIP_TRY(hipMemcpyAsync(base + off_tf, h_tf, (count + 1) * 4, hipMemcpyHostToDevice, st));
+  if (caps) {
+    hipLaunchKernelGGL(parse_plan, dim3(1), dim3(PLAN_WG), 0, st, d_tok_off, d_skip, d_live, s.count, d_tf, d_tf + count + 1);
+  } else {
+    HIP_TRY(hipMemcpyAsync(d_tf, h_tf, (count + 2) * 4, hipMemcpyHostToDevice, st));
+  }
   if ((r = zero_words(s.flags, count * 4, st)) || (r = zero_words(sout, sizeof(ScanOut), st))) return r;
   KernelTimer& tm = c.timer;
-  if (c.timing) HIP_TRY(hipEventRecord(c.ev[2], st));
+  const bool timed = c.timing && !caps;
+  if (timed) HIP_TRY(hipEventRecord(c.ev[2], st));
+  // grids: one workgroup per tile when the tile count is known, else as
+  // many as the device holds at once, striding over the tiles
+  const uint32_t g_map = (uint32_t)((ntiles + 255) / 256);
+  const uint32_t g_prof = (uint32_t)(caps && ntiles > 4096 ? 4096 : ntiles);
+  const uint32_t g_res = (uint32_t)(caps && ntiles > 2048 ? 2048 : ntiles);
+  const uint32_t g_far = (uint32_t)(caps && ntiles > 4096 ? 1024 : (ntiles + 3) / 4);
   if (ntiles) {
-    hipLaunchKernelGGL(tile_map, dim3((uint32_t)((ntiles + 255) / 256)), dim3(256), 0, st, s.tile_first, s.count,
-                       s.ntiles, const_cast<uint32_t*>(s.tile_seg));
-    HIP_TRY(tm.begin(st, "seg_profile"));
-    hipLaunchKernelGGL(seg_profile, dim3((uint32_t)ntiles), dim3(AWG), 0, st, s, rec,
-                       reinterpret_cast<uint16_t*>(base + off_pid));
-    HIP_TRY(tm.end(st));
-    HIP_TRY(tm.begin(st, "tile_scan"));
+    hipLaunchKernelGGL(pseg_map, dim3(g_map), dim3(256), 0, st, d_tf, s.count, s.nt_dev,
+                       const_cast<uint32_t*>(s.tile_seg));
+    if (!caps) HIP_TRY(tm.begin(st, "seg_profile"));
+    if (caps)
+      hipLaunchKernelGGL(seg_profile<true>, dim3(g_prof), dim3(AWG), 0, st, s, rec,
+                         reinterpret_cast<uint16_t*>(base + off_pid));
+    else
+      hipLaunchKernelGGL(seg_profile<false>, dim3(g_prof), dim3(AWG), 0, st, s, rec,
+                         reinterpret_cast<uint16_t*>(base + off_pid));
+    if (!caps) HIP_TRY(tm.end(st));
+    if (!caps) HIP_TRY(tm.begin(st, "tile_scan"));
     hipLaunchKernelGGL(tile_block_sums, dim3(nsb), dim3(256), 0, st, s, rec, bsum);
     hipLaunchKernelGGL(tile_block_scan, dim3(1), dim3(256), 0, st, bsum, nsb, sout);
     hipLaunchKernelGGL(tile_downsweep, dim3(nsb), dim3(256), 0, st, s, rec, bsum, info, sb, mb, ms);
-    HIP_TRY(tm.end(st));
+    if (!caps) HIP_TRY(tm.end(st));
   }
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpyAsync(c.h_pres.ptr, sout, sizeof(ScanOut), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipStreamSynchronize(st));
-  HIP_TRY(tm.collect());
-  const ScanOut so = *c.h_pres.as<ScanOut>();
+  // list and output sizes: the scan's totals, or at capacity the most that
+  // the framed positions can produce
+  ScanOut so;
+  if (caps) {
+    const int64_t pos = (int64_t)positions;
+    so = ScanOut{pos * g->max_prods, pos * g->max_nt, 0, 0,
+                 pos * g->max_bpush, pos * g->max_bpop, pos * g->max_spush, pos * g->max_spop};
+  } else {
+    HIP_TRY(hipMemcpyAsync(c.h_pres.ptr, sout, sizeof(ScanOut), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(tm.collect());
+    so = *c.h_pres.as<ScanOut>();
+  }
   const uint64_t np = (uint64_t)so.np, nterm = (uint64_t)so.nt;
-  if (so.np > 0x7FFFFFFF) return fail(ALPACC_ERR_LIMIT, "derivation beyond 2^31 entries");
-  const uint64_t nodes = np - nterm;
+  if (so.np > 0x7FFFFFFF && !caps) return fail(ALPACC_ERR_LIMIT, "derivation beyond 2^31 entries");
+  const uint64_t nodes = caps ? np : np - nterm;
   if ((r = c.ps_b.reserve((size_t)(so.ob + so.ub) * 2 + 64)) || (r = c.ps_c.reserve((size_t)(so.os + so.us) * 8 + 64)) ||
       (tree && ((r = c.tr_prod.reserve(nodes + 16)) || (r = c.tr_parent.reserve(nodes * 8 + 16)) ||
                 (r = c.tr_tokpar.reserve(nterm * 8 + 16)))) ||
@@ -1034,45 +1189,53 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
   int64_t* us_list = os_list + so.os;
   if (ntiles) {
     TileTree tb{}, ts{};
-    HIP_TRY(tm.begin(st, "tile_tree"));
+    if (!caps) HIP_TRY(tm.begin(st, "tile_tree"));
     if ((r = build_tile_tree(mb, ntiles, reinterpret_cast<int32_t*>(base + off_tb), st, &tb)) ||
         (r = build_tile_tree(ms, ntiles, reinterpret_cast<int32_t*>(base + off_tsx), st, &ts)))
       return r;
-    HIP_TRY(tm.end(st));
+    if (!caps) HIP_TRY(tm.end(st));
     Resolve rs{s,         reinterpret_cast<const uint16_t*>(base + off_pid),
                info,      sb,       bcap,  scap, ncap, tcap,
                ob_list,   ub_list,   os_list,  us_list,       c.tr_prod.as<uint8_t>(),
                c.tr_parent.as<int64_t>(), c.tr_tokpar.as<int64_t>(), tree ? nullptr : c.ps_d.as<uint8_t>(),
                tree ? 1 : 0};
-    void (*kern)(Resolve);
-    if (g->max_bpop <= 2 && g->max_bpush <= 3 && g->max_prods <= 4 && g->max_spush <= 3)
-      kern = seg_resolve<2, 3, 4, 3>;
-    else if (g->max_bpop <= 3 && g->max_bpush <= 4 && g->max_prods <= 4 && g->max_spush <= 4)
-      kern = seg_resolve<3, 4, 4, 4>;
-    else
-      kern = seg_resolve<4, 4, 8, 24>;
+    void (*kern[2])(Resolve);  // one tile per workgroup, striding
+    if (g->max_bpop <= 2 && g->max_bpush <= 3 && g->max_prods <= 4 && g->max_spush <= 3) {
+      kern[0] = seg_resolve<2, 3, 4, 3, false>;
+      kern[1] = seg_resolve<2, 3, 4, 3, true>;
+    } else if (g->max_bpop <= 3 && g->max_bpush <= 4 && g->max_prods <= 4 && g->max_spush <= 4) {
+      kern[0] = seg_resolve<3, 4, 4, 4, false>;
+      kern[1] = seg_resolve<3, 4, 4, 4, true>;
+    } else {
+      kern[0] = seg_resolve<4, 4, 8, 24, false>;
+      kern[1] = seg_resolve<4, 4, 8, 24, true>;
+    }
     if (!c.resolve_lds_set || c.resolve_lds_set < lds_b) {
-      HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)lds_b));
+      for (auto k : kern)
+        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds_b));
       c.resolve_lds_set = lds_b;
     }
-    HIP_TRY(tm.begin(st, "seg_resolve"));
-    hipLaunchKernelGGL(kern, dim3((uint32_t)ntiles), dim3(RWG), lds_b, st, rs);
-    HIP_TRY(tm.end(st));
+    if (!caps) HIP_TRY(tm.begin(st, "seg_resolve"));
+    hipLaunchKernelGGL(kern[caps ? 1 : 0], dim3(g_res), dim3(RWG), lds_b, st, rs);
+    if (!caps) HIP_TRY(tm.end(st));
     Far fr{s, info, sb, tb, ts, ob_list, ub_list, os_list, us_list, c.tr_parent.as<int64_t>(),
            c.tr_tokpar.as<int64_t>(), tree ? 1 : 0};
-    HIP_TRY(tm.begin(st, "far_resolve"));
-    hipLaunchKernelGGL(far_resolve, dim3((uint32_t)((ntiles + 3) / 4)), dim3(256), 0, st, fr);
-    HIP_TRY(tm.end(st));
+    if (!caps) HIP_TRY(tm.begin(st, "far_resolve"));
+    hipLaunchKernelGGL(far_resolve, dim3(g_far), dim3(256), 0, st, fr);
+    if (!caps) HIP_TRY(tm.end(st));
   }
   hipLaunchKernelGGL(seg_finish, dim3((uint32_t)((count + 255) / 256)), dim3(256), 0, st, s, info, sb, sout,
                      tree ? 1 : 0, res);
   HIP_TRY(hipGetLastError());
-  if (c.timing) HIP_TRY(hipEventRecord(c.ev[3], st));
-  HIP_TRY(hipMemcpyAsync(c.h_pres.ptr, res, count * sizeof(ParseResult), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipStreamSynchronize(st));
-  HIP_TRY(tm.collect());
-  if (c.timing) {
+  if (timed) HIP_TRY(hipEventRecord(c.ev[3], st));
+  c.cp_parse_down = Context::Copy{c.h_pres.ptr, res, count * sizeof(ParseResult), hipMemcpyDeviceToHost};
+  if (!c.capturing) HIP_TRY(hipMemcpyAsync(c.h_pres.ptr, res, count * sizeof(ParseResult), hipMemcpyDeviceToHost, st));
+  if (!caps) {
+    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(tm.collect());
+  }
+  if (timed) {
     float ms_ = 0;
     HIP_TRY(hipEventElapsedTime(&ms_, c.ev[2], c.ev[3]));
     c.parse_ms += ms_;

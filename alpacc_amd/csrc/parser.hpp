@@ -25,8 +25,16 @@ struct ParseResult {
 // compact trees land in c.tr_* (parents relative to their input), else the
 // flat derivations in c.ps_d.  Returns 0 or an error; per-input results in
 // c.pb_res[count].
+//
+// With caps (caps->bytes bounding the tokens of all strings), nothing is
+// read back before the sequence ends: the tiles are planned on the device
+// from d_tok_off, the device flags d_skip and *d_live (strings at or past it
+// are padding), h_tok_off and h_skip are not read, and c.pb_res is written
+// when the stream reaches its copy.
 int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_tok_off, const uint64_t* h_tok_off,
-                          const uint32_t* h_skip, uint64_t count, bool tree, hipStream_t st);
+                          const uint32_t* h_skip, uint64_t count, bool tree, hipStream_t st,
+                          const BatchCaps* caps = nullptr, const uint32_t* d_skip = nullptr,
+                          const uint32_t* d_live = nullptr);
 
 // One string of n ids: returns 1 (valid) / 0 (rejected) / < 0 (error) and
 // the derivation length in *num_prods (trees: nodes = *num_prods - n).

@@ -56,6 +56,7 @@ int DevBuf::reserve(size_t want) {
     return fail(ALPACC_ERR_NOMEM, "hipMalloc(%zu): %s", sz, hipGetErrorString(e));
   }
   cap = sz;
+  ++gen;
   return 0;
 }
 
@@ -85,6 +86,7 @@ int PinnedBuf::reserve(size_t want) {
     return fail(ALPACC_ERR_NOMEM, "hipHostMalloc(%zu): %s", sz, hipGetErrorString(e));
   }
   cap = sz;
+  ++gen;
   return 0;
 }
 
@@ -182,6 +184,7 @@ void alpacc_hip_close(alpacc_hip_ctx* h) {
   if (!h) return;
   Context& c = h->c;
   (void)hipSetDevice(c.device);
+  if (c.graph) (void)hipGraphExecDestroy(c.graph);
   void* ptrs[] = {c.d_class, c.d_step, c.d_compose, c.d_endo_init, c.d_delta_lex, c.d_delta_both, c.d_token_lex, c.d_token_both, c.d_window,
                   c.d_spans, c.d_profile, c.d_pair_rec, c.d_prod_info, c.d_stacks, c.d_productions, c.d_arity, c.d_isterm};
   for (void* p : ptrs)

@@ -13,10 +13,12 @@ namespace alpacc {
 
 struct ParseResult;
 
-// Grow-only device allocation: steady-state calls allocate nothing.
+// Grow-only device allocation: steady-state calls allocate nothing.  gen
+// counts the allocations (a captured launch sequence holds the pointer).
 struct DevBuf {
   void* ptr = nullptr;
   size_t cap = 0;
+  uint64_t gen = 0;
   int reserve(size_t bytes);
   void release();
   template <typename T>
@@ -29,6 +31,7 @@ struct DevBuf {
 struct PinnedBuf {
   void* ptr = nullptr;
   size_t cap = 0;
+  uint64_t gen = 0;
   int reserve(size_t bytes);
   void release();
   template <typename T>
@@ -38,20 +41,33 @@ struct PinnedBuf {
 };
 
 // Byte layout of a batch's metadata, the same on the device and in the
-// pinned host mirror: uploaded [seg_off u64 | chunk_first u32], read back
-// [seg_tok u64 | seg_flags u32], then (device only) tile_seg u32[2 * ntiles].
+// pinned host mirror: uploaded [seg_off u64 | chunk_first u32 | tiles u32 |
+// inputs that are not padding u32], read back [seg_tok u64 | seg_flags
+// u32], then (device only) tile_seg u32[2 * ntiles].
 struct BatchLayout {
-  size_t off_seg, off_cf, off_tok, off_flags, off_tseg, up_bytes, down_bytes, total;
+  size_t off_seg, off_cf, off_nt, off_live, off_tok, off_flags, off_tseg, up_bytes, down_bytes, total;
   BatchLayout(uint64_t count, uint64_t ntiles) {
     off_seg = 0;
     off_cf = (count + 1) * 8;
-    up_bytes = off_cf + (count + 1) * 4;
+    off_nt = off_cf + (count + 1) * 4;
+    off_live = off_nt + 4;
+    up_bytes = off_live + 4;
     off_tok = (up_bytes + 7) & ~size_t(7);
     off_flags = off_tok + (count + 1) * 8;
     down_bytes = off_flags + count * 4 - off_tok;
     off_tseg = (off_flags + count * 4 + 255) & ~size_t(255);
     total = off_tseg + ntiles * 8 + 64;
   }
+};
+
+// The capacity a launch sequence is sized for, so that it can be captured
+// once in a graph and replayed for every batch within it: `count` inputs
+// of `bytes` bytes in total, the first `live` of them real and the rest
+// empty padding.  Grids and workspaces follow the capacity, the kernels the
+// actual sizes (in device memory); nothing is read back before the
+// sequence ends.
+struct BatchCaps {
+  uint64_t count, bytes, live;
 };
 
 struct Context {
@@ -93,6 +109,7 @@ struct Context {
   const uint64_t* lb_tok = nullptr;    // [count + 1] first token of each input; [count] = all
   const uint32_t* d_lb_flags = nullptr;
   const uint64_t* d_lb_tok = nullptr;
+  const uint32_t* d_lb_live = nullptr;  // [1] inputs that are not padding
   // parser batches
   DevBuf pmeta;       // offsets of single-string parses
   PinnedBuf h_pmeta;  // tile_first of the parser tiles
@@ -111,9 +128,47 @@ struct Context {
   uint64_t lex_launches = 0, parse_launches = 0;
   KernelTimer timer;
 
+  // the combined lex -> parse -> tree sequence captured for a batch
+  // capacity (graph_caps), replayed for batches within it that read the
+  // same inputs on the same stream while its buffers stay where they were
+  bool use_graphs = false;
+  // while capturing, the batch functions enqueue device work only and note
+  // here the copies between host and device that the graph's runner makes
+  // around each launch
+  struct Copy {
+    void* dst = nullptr;
+    const void* src = nullptr;
+    size_t bytes = 0;
+    hipMemcpyKind kind = hipMemcpyHostToDevice;
+  };
+  bool capturing = false;
+  Copy cp_meta_up, cp_lex_down, cp_parse_down;
+  // what a replay restores (calls between replays reuse the fields)
+  struct GraphState {
+    Copy up, lex_down, parse_down;
+    uint64_t lb_count = 0, pb_count = 0;
+    const uint32_t* lb_flags = nullptr;
+    const uint64_t* lb_tok = nullptr;
+    const uint32_t* d_lb_flags = nullptr;
+    const uint64_t* d_lb_tok = nullptr;
+    const uint32_t* d_lb_live = nullptr;
+    const struct ParseResult* pb_res = nullptr;
+  } graph_state;
+  hipGraphExec_t graph = nullptr;
+  BatchCaps graph_caps{};
+  const uint8_t* graph_in = nullptr;
+  hipStream_t graph_stream = nullptr;
+  uint64_t graph_gen = 0;
+  uint64_t graph_captures = 0, graph_replays = 0;
+
   std::vector<DevBuf*> all_buffers() {
     return {&small, &lx_term, &lx_start, &lx_len, &lx_status, &in_copy, &ps_a,      &ps_b,      &ps_c,   &ps_d,
             &ps_e,  &ps_f,    &ps_g,     &ps_h,   &ps_i,      &tr_prod, &tr_parent, &tr_tokpar, &lx_meta, &pmeta};
+  }
+  // allocations of the buffers that the captured sequence uses
+  uint64_t graph_buffers_gen() const {
+    return small.gen + lx_term.gen + lx_start.gen + lx_len.gen + lx_status.gen + ps_a.gen + ps_b.gen + ps_c.gen +
+           tr_prod.gen + tr_parent.gen + tr_tokpar.gen + lx_meta.gen + h_meta.gen + h_pmeta.gen + h_pres.gen;
   }
 };
 
@@ -131,9 +186,12 @@ const char* const* grammar_names();
 // Lexer over `count` inputs packed in device memory at d_in, input i being
 // bytes [h_off[i], h_off[i + 1]) (host array).  Returns 0 or an error;
 // per-input verdicts and token offsets are in c.lb_* (host) and c.d_lb_*
-// (device), the tokens in c.lx_* with starts relative to their input.
+// (device), the tokens in c.lx_* with starts relative to their input.  With
+// caps, the host copies are written when the stream reaches them.  Without
+// `enqueue` only the host metadata of the batch is written (the copy that
+// a captured sequence uploads).
 int lex_batch_on_device(Context& c, int mode, const uint8_t* d_in, const uint64_t* h_off, uint64_t count,
-                        hipStream_t st);
+                        hipStream_t st, const BatchCaps* caps = nullptr, bool enqueue = true);
 
 // Wire records of a batch of host payloads, concatenated into one malloc'd
 // buffer; rec_off (if not null) receives the start of every input's record.

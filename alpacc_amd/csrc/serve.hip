@@ -12,6 +12,7 @@
 // alpacc_hip_run_case returns for the request alone.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstdlib>
@@ -73,6 +74,7 @@ struct alpacc_hip_server {
   std::unordered_map<uint64_t, std::unique_ptr<Req>> tickets;
   uint64_t next_ticket = 1;
   uint64_t n_batches = 0, n_reqs = 0, biggest = 0;
+  std::atomic<bool> graphs{false};  // combined batches through each lane's captured graph
 };
 
 namespace {
@@ -125,6 +127,7 @@ void worker(alpacc_hip_server* s, uint32_t g, alpacc_hip_ctx* ctx) {
     }
     uint8_t* out = nullptr;
     size_t out_len = 0;
+    ctx->c.use_graphs = s->graphs.load();
     const int rc = alpacc::batch_records(ctx->c, mode, packed.data(), off.data(), batch.size(), &out, &out_len, &rec_off);
     const std::string err = rc < 0 ? alpacc_hip_last_error() : "";
     std::shared_ptr<uint8_t> shared(out, free);
@@ -299,6 +302,12 @@ int alpacc_hip_server_replay(alpacc_hip_server* s, uint64_t count, const int32_t
   }
   *records = o;
   *records_len = total;
+  return 0;
+}
+
+int alpacc_hip_server_set_graphs(alpacc_hip_server* s, int on) {
+  if (!s) return alpacc::fail(ALPACC_ERR_ARG, "null argument");
+  s->graphs.store(on != 0);
   return 0;
 }
 

@@ -98,6 +98,9 @@ def load_library(path: str = LIB_PATH):
                                            C.POINTER(u64), C.POINTER(u64), C.POINTER(u64), C.POINTER(vp),
                                            C.POINTER(u64)]),
         "alpacc_hip_server_stats": (i32, [vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64)]),
+        "alpacc_hip_set_graphs": (i32, [vp, i32]),
+        "alpacc_hip_graph_stats": (i32, [vp, C.POINTER(u64), C.POINTER(u64)]),
+        "alpacc_hip_server_set_graphs": (i32, [vp, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -254,6 +257,17 @@ class Backend:
             token_parents=_take(out.token_parents, T * 8, "<i8"),
         )
 
+    # -- graphs for combined-mode batches -----------------------------------------
+    def set_graphs(self, on: bool = True):
+        """Run combined-mode batches through a captured HIP graph."""
+        _check(self.lib.alpacc_hip_set_graphs(self.h, int(on)) >= 0 and 1 or -1)
+
+    def graph_stats(self):
+        """(captures, replays) of the batch graph so far."""
+        c, r = C.c_uint64(), C.c_uint64()
+        _check(self.lib.alpacc_hip_graph_stats(self.h, C.byref(c), C.byref(r)) >= 0 and 1 or -1)
+        return c.value, r.value
+
     # -- stage timing (HIP events on the launch stream) ------------------------------
     def set_timing(self, on: bool = True):
         _check(self.lib.alpacc_hip_set_timing(self.h, int(on)) >= 0 and 1 or -1)
@@ -336,6 +350,10 @@ class Server:
             C.c_char_p(packed), offs.ctypes.data_as(u64p), arr.ctypes.data_as(u64p), lat.ctypes.data_as(u64p),
             C.byref(rec), C.byref(ln)))
         return _take(rec.value, ln.value, np.uint8).tobytes(), lat
+
+    def set_graphs(self, on: bool = True):
+        """Combined-mode batches of every lane through its captured graph."""
+        _check(self.lib.alpacc_hip_server_set_graphs(self.h, int(on)) >= 0 and 1 or -1)
 
     def stats(self):
         b, r, m = C.c_uint64(), C.c_uint64(), C.c_uint64()

@@ -168,6 +168,18 @@ def batch_workload(be, count: int, steps: int, seed: int = 3):
         be.lex_parse_batch_device(base, offs, stream=stream)
     torch.cuda.synchronize()
     batched_s = (time.perf_counter() - t0) / steps
+    # the same batch through the captured graph (the first call captures it)
+    be.set_graphs(True)
+    bg = be.lex_parse_batch_device(base, offs, stream=stream)
+    graph_valid = int(np.ctypeslib.as_array((C_u8 * count).from_address(bg.valid)).sum())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        be.lex_parse_batch_device(base, offs, stream=stream)
+    torch.cuda.synchronize()
+    graph_s = (time.perf_counter() - t0) / steps
+    captures, replays = be.graph_stats()
+    be.set_graphs(False)
     # plain: one call per input, in input order
     for i in range(min(count, 200)):  # warmup of the per-input path
         be.lex_parse_device(base + int(offs[i]), int(offs[i + 1] - offs[i]), stream=stream)
@@ -185,6 +197,8 @@ def batch_workload(be, count: int, steps: int, seed: int = 3):
         "workload": f"{count} json inputs, lognormal(ln 4096, 1.5) bytes clipped to [64, 4 MiB], seed {seed}",
         "count": count, "bytes": total, "valid": valid, "plain_valid": plain_valid,
         "batched_ms": round(batched_s * 1e3, 3), "batched_GBps": round(total / batched_s / 1e9, 3),
+        "graph_ms": round(graph_s * 1e3, 3), "graph_GBps": round(total / graph_s / 1e9, 3),
+        "graph_valid": graph_valid, "graph_captures": captures, "graph_replays": replays,
         "plain_ms": round(plain_s * 1e3, 3), "plain_GBps": round(total / plain_s / 1e9, 3),
         "speedup_vs_plain": round(plain_s / batched_s, 2),
     }
@@ -213,8 +227,9 @@ def serve_trace(count: int, rate: float, seed: int, max_bytes: int = 16 << 20):
 
 def serve_workload(rates, seconds: float, seed: int = 4, plain: bool = True):
     """Serving: the trace replayed open-loop against the continuous-batching
-    server (two lanes per grammar) and against the plain server (one request
-    per launch sequence, one lane), both with json and arithmetic resident."""
+    server (two lanes per grammar), the same with each lane's combined
+    batches through its captured graph, and the plain server (one request
+    per launch sequence, one lane), all with json and arithmetic resident."""
     import numpy as np
     from alpacc_amd import hip
     names = ["json", "arithmetic"]
@@ -227,8 +242,10 @@ def serve_workload(rates, seconds: float, seed: int = 4, plain: bool = True):
         total = sum(len(x) for x in payloads)
         modes = ["both"] * count
         row = {"rate_req_s": rate, "requests": count, "bytes": total}
-        for label, lanes, max_batch in (("served", 2, 0), ("plain", 1, 1)) if plain else (("served", 2, 0),):
+        configs = [("served", 2, 0, False), ("served_graph", 2, 0, True)] + ([("plain", 1, 1, False)] if plain else [])
+        for label, lanes, max_batch, graphs in configs:
             srv = hip.Server(names, lanes=lanes, hbm_budget=64 << 30, max_batch=max_batch)
+            srv.set_graphs(graphs)
             # warmup: one untimed replay of the trace, so every lane's
             # workspaces have grown to its batches before the timed replay
             srv.replay(gidx, modes, payloads, arrival)
@@ -377,8 +394,9 @@ def main(argv=None):
     if a.batch_count:
         log(f"batching workload: {a.batch_count} inputs")
         batch_items, batched = batch_workload(be, a.batch_count, a.batch_steps)
-        log(f"batched {batched['batched_GBps']} GB/s, plain {batched['plain_GBps']} GB/s, "
-            f"x{batched['speedup_vs_plain']}")
+        log(f"batched {batched['batched_GBps']} GB/s (graph {batched['graph_GBps']} GB/s, "
+            f"{batched['graph_captures']} capture, {batched['graph_replays']} replays), "
+            f"plain {batched['plain_GBps']} GB/s, x{batched['speedup_vs_plain']}")
     served = None
     if a.serve_rates:
         rates = [float(x) for x in a.serve_rates.split(",") if x]

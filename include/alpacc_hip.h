@@ -165,6 +165,22 @@ typedef struct {
 int alpacc_hip_lex_parse_batch_device(alpacc_hip_ctx *ctx, const uint8_t *d_packed, const uint64_t *offsets,
                                       uint64_t count, alpacc_hip_batch *out, void *stream);
 
+/* Graphs for combined-mode batches (no reference counterpart: the
+ * reference launches every kernel from the host, backends/cuda/cli.cu:
+ * 236-274).  While on, alpacc_hip_run_batch (combined mode) and
+ * alpacc_hip_lex_parse_batch_device run the lex -> parse -> tree sequence
+ * as one HIP graph: it is sized for a capacity (inputs and bytes rounded up
+ * to powers of two, never shrinking), planned on the device (no host
+ * synchronisation between lexer and parser) and captured once; later
+ * batches within the capacity that read the same input pointer on the same
+ * stream replay it.  A batch beyond it, a moved workspace or another input
+ * pointer captures again (the capture call first runs the sequence
+ * eagerly).  Workspaces are sized for the capacity's worst case.  Per-kernel
+ * timing does not cover graph runs.  graph_stats counts captures and
+ * replays. */
+int alpacc_hip_set_graphs(alpacc_hip_ctx *ctx, int on);
+int alpacc_hip_graph_stats(const alpacc_hip_ctx *ctx, uint64_t *captures, uint64_t *replays);
+
 /* ---- serving ---------------------------------------------------------------
  * Continuous batching over resident grammars (no reference counterpart: the
  * reference's server answers one request at a time, backends/c/cli.c:
@@ -194,6 +210,10 @@ int alpacc_hip_server_wait(alpacc_hip_server *srv, uint64_t ticket, uint8_t **re
 int alpacc_hip_server_replay(alpacc_hip_server *srv, uint64_t count, const int32_t *grammar, const int32_t *mode,
                              const uint8_t *packed, const uint64_t *offsets, const uint64_t *arrival_ns,
                              uint64_t *latency_ns, uint8_t **records, uint64_t *records_len);
+
+/* Combined-mode batches of every lane through its captured graph
+ * (alpacc_hip_set_graphs). */
+int alpacc_hip_server_set_graphs(alpacc_hip_server *srv, int on);
 
 /* Batches run, requests answered and the largest batch so far. */
 int alpacc_hip_server_stats(alpacc_hip_server *srv, uint64_t *batches, uint64_t *requests, uint64_t *max_batch);

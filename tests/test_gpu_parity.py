@@ -347,6 +347,62 @@ def test_batch_token_closing_at_tile_start(backends, mode):
         assert d is None, f"{grammar}/{mode}: differs at byte {d}"
 
 
+def test_batch_graph_replays():
+    """Combined batches through the captured graph: the first (largest)
+    batch captures it, smaller ones replay it with eager calls on the same
+    context between the replays, and every record of every replay equals
+    the oracle's."""
+    from alpacc_amd import hip
+    for grammar in ("json", "arithmetic"):
+        rng = random.Random(zlib.crc32(f"graph {grammar}".encode()))
+        be = hip.Backend(grammar)
+        try:
+            be.set_graphs(True)
+            for k, count in enumerate((60, 20, 45, 7, 60, 1)):
+                payloads = batch_payloads(grammar, "both", rng, count)
+                if k == 0:
+                    payloads.append(datagen.generate(grammar, 300000, seed=5, conforming=True).tobytes())
+                print(f"graph batch {grammar} #{k} count={len(payloads)} bytes={sum(map(len, payloads))}", flush=True)
+                got = be.run_batch("both", payloads)
+                exp = b"".join(pyoracle.record(grammar, "both", x) for x in payloads)
+                d = first_diff(got, exp)
+                assert d is None, f"{grammar} graph batch {k}: differs at byte {d}"
+                x = payloads[-1][:5000]
+                assert be.run_case("both", x) == pyoracle.record(grammar, "both", x)
+            captures, replays = be.graph_stats()
+            print(f"{grammar}: {captures} captures, {replays} replays", flush=True)
+            assert captures >= 1 and replays >= 3
+        finally:
+            be.close()
+
+
+def test_batch_device_graph_replays():
+    """lex_parse_batch_device through the graph: batches read from one device
+    buffer replay the captured sequence, and every replay's device arrays
+    match the oracle."""
+    import torch
+    from alpacc_amd import hip
+    rng = random.Random(21)
+    be = hip.Backend("json")
+    try:
+        be.set_graphs(True)
+        dev = torch.zeros(4 << 20, dtype=torch.uint8, device="cuda")
+        for k, n in enumerate((40, 12, 33, 12, 40)):
+            payloads = batch_payloads("json", "both", rng, n)
+            if k == 0:  # the capacity covers the later batches
+                payloads.append(datagen.generate("json", 1500000, seed=6, conforming=True).tobytes())
+            buf = b"".join(payloads)
+            assert len(buf) <= dev.numel()
+            dev[:len(buf)] = torch.frombuffer(bytearray(buf), dtype=torch.uint8).to("cuda")
+            torch.cuda.synchronize()
+            print(f"device graph batch #{k} count={n} bytes={len(buf)}", flush=True)
+            check_device_batch(be, dev, payloads)
+        captures, replays = be.graph_stats()
+        assert captures >= 1 and replays >= 2, (captures, replays)
+    finally:
+        be.close()
+
+
 def test_batch_alone_and_in_mix(backends):
     """Each result is the same alone and in a mix: every input of a batch
     against run_case on that input."""
@@ -360,13 +416,19 @@ def test_batch_alone_and_in_mix(backends):
 
 def test_batch_device_arrays(backends):
     import torch
-    from alpacc_amd import wire
     rng = random.Random(3)
     payloads = batch_payloads("json", "both", rng, 25)
+    dev = torch.from_numpy(np.frombuffer(b"".join(payloads) + b"\0", dtype=np.uint8).copy()).to("cuda")
+    check_device_batch(backends["json"], dev, payloads)
+
+
+def check_device_batch(be, dev, payloads):
+    """lex_parse_batch_device over payloads packed at the start of the device
+    tensor dev: every input's device arrays against the oracle's record."""
+    import torch
+    from alpacc_amd import wire
     offs = np.zeros(len(payloads) + 1, dtype=np.uint64)
     offs[1:] = np.cumsum([len(x) for x in payloads])
-    dev = torch.from_numpy(np.frombuffer(b"".join(payloads) + b"\0", dtype=np.uint8).copy()).to("cuda")
-    be = backends["json"]
     b = be.lex_parse_batch_device(dev.data_ptr(), offs)
     torch.cuda.synchronize()
     lay = wire.Layout("both", 1, 8, 2, 1)
