@@ -501,23 +501,28 @@ def serve_trace(rng, count, rate, max_bytes=1 << 20):
     return names, gidx, modes, payloads, arrival
 
 
-def test_server_replay_matches_alone_and_oracle(backends):
+@pytest.mark.parametrize("graphs", [False, True])
+def test_server_replay_matches_alone_and_oracle(backends, graphs):
     """Continuous batching: every answer of a mixed open-loop trace equals
-    the oracle's and the answer to the request alone."""
+    the oracle's and the answer to the request alone (also with each lane's
+    combined batches replaying its captured graph, twice over the trace)."""
     from alpacc_amd import hip
     rng = random.Random(11)
     names, gidx, modes, payloads, arrival = serve_trace(rng, 400, 5000.0)
     srv = hip.Server(names, lanes=2)
     try:
+        srv.set_graphs(graphs)
         recs, lat = srv.replay(gidx, modes, payloads, arrival)
+        recs2, _ = srv.replay(gidx, modes, payloads, arrival) if graphs else (recs, None)
         stats = srv.stats()
     finally:
         srv.close()
     exp = b"".join(pyoracle.record(names[g], m, x) for g, m, x in zip(gidx, modes, payloads))
-    assert recs == exp
+    assert first_diff(recs, exp) is None
+    assert first_diff(recs2, exp) is None
     alone = b"".join(backends[names[g]].run_case(m, x) for g, m, x in zip(gidx, modes, payloads))
-    assert recs == alone
-    assert stats["requests"] == len(payloads)
+    assert first_diff(recs, alone) is None
+    assert stats["requests"] == len(payloads) * (2 if graphs else 1)
     print("server stats", stats, "p50 ms", float(np.percentile(lat, 50)) / 1e6, flush=True)
 
 
