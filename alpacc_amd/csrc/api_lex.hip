@@ -33,18 +33,19 @@ int lex_batch_on_device(Context& c, int mode, const uint8_t* d_in, const uint64_
                 (unsigned long long)h_off[count], (unsigned long long)LEX_MAX_INPUT);
   // lane prefixes keep a flag in bit 15 of their endomorphism ids
   if (g->nendo > 0x8000) return fail(ALPACC_ERR_LIMIT, "%d lexer endomorphisms exceed 32768", g->nendo);
-  // every input owns whole chunks of LEX_IPT bytes, at least one
+  // every input owns whole chunks of ipt bytes, at least one
+  const uint64_t ipt = (uint64_t)c.lex_ipt;
   uint64_t nchunks = 0;
   for (uint64_t i = 0; i < count; ++i) {
     if (h_off[i + 1] < h_off[i]) return fail(ALPACC_ERR_ARG, "offsets decrease at input %llu", (unsigned long long)i);
     const uint64_t n = h_off[i + 1] - h_off[i];
-    nchunks += n ? (n + LEX_IPT - 1) / LEX_IPT : 1;
+    nchunks += n ? (n + ipt - 1) / ipt : 1;
   }
   if (nchunks > 0x7FFFFFFFull) return fail(ALPACC_ERR_LIMIT, "too many lexer chunks");
   const uint64_t nt_actual = (nchunks + LEX_WG - 1) / LEX_WG;
-  // at capacity: at most bytes / LEX_IPT full chunks plus one partial (or
+  // at capacity: at most bytes / ipt full chunks plus one partial (or
   // empty) chunk per input
-  const uint64_t ntiles = caps ? (caps->bytes / LEX_IPT + caps->count + LEX_WG - 1) / LEX_WG : nt_actual;
+  const uint64_t ntiles = caps ? (caps->bytes / ipt + caps->count + LEX_WG - 1) / LEX_WG : nt_actual;
   const BatchLayout L(count, ntiles);
   int r;
   if ((r = c.lx_meta.reserve(L.total)) || (r = c.h_meta.reserve(L.total))) return r;
@@ -57,7 +58,7 @@ int lex_batch_on_device(Context& c, int mode, const uint8_t* d_in, const uint64_
     h_seg[i] = h_off[i];
     h_cf[i] = k;
     const uint64_t n = h_off[i + 1] - h_off[i];
-    k += n ? (uint32_t)((n + LEX_IPT - 1) / LEX_IPT) : 1u;
+    k += n ? (uint32_t)((n + ipt - 1) / ipt) : 1u;
   }
   h_seg[count] = h_off[count];
   h_cf[count] = k;
@@ -111,7 +112,9 @@ int lex_batch_on_device(Context& c, int mode, const uint8_t* d_in, const uint64_
   p.tile_blk = reinterpret_cast<uint32_t*>(sb + tb_off);
   p.lane_prefix = reinterpret_cast<uint16_t*>(sb + lp_off);
   p.ntiles = (uint32_t)ntiles;
-  if (!c.lex_grid) HIP_TRY(lex_grid_size(p, &c.lex_grid, &c.lex_grid2));
+  p.ipt = c.lex_ipt;
+  p.lb_win = c.lex_lb;
+  if (!c.lex_grid) HIP_TRY(lex_grid_size(p, c.lex_wpc, &c.lex_grid, &c.lex_grid2));
   c.cp_meta_up = Context::Copy{dm, hm, L.up_bytes, hipMemcpyHostToDevice};
   if (!c.capturing) HIP_TRY(hipMemcpyAsync(dm, hm, L.up_bytes, hipMemcpyHostToDevice, st));
   if ((r = zero_words(c.small.ptr, 16, st)) || (r = zero_words(c.lx_status.ptr, st_bytes, st)) ||
@@ -244,6 +247,76 @@ int alpacc_hip_lex_device(alpacc_hip_ctx* h, int mode, const uint8_t* d_str, uin
   d_out->starts = c.lx_start.as<int64_t>();
   d_out->lengths = c.lx_len.ptr;
   return r;
+}
+
+int alpacc_hip_set_lex_shape(alpacc_hip_ctx* h, const alpacc_hip_lex_shape* shape) {
+  if (!h || !shape) return fail(ALPACC_ERR_ARG, "null argument");
+  if (!lex_shape_ok(shape->items_per_thread, shape->lookback_window) || shape->workgroups_per_cu < 0)
+    return fail(ALPACC_ERR_ARG, "lexer shape %d bytes per lane, %d workgroups per CU, window %d is not one of the kernels'",
+                shape->items_per_thread, shape->workgroups_per_cu, shape->lookback_window);
+  Context& c = h->c;
+  HIP_TRY(hipSetDevice(c.device));
+  c.lex_ipt = shape->items_per_thread;
+  c.lex_wpc = shape->workgroups_per_cu;
+  c.lex_lb = shape->lookback_window;
+  c.lex_grid = c.lex_grid2 = 0;
+  if (c.graph) {  // captured with the old kernels
+    (void)hipGraphExecDestroy(c.graph);
+    c.graph = nullptr;
+  }
+  return 0;
+}
+
+int alpacc_hip_get_lex_shape(const alpacc_hip_ctx* h, alpacc_hip_lex_shape* shape) {
+  if (!h || !shape) return fail(ALPACC_ERR_ARG, "null argument");
+  shape->items_per_thread = h->c.lex_ipt;
+  shape->workgroups_per_cu = h->c.lex_wpc;
+  shape->lookback_window = h->c.lex_lb;
+  shape->ms = 0;
+  return 0;
+}
+
+int alpacc_hip_autotune_lexer(alpacc_hip_ctx* h, int mode, const uint8_t* d_sample, uint64_t n, int reps,
+                              alpacc_hip_lex_shape* best) {
+  if (!h || !best || !d_sample || n == 0 || reps <= 0) return fail(ALPACC_ERR_ARG, "null argument or empty sample");
+  if (mode != ALPACC_MODE_LEXER && mode != ALPACC_MODE_BOTH) return fail(ALPACC_ERR_ARG, "bad lexer mode %d", mode);
+  Context& c = h->c;
+  HIP_TRY(hipSetDevice(c.device));
+  for (hipEvent_t& ev : c.ev)
+    if (!ev) HIP_TRY(hipEventCreate(&ev));
+  const bool timing = c.timing, ktiming = c.timer.on;
+  c.timing = true;  // the lexer's stage events time each run
+  c.timer.on = false;
+  static const int ipts[] = {64, 128}, wpcs[] = {2, 3, 4, 0}, wins[] = {16, 32, 64};
+  alpacc_hip_lex_shape top{c.lex_ipt, c.lex_wpc, c.lex_lb, 0};
+  double top_ms = -1;
+  int r = 0;
+  for (int ipt : ipts)
+    for (int wpc : wpcs)
+      for (int win : wins) {
+        const alpacc_hip_lex_shape sh{ipt, wpc, win, 0};
+        if ((r = alpacc_hip_set_lex_shape(h, &sh)) < 0) break;
+        uint64_t count = 0;
+        if ((r = lex_on_device(c, mode, d_sample, n, c.stream, &count)) < 0) break;  // warm
+        c.lex_ms = 0;
+        c.lex_launches = 0;
+        for (int i = 0; i < reps && r >= 0; ++i) r = lex_on_device(c, mode, d_sample, n, c.stream, &count);
+        if (r < 0) break;
+        const double ms = c.lex_ms / (double)reps;
+        if (top_ms < 0 || ms < top_ms) {
+          top_ms = ms;
+          top = sh;
+        }
+      }
+  c.timing = timing;
+  c.timer.on = ktiming;
+  c.lex_ms = 0;
+  c.lex_launches = 0;
+  if (r < 0) return r;
+  top.ms = top_ms;
+  if ((r = alpacc_hip_set_lex_shape(h, &top)) < 0) return r;
+  *best = top;
+  return 0;
 }
 
 int alpacc_hip_lex_string(alpacc_hip_ctx* h, int mode, const uint8_t* str, uint64_t n, alpacc_hip_lexemes* out) {

@@ -48,7 +48,6 @@ namespace {
 typedef uint32_t u32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
 
 constexpr int WG = LEX_WG;
-constexpr int IPT = LEX_IPT;
 constexpr int NW = WG / WAVE;
 
 // A tile: the inputs of its first and last chunks and the first of them.
@@ -60,6 +59,7 @@ struct TileSpan {
   __device__ bool multi() const { return s_hi != s_lo; }
 };
 
+template <int IPT>
 __device__ __forceinline__ TileSpan tile_span(const LexParams& p, uint32_t tile) {
   TileSpan t;
   t.s_lo = p.tile_seg[2 * tile];
@@ -102,6 +102,7 @@ struct LaneSpan {
   bool first;  // the input's first chunk
 };
 
+template <int IPT>
 __device__ __forceinline__ LaneSpan lane_span(const LexParams& p, const TileSpan& ts, uint32_t tile,
                                               const uint16_t* s_ichunk) {
   LaneSpan l;
@@ -130,6 +131,7 @@ __device__ __forceinline__ LaneSpan lane_span(const LexParams& p, const TileSpan
 // first byte at byte `shift`, a[IPT / 4] is read only when the lane is
 // misaligned (it holds the lane's last byte), so no byte outside the dwords
 // of the lane's own bytes is touched.
+template <int IPT>
 __device__ __forceinline__ void load_lane(const LexParams& p, uint64_t base, uint32_t shift, uint32_t* a) {
   const u32x4a4* src = reinterpret_cast<const u32x4a4*>(p.in + base - shift);
 #pragma unroll
@@ -146,6 +148,7 @@ __device__ __forceinline__ void load_lane(const LexParams& p, uint64_t base, uin
 // The dwords holding the lane's first `valid` bytes (fewer than IPT: the
 // input ends inside the lane), zero beyond them: only dwords holding one of
 // the input's bytes are read.
+template <int IPT>
 __device__ __forceinline__ void load_lane_part(const LexParams& p, uint64_t base, uint32_t shift, uint32_t valid,
                                                uint32_t* a) {
   const uint32_t* src = reinterpret_cast<const uint32_t*>(p.in + base - shift);
@@ -179,6 +182,7 @@ __device__ __forceinline__ void seg_comp_scan(const uint16_t* __restrict__ compo
 
 // ------------------------------------------------------------------- L1
 
+template <int IPT>
 __global__ __launch_bounds__(WG) void lex_fold(LexParams p) {
   extern __shared__ __align__(16) uint8_t lds[];
   uint8_t* s_class = lds;
@@ -195,12 +199,12 @@ __global__ __launch_bounds__(WG) void lex_fold(LexParams p) {
 
   const uint32_t nt = *p.nt_dev;
   for (uint32_t tile = blockIdx.x; tile < nt; tile += gridDim.x) {
-    const TileSpan ts = tile_span(p, tile);
+    const TileSpan ts = tile_span<IPT>(p, tile);
     if (ts.multi()) {
       tile_inputs(p, ts, tile, s_ichunk);
       __syncthreads();
     }
-    const LaneSpan ls = lane_span(p, ts, tile, s_ichunk);
+    const LaneSpan ls = lane_span<IPT>(p, ts, tile, s_ichunk);
     const uint32_t valid = ls.valid;
     uint32_t e = 0;  // identity
     // the lane's bytes fully unrolled from registers (indexed with
@@ -208,7 +212,7 @@ __global__ __launch_bounds__(WG) void lex_fold(LexParams p) {
     // past them keep the state
     if (valid == IPT) {
       uint32_t a[IPT / 4 + 1];
-      load_lane(p, ls.base, ls.shift, a);
+      load_lane<IPT>(p, ls.base, ls.shift, a);
 #pragma unroll
       for (int k = 0; k < IPT / 4; ++k) {
         const uint32_t wk = lane_word(a, k, ls.shift, e);
@@ -217,7 +221,7 @@ __global__ __launch_bounds__(WG) void lex_fold(LexParams p) {
       }
     } else if (valid > 0) {
       uint32_t a[IPT / 4 + 1];
-      load_lane_part(p, ls.base, ls.shift, valid, a);
+      load_lane_part<IPT>(p, ls.base, ls.shift, valid, a);
 #pragma unroll
       for (int k = 0; k < IPT / 4; ++k) {
         const uint32_t wk = lane_word(a, k, ls.shift, e);
@@ -404,7 +408,7 @@ struct EmitLayout {
   }
 };
 
-template <bool SCALED>
+template <bool SCALED, int IPT>
 __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
   using RC = RowCode<SCALED>;
   extern __shared__ __align__(16) uint8_t lds[];
@@ -435,12 +439,12 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
     __syncthreads();
     const uint32_t tile = __builtin_amdgcn_readfirstlane(w.tile);
     if (tile >= nt) break;
-    const TileSpan ts = tile_span(p, tile);
+    const TileSpan ts = tile_span<IPT>(p, tile);
     if (ts.multi()) {
       tile_inputs(p, ts, tile, s_ichunk);
       __syncthreads();
     }
-    const LaneSpan ls = lane_span(p, ts, tile, s_ichunk);
+    const LaneSpan ls = lane_span<IPT>(p, ts, tile, s_ichunk);
     const uint64_t tbase = ts.tbase, base = ls.base;
     const uint32_t valid = ls.valid;
     const uint32_t row0 = lds0 + lane_state(p, tile, tid) * ROWB;
@@ -452,7 +456,7 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
     // to the sink once per 4 bytes (so at most 3 rows past it); in the lane
     // holding an input's last bytes, the positions past them take the row's
     // stay transition.
-    uint32_t row = row0, waddr = lterm0 + tid, bm[IPT / 32] = {0, 0, 0, 0}, km[IPT / 32] = {0, 0, 0, 0};
+    uint32_t row = row0, waddr = lterm0 + tid, bm[IPT / 32] = {}, km[IPT / 32] = {};
     // off2 = 2 * byte: the transition's offset in the row
     auto step1 = [&](uint32_t off2, uint32_t& mb, uint32_t& mk, uint32_t bit) {
       const uint32_t x = lds_load16(row + off2);
@@ -467,7 +471,7 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
     };
     if (valid == IPT) {
       uint32_t a[IPT / 4 + 1];
-      load_lane(p, base, ls.shift, a);
+      load_lane<IPT>(p, base, ls.shift, a);
 #pragma unroll
       for (int k = 0; k < IPT / 4; ++k) {
         const uint32_t wk = lane_word(a, k, ls.shift, row);
@@ -482,7 +486,7 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
       }
     } else if (valid > 0) {
       uint32_t a[IPT / 4 + 1];
-      load_lane_part(p, base, ls.shift, valid, a);
+      load_lane_part<IPT>(p, base, ls.shift, valid, a);
 #pragma unroll
       for (int k = 0; k < IPT / 4; ++k) {
         const uint32_t wk = lane_word(a, k, ls.shift, row);
@@ -554,9 +558,11 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
       } else {
         if (lane == 0) publish64(p.count_status + tile, ST64_AGG | agg);
         int64_t top = (int64_t)tile - 1;
+        const uint32_t win = p.lb_win;  // predecessor tiles per probe, 1..WAVE
         for (;;) {
           const int64_t idx = top - (int64_t)lane;
-          const uint64_t st = idx >= 0 ? observe64(p.count_status + idx) : ST64_PRE;
+          // lanes past the window read nothing: an aggregate of zero
+          const uint64_t st = lane >= win ? ST64_AGG : (idx >= 0 ? observe64(p.count_status + idx) : ST64_PRE);
           const uint64_t fl = st & ST64_MASK;
           const uint64_t zero = __ballot(fl == 0);
           const uint64_t prem = __ballot(fl == ST64_PRE);
@@ -577,7 +583,7 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
           xc += c;
           xl = xl > l ? xl : l;
           if (first_pre < WAVE) break;
-          top -= WAVE;
+          top -= win;
         }
         if (lane == 0) {
           const uint32_t ic = xc + tc, il = xl > tl ? xl : tl;
@@ -709,8 +715,10 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
   }
 }
 
+// Persistent grid of a kernel: its resident workgroups per CU (at most wpc
+// when wpc > 0) times the CUs.
 template <typename K>
-hipError_t occupancy_grid(K k, size_t lds, int* grid) {
+hipError_t occupancy_grid(K k, size_t lds, int wpc, int* grid) {
   int dev = 0, cus = 0, per = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;
@@ -720,6 +728,7 @@ hipError_t occupancy_grid(K k, size_t lds, int* grid) {
   if (e != hipSuccess) return e;
   e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(k), WG, lds);
   if (e != hipSuccess) return e;
+  if (wpc > 0 && per > wpc) per = wpc;
   *grid = cus * (per < 1 ? 1 : per);
   return hipSuccess;
 }
@@ -732,18 +741,31 @@ namespace {
 // no static LDS, so its dynamic LDS starts at address 0.
 bool emit_scaled(const LexParams& p) {
   static const bool no_static_lds = [] {
-    hipFuncAttributes fa{};
-    return hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(lex_emit<true>)) == hipSuccess &&
-           fa.sharedSizeBytes == 0;
+    hipFuncAttributes fa{}, fb{};
+    return hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(lex_emit<true, 128>)) == hipSuccess &&
+           hipFuncGetAttributes(&fb, reinterpret_cast<const void*>(lex_emit<true, 64>)) == hipSuccess &&
+           fa.sharedSizeBytes == 0 && fb.sharedSizeBytes == 0;
   }();
   return no_static_lds && (uint32_t)p.nstate * ROWB <= 0x4000u;
 }
 // Kept tokens per lane whose terminals walk 1 keeps in LDS: as many as fit
-// in 40 KiB per workgroup (four workgroups per CU), between 16 and IPT.
+// in 40 KiB per workgroup (four workgroups per CU), between 16 and the
+// lane's bytes.
 uint32_t emit_rcap(const LexParams& p) {
   const uint32_t budget = 40 * 1024, fixed = EmitLayout((uint32_t)p.nstate, 0).total + 16;
-  const uint32_t r = fixed < budget ? (budget - fixed) / WG : 0;
-  return r < 16 ? 16 : (r > (uint32_t)IPT ? (uint32_t)IPT : r);
+  const uint32_t r = fixed < budget ? (budget - fixed) / WG : 0, ipt = (uint32_t)p.ipt;
+  return r < 16 ? 16 : (r > ipt ? ipt : r);
+}
+
+// The kernels of an items-per-thread shape.
+struct LexKernels {
+  void (*fold)(LexParams);
+  void (*emit)(LexParams);
+};
+LexKernels lex_kernels(const LexParams& p) {
+  const bool sc = emit_scaled(p);
+  if (p.ipt == 64) return {lex_fold<64>, sc ? lex_emit<true, 64> : lex_emit<false, 64>};
+  return {lex_fold<128>, sc ? lex_emit<true, 128> : lex_emit<false, 128>};
 }
 }  // namespace
 
@@ -768,12 +790,14 @@ __global__ void lex_tile_map(const uint32_t* chunk_first, uint32_t nseg, const u
 }
 }  // namespace
 
-hipError_t lex_grid_size(const LexParams& p, int* fold_grid, int* emit_grid) {
-  hipError_t e = occupancy_grid(lex_fold, lex_fold_lds(p), fold_grid);
+hipError_t lex_grid_size(const LexParams& p, int wpc, int* fold_grid, int* emit_grid) {
+  const LexKernels k = lex_kernels(p);
+  hipError_t e = occupancy_grid(k.fold, lex_fold_lds(p), wpc, fold_grid);
   if (e != hipSuccess) return e;
-  return emit_scaled(p) ? occupancy_grid(lex_emit<true>, lex_emit_lds(p), emit_grid)
-                        : occupancy_grid(lex_emit<false>, lex_emit_lds(p), emit_grid);
+  return occupancy_grid(k.emit, lex_emit_lds(p), wpc, emit_grid);
 }
+
+bool lex_shape_ok(int ipt, int lb_win) { return (ipt == 64 || ipt == 128) && lb_win >= 1 && lb_win <= WAVE; }
 
 hipError_t lex_launch(const LexParams& p, int fold_grid, int emit_grid, hipStream_t st, KernelTimer& tm) {
   const uint32_t nt = p.ntiles;
@@ -782,8 +806,9 @@ hipError_t lex_launch(const LexParams& p, int fold_grid, int emit_grid, hipStrea
   hipError_t e;
   hipLaunchKernelGGL(lex_tile_map, dim3((2 * nt + 255) / 256), dim3(256), 0, st, p.chunk_first, p.nseg, p.nt_dev,
                      p.tile_seg);
+  const LexKernels k = lex_kernels(p);
   if ((e = tm.begin(st, "lex_fold")) != hipSuccess) return e;
-  hipLaunchKernelGGL(lex_fold, dim3(g1), dim3(WG), lex_fold_lds(p), st, p);
+  hipLaunchKernelGGL(k.fold, dim3(g1), dim3(WG), lex_fold_lds(p), st, p);
   if ((e = tm.end(st)) != hipSuccess || (e = tm.begin(st, "lex_tile_scan")) != hipSuccess) return e;
   const uint32_t nblk = (nt + TSB - 1) / TSB;
   hipLaunchKernelGGL(lex_tile_scan1, dim3(nblk), dim3(TSB), 0, st, p);
@@ -792,10 +817,7 @@ hipError_t lex_launch(const LexParams& p, int fold_grid, int emit_grid, hipStrea
   if ((e = tm.end(st)) != hipSuccess || (e = tm.begin(st, "lex_emit")) != hipSuccess) return e;
   LexParams q = p;
   q.rcap = emit_rcap(p);
-  if (emit_scaled(p))
-    hipLaunchKernelGGL(lex_emit<true>, dim3(g3), dim3(WG), lex_emit_lds(p), st, q);
-  else
-    hipLaunchKernelGGL(lex_emit<false>, dim3(g3), dim3(WG), lex_emit_lds(p), st, q);
+  hipLaunchKernelGGL(k.emit, dim3(g3), dim3(WG), lex_emit_lds(p), st, q);
   if ((e = tm.end(st)) != hipSuccess) return e;
   return hipGetLastError();
 }

@@ -9,8 +9,8 @@
 namespace alpacc {
 
 constexpr int LEX_WG = 256;   // lanes per workgroup (4 waves)
-constexpr int LEX_IPT = 128;  // input bytes per lane (a chunk); a tile is 256 chunks, 32 KiB
-constexpr uint32_t LEX_TILE = (uint32_t)LEX_WG * LEX_IPT;
+constexpr int LEX_IPT = 128;  // input bytes per lane (a chunk) by default; 64 is the other shape
+constexpr int LEX_LB_WIN = 64;  // predecessor tiles per lookback probe by default
 
 constexpr uint32_t LEX_REJECT_FINAL = 1u;     // last state not accepting
 constexpr uint32_t LEX_REJECT_OVERFLOW = 2u;  // a kept token is longer than length_t
@@ -60,6 +60,9 @@ struct LexParams {
   const uint32_t* nt_dev;
   uint32_t ntiles;
   uint32_t rcap;  // set by lex_launch: kept tokens per lane lex_emit keeps terminals for
+  // shape: input bytes per lane (64 or 128: the chunk) and predecessor
+  // tiles per lookback probe (1..64)
+  int ipt, lb_win;
 };
 
 __device__ __forceinline__ void store_length(const LexParams& p, uint64_t slot, uint64_t len) {
@@ -73,7 +76,10 @@ __device__ __forceinline__ void store_length(const LexParams& p, uint64_t slot, 
 
 size_t lex_fold_lds(const LexParams& p);
 size_t lex_emit_lds(const LexParams& p);
-hipError_t lex_grid_size(const LexParams& p, int* fold_grid, int* emit_grid);
+// Persistent grids of the fold and emit kernels of p's shape, at most wpc
+// workgroups per CU (0: as many as fit).
+hipError_t lex_grid_size(const LexParams& p, int wpc, int* fold_grid, int* emit_grid);
+bool lex_shape_ok(int ipt, int lb_win);
 hipError_t lex_launch(const LexParams& p, int fold_grid, int emit_grid, hipStream_t st, KernelTimer& tm);
 
 }  // namespace alpacc

@@ -74,7 +74,12 @@ struct Context {
   const GrammarDesc* g = nullptr;
   int device = 0;
   hipStream_t stream = nullptr;
-  int lex_grid = 0, lex_grid2 = 0;  // persistent grids of the fold and emit kernels
+  // lexer shape: input bytes per lane (the chunk), predecessor tiles per
+  // lookback probe, workgroups per CU of the persistent kernels (0 = as
+  // many as fit); lex_grid* are the persistent grids of that shape (0 =
+  // not computed yet)
+  int lex_ipt = 128, lex_lb = 64, lex_wpc = 0;
+  int lex_grid = 0, lex_grid2 = 0;
   size_t resolve_lds_set = 0;        // dynamic LDS granted to the parser's resolve kernel
   // resident tables
   uint8_t* d_class = nullptr;

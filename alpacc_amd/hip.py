@@ -44,6 +44,11 @@ class Batch(C.Structure):
                 ("token_parents", C.c_void_p)]
 
 
+class LexShape(C.Structure):
+    _fields_ = [("items_per_thread", C.c_int), ("workgroups_per_cu", C.c_int), ("lookback_window", C.c_int),
+                ("ms", C.c_double)]
+
+
 class HipError(RuntimeError):
     pass
 
@@ -101,6 +106,9 @@ def load_library(path: str = LIB_PATH):
         "alpacc_hip_set_graphs": (i32, [vp, i32]),
         "alpacc_hip_graph_stats": (i32, [vp, C.POINTER(u64), C.POINTER(u64)]),
         "alpacc_hip_server_set_graphs": (i32, [vp, i32]),
+        "alpacc_hip_set_lex_shape": (i32, [vp, C.POINTER(LexShape)]),
+        "alpacc_hip_get_lex_shape": (i32, [vp, C.POINTER(LexShape)]),
+        "alpacc_hip_autotune_lexer": (i32, [vp, i32, vp, u64, i32, C.POINTER(LexShape)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -256,6 +264,26 @@ class Backend:
             parents=_take(out.parents, N * 8, "<i8"),
             token_parents=_take(out.token_parents, T * 8, "<i8"),
         )
+
+    # -- lexer launch shape ----------------------------------------------------
+    def set_lex_shape(self, items_per_thread: int = 128, workgroups_per_cu: int = 0, lookback_window: int = 64):
+        sh = LexShape(items_per_thread, workgroups_per_cu, lookback_window, 0.0)
+        _check(self.lib.alpacc_hip_set_lex_shape(self.h, C.byref(sh)) >= 0 and 1 or -1)
+
+    def lex_shape(self) -> dict:
+        sh = LexShape()
+        _check(self.lib.alpacc_hip_get_lex_shape(self.h, C.byref(sh)) >= 0 and 1 or -1)
+        return {"items_per_thread": sh.items_per_thread, "workgroups_per_cu": sh.workgroups_per_cu,
+                "lookback_window": sh.lookback_window}
+
+    def autotune_lexer(self, d_ptr: int, n: int, mode: str = "both", reps: int = 3) -> dict:
+        """Time every lexer shape on a device-resident sample, keep the
+        fastest; returns it with its time (ms)."""
+        sh = LexShape()
+        _check(self.lib.alpacc_hip_autotune_lexer(self.h, MODES[mode], C.c_void_p(d_ptr), n, reps,
+                                                  C.byref(sh)) >= 0 and 1 or -1)
+        return {"items_per_thread": sh.items_per_thread, "workgroups_per_cu": sh.workgroups_per_cu,
+                "lookback_window": sh.lookback_window, "ms": round(sh.ms, 4)}
 
     # -- graphs for combined-mode batches -----------------------------------------
     def set_graphs(self, on: bool = True):

@@ -117,6 +117,29 @@ int alpacc_hip_set_timing(alpacc_hip_ctx *ctx, int on);
 int alpacc_hip_timing(const alpacc_hip_ctx *ctx, const char *stage, double *total_ms, uint64_t *runs);
 const char *alpacc_hip_timed_kernels(const alpacc_hip_ctx *ctx);
 
+/* Launch shape of the lexer.  The reference bakes BLOCK_SIZE and
+ * ITEMS_PER_THREAD into the generated CUDA source from a per-architecture
+ * table (backends/cuda/cli.cu:50-82, picked by the --sm-arch codegen
+ * constant, src/Alpacc/Generator/Cuda/Generator.hs:77-92); here the shape is
+ * a property of the context, measured per grammar: items_per_thread = input
+ * bytes per lane (64 or 128), workgroups_per_cu = persistent workgroups of
+ * the fold and emit kernels per CU (0 = as many as fit), lookback_window =
+ * predecessor tiles examined per lookback probe (1..64); ms = the lexer's
+ * time on the tuning sample.  autotune_lexer times every shape of {64, 128}
+ * x {2, 3, 4, 0} x {16, 32, 64} on `reps` runs over a sample in device
+ * memory (d_sample, n bytes), keeps the fastest in the context and returns
+ * it; the stage timing totals are reset.  Every shape gives the same
+ * results. */
+typedef struct {
+  int items_per_thread, workgroups_per_cu, lookback_window;
+  double ms;
+} alpacc_hip_lex_shape;
+
+int alpacc_hip_set_lex_shape(alpacc_hip_ctx *ctx, const alpacc_hip_lex_shape *shape);
+int alpacc_hip_get_lex_shape(const alpacc_hip_ctx *ctx, alpacc_hip_lex_shape *shape);
+int alpacc_hip_autotune_lexer(alpacc_hip_ctx *ctx, int mode, const uint8_t *d_sample, uint64_t n, int reps,
+                              alpacc_hip_lex_shape *best);
+
 /* ---- device-memory entry points ---------------------------------------- */
 /* Input already in device memory; results stay in device buffers owned by the
  * context, valid until the next call on it.  Only the count and verdict cross

@@ -403,6 +403,64 @@ def test_batch_device_graph_replays():
         be.close()
 
 
+SHAPES = [(64, 2, 16), (64, 0, 64), (128, 3, 1), (128, 4, 32)]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: f"ipt{s[0]}-wpc{s[1]}-win{s[2]}")
+def test_lexer_shapes(shape):
+    """Every lexer launch shape the tuner can pick gives the oracle's
+    records: generated text around chunk and tile edges, flips, dense
+    tokens and a batch of many small inputs."""
+    from alpacc_amd import hip
+    ipt, wpc, win = shape
+    bes = {}
+    try:
+        for g in ("json", "arithmetic", "sexp"):
+            bes[g] = hip.Backend(g)
+            bes[g].set_lex_shape(ipt, wpc, win)
+            assert bes[g].lex_shape() == {"items_per_thread": ipt, "workgroups_per_cu": wpc, "lookback_window": win}
+        for g in ("json", "arithmetic", "sexp"):
+            for n in (1, 63, 64, 65, 127, 128, 129, 16383, 16384, 16385, 100003, 1 << 20):
+                data = datagen.generate(g, n, seed=n, conforming=n % 2 == 0).tobytes()
+                for mode in ("lexer", "both"):
+                    check(bes, g, mode, data, f"shape {shape}")
+        check(bes, "json", "both", b"[" + b"1," * (1 << 19) + b"1]", f"dense, shape {shape}")
+        rng = random.Random(zlib.crc32(f"shape {shape}".encode()))
+        payloads = [datagen.generate("json", rng.randrange(0, 300), seed=i, conforming=True).tobytes() for i in range(700)]
+        got = bes["json"].run_batch("both", payloads)
+        exp = b"".join(pyoracle.record("json", "both", x) for x in payloads)
+        assert first_diff(got, exp) is None, f"batch of small inputs, shape {shape}"
+    finally:
+        for b in bes.values():
+            b.close()
+
+
+def test_autotune_lexer():
+    """The tuner picks a shape of its search space from a device-resident
+    sample, keeps it in the context, and results stay the oracle's."""
+    import torch
+    from alpacc_amd import hip
+    be = hip.Backend("json")
+    try:
+        data = datagen.generate("json", 4 << 20, seed=9, conforming=True).tobytes()
+        dev = torch.frombuffer(bytearray(data), dtype=torch.uint8).to("cuda")
+        torch.cuda.synchronize()
+        best = be.autotune_lexer(dev.data_ptr(), len(data), "both", reps=2)
+        print("tuned shape", best, flush=True)
+        assert best["items_per_thread"] in (64, 128) and best["workgroups_per_cu"] in (0, 2, 3, 4)
+        assert best["lookback_window"] in (16, 32, 64) and best["ms"] > 0
+        assert be.lex_shape() == {k: best[k] for k in ("items_per_thread", "workgroups_per_cu", "lookback_window")}
+        for n in (5000, 300000):
+            x = data[:n]
+            for mode in ("lexer", "both"):
+                got = be.run_case(mode, x)
+                assert first_diff(got, pyoracle.record("json", mode, x)) is None
+        with pytest.raises(hip.HipError):
+            be.set_lex_shape(96, 0, 64)
+    finally:
+        be.close()
+
+
 def test_batch_alone_and_in_mix(backends):
     """Each result is the same alone and in a mix: every input of a batch
     against run_case on that input."""
