@@ -287,7 +287,7 @@ int alpacc_hip_autotune_lexer(alpacc_hip_ctx* h, int mode, const uint8_t* d_samp
   const bool timing = c.timing, ktiming = c.timer.on;
   c.timing = true;  // the lexer's stage events time each run
   c.timer.on = false;
-  static const int ipts[] = {64, 128}, wpcs[] = {2, 3, 4, 0}, wins[] = {16, 32, 64};
+  static const int ipts[] = {64, 128}, wpcs[] = {2, 3, 4, 0}, wins[] = {4, 8, 16, 32, 64};
   alpacc_hip_lex_shape top{c.lex_ipt, c.lex_wpc, c.lex_lb, 0};
   double top_ms = -1;
   int r = 0;

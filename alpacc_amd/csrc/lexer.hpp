@@ -10,7 +10,7 @@ namespace alpacc {
 
 constexpr int LEX_WG = 256;   // lanes per workgroup (4 waves)
 constexpr int LEX_IPT = 128;  // input bytes per lane (a chunk) by default; 64 is the other shape
-constexpr int LEX_LB_WIN = 64;  // predecessor tiles per lookback probe by default
+constexpr int LEX_LB_WIN = 16;  // predecessor tiles per lookback probe by default
 
 constexpr uint32_t LEX_REJECT_FINAL = 1u;     // last state not accepting
 constexpr uint32_t LEX_REJECT_OVERFLOW = 2u;  // a kept token is longer than length_t

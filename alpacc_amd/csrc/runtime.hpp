@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "grammar.hpp"
+#include "lexer.hpp"
 #include "timer.hpp"
 
 namespace alpacc {
@@ -78,7 +79,7 @@ struct Context {
   // lookback probe, workgroups per CU of the persistent kernels (0 = as
   // many as fit); lex_grid* are the persistent grids of that shape (0 =
   // not computed yet)
-  int lex_ipt = 128, lex_lb = 64, lex_wpc = 0;
+  int lex_ipt = LEX_IPT, lex_lb = LEX_LB_WIN, lex_wpc = 0;
   int lex_grid = 0, lex_grid2 = 0;
   size_t resolve_lds_set = 0;        // dynamic LDS granted to the parser's resolve kernel
   // resident tables

@@ -266,7 +266,7 @@ class Backend:
         )
 
     # -- lexer launch shape ----------------------------------------------------
-    def set_lex_shape(self, items_per_thread: int = 128, workgroups_per_cu: int = 0, lookback_window: int = 64):
+    def set_lex_shape(self, items_per_thread: int = 128, workgroups_per_cu: int = 0, lookback_window: int = 16):
         sh = LexShape(items_per_thread, workgroups_per_cu, lookback_window, 0.0)
         _check(self.lib.alpacc_hip_set_lex_shape(self.h, C.byref(sh)) >= 0 and 1 or -1)
 

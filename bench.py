@@ -302,6 +302,7 @@ def main(argv=None):
     ap.add_argument("--variant", choices=["conforming", "literal"], default="conforming")
     ap.add_argument("--cpu-sample", type=int, default=256 << 20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default lexer shape instead of tuning it")
     ap.add_argument("--batch-count", type=int, default=20000, help="inputs of the batching workload (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--serve-rates", default="2000,10000,50000", help="request rates of the serving workload ('' = skip)")
@@ -339,6 +340,9 @@ def main(argv=None):
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream().cuda_stream
     be = hip.Backend("json", device=0)
+    # the lexer's launch shape, tuned on this input before the timed region
+    shape = be.lex_shape() if a.no_autotune else be.autotune_lexer(dev.data_ptr(), n, "both", reps=3)
+    log(f"lexer shape: {shape}")
 
     def step():
         ok, tree = be.lex_parse_device(dev.data_ptr(), n, stream=stream)
@@ -427,7 +431,8 @@ def main(argv=None):
         "data": f"synthetic ({a.variant} JSON, seed 1)",
         "config": {"workload": f"json lex+parse, one {n}-byte input (combined mode, device-resident)",
                    "grammar": "grammars/json.alp", "bytes": n, "tokens": T, "tree_nodes": N,
-                   "valid": bool(ok), "variant": a.variant},
+                   "valid": bool(ok), "variant": a.variant,
+                   "lexer_shape": {k: shape[k] for k in ("items_per_thread", "workgroups_per_cu", "lookback_window")}},
         "pct_hbm_peak_input": round(100 * n / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 2),
         "path_algorithmic_GBps": round(path_bytes / (ms / 1e3) / 1e9, 2),
         "kernels_ms": {k: round(v, 4) for k, v in sorted(kernels.items(), key=lambda x: -x[1])},

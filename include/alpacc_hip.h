@@ -126,7 +126,7 @@ const char *alpacc_hip_timed_kernels(const alpacc_hip_ctx *ctx);
  * the fold and emit kernels per CU (0 = as many as fit), lookback_window =
  * predecessor tiles examined per lookback probe (1..64); ms = the lexer's
  * time on the tuning sample.  autotune_lexer times every shape of {64, 128}
- * x {2, 3, 4, 0} x {16, 32, 64} on `reps` runs over a sample in device
+ * x {2, 3, 4, 0} x {4, 8, 16, 32, 64} on `reps` runs over a sample in device
  * memory (d_sample, n bytes), keeps the fastest in the context and returns
  * it; the stage timing totals are reset.  Every shape gives the same
  * results. */

@@ -448,7 +448,7 @@ def test_autotune_lexer():
         best = be.autotune_lexer(dev.data_ptr(), len(data), "both", reps=2)
         print("tuned shape", best, flush=True)
         assert best["items_per_thread"] in (64, 128) and best["workgroups_per_cu"] in (0, 2, 3, 4)
-        assert best["lookback_window"] in (16, 32, 64) and best["ms"] > 0
+        assert best["lookback_window"] in (4, 8, 16, 32, 64) and best["ms"] > 0
         assert be.lex_shape() == {k: best[k] for k in ("items_per_thread", "workgroups_per_cu", "lookback_window")}
         for n in (5000, 300000):
             x = data[:n]
