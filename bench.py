@@ -303,6 +303,8 @@ def main(argv=None):
     ap.add_argument("--cpu-sample", type=int, default=256 << 20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default lexer shape instead of tuning it")
+    ap.add_argument("--lexer-shape", metavar="IPT,WPC,WIN", default="",
+                    help="use this lexer shape (bytes per lane, workgroups per CU, lookback window) instead of tuning")
     ap.add_argument("--batch-count", type=int, default=20000, help="inputs of the batching workload (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--serve-rates", default="2000,10000,50000", help="request rates of the serving workload ('' = skip)")
@@ -341,7 +343,9 @@ def main(argv=None):
     stream = torch.cuda.current_stream().cuda_stream
     be = hip.Backend("json", device=0)
     # the lexer's launch shape, tuned on this input before the timed region
-    shape = be.lex_shape() if a.no_autotune else be.autotune_lexer(dev.data_ptr(), n, "both", reps=3)
+    if a.lexer_shape:
+        be.set_lex_shape(*[int(x) for x in a.lexer_shape.split(",")])
+    shape = be.lex_shape() if a.no_autotune or a.lexer_shape else be.autotune_lexer(dev.data_ptr(), n, "both", reps=3)
     log(f"lexer shape: {shape}")
 
     def step():
