@@ -103,14 +103,37 @@ int both_record(Context& c, uint64_t n, const uint8_t* payload, std::vector<uint
     o += bytes;
     return 0;
   };
-  if ((r = pull(t.tokens.terminals, T * tb)) || (r = pull(t.tokens.starts, T * ib)) ||
+  // i64 on the device; a 32-bit index_t keeps the low half of each (values
+  // stay below 2^31: one launch takes fewer than 2^31 bytes)
+  std::vector<int64_t> wide;
+  struct Narrow {
+    uint8_t* at;
+    size_t from, count;
+  };
+  std::vector<Narrow> narrow;
+  auto pull_index = [&](const int64_t* d, uint64_t count) -> int {
+    if (ib == 8) return pull(d, count * 8);
+    const size_t from = wide.size();
+    wide.resize(from + count);
+    if (count) HIP_TRY(hipMemcpyAsync(wide.data() + from, d, count * 8, hipMemcpyDeviceToHost, st));
+    narrow.push_back({o, from, count});
+    o += count * ib;
+    return 0;
+  };
+  wide.reserve(ib == 8 ? 0 : (size_t)(2 * T + N));
+  if ((r = pull(t.tokens.terminals, T * tb)) || (r = pull_index(t.tokens.starts, T)) ||
       (r = pull(t.tokens.lengths, T * lb)))
     return r;
   memcpy(o, &N, 8);
   o += 8;
-  if ((r = pull(t.productions, N * pb)) || (r = pull(t.parents, N * ib)) || (r = pull(t.token_parents, T * ib)))
+  if ((r = pull(t.productions, N * pb)) || (r = pull_index(t.parents, N)) || (r = pull_index(t.token_parents, T)))
     return r;
   HIP_TRY(hipStreamSynchronize(st));
+  for (const Narrow& x : narrow)
+    for (size_t k = 0; k < x.count; ++k) {
+      const int32_t v = (int32_t)wide[x.from + k];
+      memcpy(x.at + k * ib, &v, ib);
+    }
   return ALPACC_VALID;
 }
 

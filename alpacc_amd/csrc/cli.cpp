@@ -3,6 +3,7 @@
 //
 //   alpacc-hip -g GRAMMAR [--lexer|--parser] [-i FILE] [-o FILE]
 //              [--timeit] [--server] [--layout] [--device N]
+//              [--index32] [--length N]
 //
 // Invoked through a link named GRAMMAR or GRAMMAR_MODE (MODE = lexer, parser,
 // both) it needs no -g / mode flag, so it can stand in for a generated
@@ -31,6 +32,7 @@ struct Prog {
   std::string grammar;
   int mode = ALPACC_MODE_BOTH;
   int device = 0;
+  int index32 = 0, length_bits = 0;  // --index32, --length N
 };
 
 // Reads one frame's payload into `packed`.  0 = ok, -1 = framing error
@@ -67,7 +69,7 @@ int read_frame(Prog& p, FILE* in, std::vector<uint8_t>& packed, std::vector<uint
 // Answers the frames read so far with one batch call.
 int answer(Prog& p, const std::vector<uint8_t>& packed, const std::vector<uint64_t>& off, FILE* out) {
   if (off.size() < 2) return 0;
-  if (!p.ctx && alpacc_hip_open(p.grammar.c_str(), p.device, &p.ctx) < 0) {
+  if (!p.ctx && alpacc_hip_open_variant(p.grammar.c_str(), p.device, p.index32, p.length_bits, &p.ctx) < 0) {
     fprintf(stderr, "error: %s\n", alpacc_hip_last_error());
     return -1;
   }
@@ -98,7 +100,7 @@ int serve_batch(Prog& p, FILE* in, FILE* out, uint64_t count) {
     if (off.size() == before) continue;
     if (!p.srv) {
       const char* g = p.grammar.c_str();
-      if (alpacc_hip_server_open(&g, 1, p.device, 2, 0, 0, &p.srv) < 0) {
+      if (alpacc_hip_server_open_variant(&g, 1, p.device, 2, 0, 0, p.index32, p.length_bits, &p.srv) < 0) {
         fprintf(stderr, "error: %s\n", alpacc_hip_last_error());
         return -1;
       }
@@ -170,6 +172,8 @@ void usage(const char* prog) {
           "  --server   counted-batch loop until EOF (flush per batch)\n"
           "  --layout   print native type sizes and exit\n"
           "  --device N GPU ordinal (default 0)\n"
+          "  --index32  32-bit index_t (and length_t when the grammar sets no length)\n"
+          "  --length N length_t of N bits (8, 16, 32 or 64)\n"
           "  -h/--help  show this message\n",
           prog);
 }
@@ -219,6 +223,10 @@ int main(int argc, char** argv) {
       outp = argv[++i];
     } else if (a == "--device" && i + 1 < argc) {
       device = atoi(argv[++i]);
+    } else if (a == "--index32") {
+      p.index32 = 1;
+    } else if (a == "--length" && i + 1 < argc) {
+      p.length_bits = atoi(argv[++i]);
     } else {
       fprintf(stderr, "error: unknown argument '%s'\n", argv[i]);
       usage(argv[0]);
@@ -233,7 +241,7 @@ int main(int argc, char** argv) {
   p.device = device;
   {
     char buf[512];
-    if (alpacc_hip_grammar_layout(grammar.c_str(), p.mode, buf, sizeof buf) < 0) {
+    if (alpacc_hip_variant_layout(grammar.c_str(), p.mode, p.index32, p.length_bits, buf, sizeof buf) < 0) {
       fprintf(stderr, "error: %s\n", alpacc_hip_last_error());
       return 1;
     }

@@ -47,6 +47,7 @@ struct GrammarDesc {
   int max_bpop, max_bpush;       // per window, after cancellation inside the window
   int max_spop, max_spush;
   int max_nt;                    // terminal productions per window
+  int length_param;              // the grammar's `length` (bits), 0 when it has none: length_t follows index_t
 };
 
 const GrammarDesc* find_grammar(const char* name);

@@ -202,11 +202,7 @@ void alpacc_hip_close(alpacc_hip_ctx* h) {
   delete h;
 }
 
-int alpacc_hip_grammar_layout(const char* grammar, int mode, char* buf, size_t cap) {
-  if (!grammar || !buf) return fail(ALPACC_ERR_ARG, "null argument");
-  if (int r = check_mode(mode)) return r;
-  const GrammarDesc* g = find_grammar(grammar);
-  if (!g) return fail(ALPACC_ERR_ARG, "unknown grammar '%s'", grammar);
+static int layout_text(const GrammarDesc* g, int mode, char* buf, size_t cap) {
   std::string s;
   char line[64];
   int tb = mode == ALPACC_MODE_LEXER ? g->terminal_bytes_lex : g->terminal_bytes_both;
@@ -229,9 +225,57 @@ int alpacc_hip_grammar_layout(const char* grammar, int mode, char* buf, size_t c
   return (int)s.size();
 }
 
+int alpacc_hip_grammar_layout(const char* grammar, int mode, char* buf, size_t cap) {
+  if (!grammar || !buf) return fail(ALPACC_ERR_ARG, "null argument");
+  if (int r = check_mode(mode)) return r;
+  const GrammarDesc* g = find_grammar(grammar);
+  if (!g) return fail(ALPACC_ERR_ARG, "unknown grammar '%s'", grammar);
+  return layout_text(g, mode, buf, cap);
+}
+
 int alpacc_hip_layout(const alpacc_hip_ctx* h, int mode, char* buf, size_t cap) {
-  if (!h) return fail(ALPACC_ERR_ARG, "null argument");
-  return alpacc_hip_grammar_layout(h->c.g->name, mode, buf, cap);
+  if (!h || !buf) return fail(ALPACC_ERR_ARG, "null argument");
+  if (int r = check_mode(mode)) return r;
+  return layout_text(h->c.g, mode, buf, cap);  // the context's widths
+}
+
+// The descriptor of a width variant (see alpacc_hip_open_variant).
+static int variant_desc(const GrammarDesc* g, int index32, int length_bits, GrammarDesc* v) {
+  if (length_bits != 0 && length_bits != 8 && length_bits != 16 && length_bits != 32 && length_bits != 64)
+    return fail(ALPACC_ERR_ARG, "length %d is not 8, 16, 32 or 64", length_bits);
+  *v = *g;
+  v->index_bytes = index32 ? 4 : 8;
+  // length_t: the requested width, else the grammar's `length`, else the
+  // unsigned counterpart of index_t
+  const int bits = length_bits ? length_bits : v->length_param;
+  v->length_bytes = bits ? bits / 8 : v->index_bytes;
+  return 0;
+}
+
+int alpacc_hip_variant_layout(const char* grammar, int mode, int index32, int length_bits, char* buf, size_t cap) {
+  if (!grammar || !buf) return fail(ALPACC_ERR_ARG, "null argument");
+  if (int r = check_mode(mode)) return r;
+  const GrammarDesc* g = find_grammar(grammar);
+  if (!g) return fail(ALPACC_ERR_ARG, "unknown grammar '%s'", grammar);
+  GrammarDesc v;
+  if (int r = variant_desc(g, index32, length_bits, &v)) return r;
+  return layout_text(&v, mode, buf, cap);
+}
+
+int alpacc_hip_open_variant(const char* grammar, int device, int index32, int length_bits, alpacc_hip_ctx** out) {
+  if (!out || !grammar) return fail(ALPACC_ERR_ARG, "null argument");
+  *out = nullptr;
+  const GrammarDesc* g = find_grammar(grammar);
+  if (!g) return fail(ALPACC_ERR_ARG, "unknown grammar '%s'", grammar);
+  GrammarDesc v;
+  if (int r = variant_desc(g, index32, length_bits, &v)) return r;
+  alpacc_hip_ctx* h = nullptr;
+  int r = alpacc_hip_open(grammar, device, &h);
+  if (r < 0) return r;
+  h->c.gv = v;
+  h->c.g = &h->c.gv;
+  *out = h;
+  return 0;
 }
 
 uint64_t alpacc_hip_grammar_payload_bytes(const char* grammar, int mode, uint64_t n) {

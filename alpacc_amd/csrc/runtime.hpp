@@ -72,7 +72,8 @@ struct BatchCaps {
 };
 
 struct Context {
-  const GrammarDesc* g = nullptr;
+  const GrammarDesc* g = nullptr;  // the grammar, or gv for a width variant
+  GrammarDesc gv{};
   int device = 0;
   hipStream_t stream = nullptr;
   // lexer shape: input bytes per lane (the chunk), predecessor tiles per

@@ -170,6 +170,12 @@ extern "C" {
 
 int alpacc_hip_server_open(const char* const* grammars, int ngrammars, int device, int lanes, uint64_t hbm_budget,
                            uint64_t max_batch, alpacc_hip_server** out) {
+  return alpacc_hip_server_open_variant(grammars, ngrammars, device, lanes, hbm_budget, max_batch, 0, 0, out);
+}
+
+int alpacc_hip_server_open_variant(const char* const* grammars, int ngrammars, int device, int lanes,
+                                   uint64_t hbm_budget, uint64_t max_batch, int index32, int length_bits,
+                                   alpacc_hip_server** out) {
   if (!grammars || ngrammars <= 0 || lanes <= 0 || !out) return alpacc::fail(ALPACC_ERR_ARG, "bad server arguments");
   *out = nullptr;
   auto* s = new alpacc_hip_server();
@@ -182,7 +188,7 @@ int alpacc_hip_server_open(const char* const* grammars, int ngrammars, int devic
     s->names.emplace_back(grammars[g]);
     for (int l = 0; l < lanes; ++l) {
       alpacc_hip_ctx* c = nullptr;
-      const int r = alpacc_hip_open(grammars[g], device, &c);
+      const int r = alpacc_hip_open_variant(grammars[g], device, index32, length_bits, &c);
       if (r < 0) {
         for (alpacc_hip_ctx* x : s->ctxs) alpacc_hip_close(x);
         delete s;

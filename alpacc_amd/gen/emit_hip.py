@@ -260,6 +260,7 @@ def emit(g: Grammar) -> str:
         f"  {ns}::PAIR_REC, {ns}::PROD_INFO,",
         f"  {max(1, d['max_bpop'])}, {max(1, d['max_bpush'])}, {max(1, d['max_spop'])}, {max(1, d['max_spush'])},",
         f"  {max(1, d['max_nt'])},",
+        f"  {g.length_bits or 0},",
         "};",
         "}  // namespace alpacc",
         "",

@@ -107,6 +107,10 @@ def load_library(path: str = LIB_PATH):
         "alpacc_hip_graph_stats": (i32, [vp, C.POINTER(u64), C.POINTER(u64)]),
         "alpacc_hip_server_set_graphs": (i32, [vp, i32]),
         "alpacc_hip_set_lex_shape": (i32, [vp, C.POINTER(LexShape)]),
+        "alpacc_hip_open_variant": (i32, [C.c_char_p, i32, i32, i32, C.POINTER(vp)]),
+        "alpacc_hip_variant_layout": (i32, [C.c_char_p, i32, i32, i32, C.c_char_p, C.c_size_t]),
+        "alpacc_hip_server_open_variant": (i32, [C.POINTER(C.c_char_p), i32, i32, i32, u64, u64, i32, i32,
+                                                 C.POINTER(vp)]),
         "alpacc_hip_get_lex_shape": (i32, [vp, C.POINTER(LexShape)]),
         "alpacc_hip_autotune_lexer": (i32, [vp, i32, vp, u64, i32, C.POINTER(LexShape)]),
     }
@@ -154,11 +158,15 @@ def _take(ptr, nbytes: int, dtype) -> np.ndarray:
 class Backend:
     """One grammar resident on one device."""
 
-    def __init__(self, grammar: str, device: int = 0):
+    def __init__(self, grammar: str, device: int = 0, index32: bool = False, length: int = 0):
+        """index32 / length (bits): the width variant of alpacc_hip_open_variant."""
         self.lib = load_library()
         self.grammar = grammar
         h = C.c_void_p()
-        _check(self.lib.alpacc_hip_open(grammar.encode(), device, C.byref(h)))
+        if index32 or length:
+            _check(self.lib.alpacc_hip_open_variant(grammar.encode(), device, int(index32), int(length), C.byref(h)))
+        else:
+            _check(self.lib.alpacc_hip_open(grammar.encode(), device, C.byref(h)))
         self.h = h
         self.layouts = {m: self._layout(m) for m in MODES}
 

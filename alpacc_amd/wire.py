@@ -108,6 +108,35 @@ def decode_record(buf: bytes, off: int, layout: Layout):
     return rec, off
 
 
+def encode_record(rec: Record, layout: Layout) -> bytes:
+    """The response record of ``rec`` in ``layout`` (the inverse of
+    decode_record; integer fields are narrowed or widened to its widths)."""
+    if not rec.valid:
+        return b"\x00"
+    out = io.BytesIO()
+    out.write(b"\x01")
+    if layout.mode == "parser":
+        out.write(struct.pack("<Q", len(rec.productions)))
+        out.write(np.asarray(rec.productions, dtype=f"<u{layout.production}").tobytes())
+        return out.getvalue()
+    cnt = len(rec.terminals)
+    out.write(struct.pack("<Q", cnt))
+    if layout.mode == "lexer":
+        dt = np.dtype([("t", f"<u{layout.terminal}"), ("s", f"<i{layout.index}"), ("l", f"<u{layout.length}")])
+        arr = np.empty(cnt, dtype=dt)
+        arr["t"], arr["s"], arr["l"] = rec.terminals, rec.starts, rec.lengths
+        out.write(arr.tobytes())
+        return out.getvalue()
+    out.write(np.asarray(rec.terminals, dtype=f"<u{layout.terminal}").tobytes())
+    out.write(np.asarray(rec.starts, dtype=f"<i{layout.index}").tobytes())
+    out.write(np.asarray(rec.lengths, dtype=f"<u{layout.length}").tobytes())
+    out.write(struct.pack("<Q", len(rec.nodes)))
+    out.write(np.asarray(rec.nodes, dtype=f"<u{layout.production}").tobytes())
+    out.write(np.asarray(rec.parents, dtype=f"<i{layout.index}").tobytes())
+    out.write(np.asarray(rec.token_parents, dtype=f"<i{layout.index}").tobytes())
+    return out.getvalue()
+
+
 def decode_batch(buf: bytes, layout: Layout) -> List[Record]:
     (count,) = struct.unpack_from("<Q", buf, 0)
     off = 8

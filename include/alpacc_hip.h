@@ -65,6 +65,17 @@ typedef struct {
  * program's static const tables, src/Alpacc/Generator/C/Lexer.hs:16-58 and
  * C/Parser.hs:21-95). */
 int alpacc_hip_open(const char *grammar, int device, alpacc_hip_ctx **out);
+
+/* A context whose integer widths follow the generator's --index32 flag and
+ * the grammar's `length` parameter (src/Alpacc/Generator/C/Generator.hs:
+ * 36-53, docs/wire-protocols.md:37-50) chosen at open time instead of at
+ * generation time: index32 != 0 makes index_t int32 (starts, parents);
+ * length_bits (8, 16, 32, 64) sets length_t, 0 keeps the grammar's `length`,
+ * or without one the unsigned counterpart of index_t.  Records, layouts and
+ * the length overflow rule follow these widths. */
+int alpacc_hip_open_variant(const char *grammar, int device, int index32, int length_bits, alpacc_hip_ctx **ctx);
+/* print_layout of such a variant, without a context. */
+int alpacc_hip_variant_layout(const char *grammar, int mode, int index32, int length_bits, char *buf, size_t cap);
 void alpacc_hip_close(alpacc_hip_ctx *ctx);
 const char *alpacc_hip_last_error(void);
 
@@ -218,6 +229,10 @@ typedef struct alpacc_hip_server alpacc_hip_server;
 
 int alpacc_hip_server_open(const char *const *grammars, int ngrammars, int device, int lanes, uint64_t hbm_budget,
                            uint64_t max_batch, alpacc_hip_server **out);
+/* The same with every lane's context a width variant (alpacc_hip_open_variant). */
+int alpacc_hip_server_open_variant(const char *const *grammars, int ngrammars, int device, int lanes,
+                                   uint64_t hbm_budget, uint64_t max_batch, int index32, int length_bits,
+                                   alpacc_hip_server **out);
 void alpacc_hip_server_close(alpacc_hip_server *srv);
 
 /* Asynchronous request (payload copied); *ticket names it for _wait. */

@@ -18,17 +18,28 @@ BUILD = os.path.join(HERE, "build")
 _libs = {}
 
 
-def ensure_built(grammar: str) -> None:
-    if all(os.path.exists(os.path.join(BUILD, f"lib{grammar}_{m}.so")) for m in ("lexer", "parser", "both")):
+def _sfx(index32: bool, length: int) -> str:
+    return ("_i32" if index32 else "") + (f"_l{length}" if length else "")
+
+
+def ensure_built(grammar: str, index32: bool = False, length: int = 0) -> None:
+    sfx = _sfx(index32, length)
+    modes = ("lexer", "both") if sfx else ("lexer", "parser", "both")
+    if all(os.path.exists(os.path.join(BUILD, f"lib{grammar}_{m}{sfx}.so")) for m in modes):
         return
-    subprocess.check_call([sys.executable, os.path.join(HERE, "build.py"), grammar])
+    if sfx:
+        sys.path.insert(0, HERE)
+        import build as oracle_build
+        oracle_build.build_variant(grammar, index32, length)
+    else:
+        subprocess.check_call([sys.executable, os.path.join(HERE, "build.py"), grammar])
 
 
-def _lib(grammar: str, mode: str):
-    key = (grammar, mode)
+def _lib(grammar: str, mode: str, index32: bool = False, length: int = 0):
+    key = (grammar, mode, index32, length)
     if key not in _libs:
-        ensure_built(grammar)
-        lib = C.CDLL(os.path.join(BUILD, f"lib{grammar}_{mode}.so"))
+        ensure_built(grammar, index32, length)
+        lib = C.CDLL(os.path.join(BUILD, f"lib{grammar}_{mode}{_sfx(index32, length)}.so"))
         lib.oracle_response.restype = C.c_int
         lib.oracle_response.argtypes = [C.c_uint64, C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64)]
         lib.oracle_free.argtypes = [C.c_void_p]
@@ -38,9 +49,10 @@ def _lib(grammar: str, mode: str):
     return _libs[key]
 
 
-def record(grammar: str, mode: str, payload: bytes, n: int = None) -> bytes:
-    """The oracle's response record for one request."""
-    lib = _lib(grammar, mode)
+def record(grammar: str, mode: str, payload: bytes, n: int = None, index32: bool = False, length: int = 0) -> bytes:
+    """The oracle's response record for one request (of a width variant:
+    index32, length bits)."""
+    lib = _lib(grammar, mode, index32, length)
     if n is None:
         n = len(payload)
     rec = C.c_void_p()

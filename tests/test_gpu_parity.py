@@ -461,6 +461,38 @@ def test_autotune_lexer():
         be.close()
 
 
+@pytest.mark.parametrize("index32,length", [(True, 0), (False, 8), (True, 64)])
+def test_width_variants(index32, length):
+    """Contexts opened as width variants (--index32, length) answer every
+    request as the oracle built with the same widths: single requests
+    (including tokens too long for a narrow length_t), a batch, the layout."""
+    import subprocess
+    from alpacc_amd import hip
+    for grammar in ("json", "arithmetic"):
+        be = hip.Backend(grammar, index32=index32, length=length)
+        try:
+            kv = dict(l.split("=") for l in be.layout_text("both").split())
+            assert int(kv["index_t"]) == (4 if index32 else 8)
+            long_tok = b'"' + b"a" * 300 + b'"' if grammar == "json" else b"x" * 300 + b"+1"
+            inputs = [datagen.generate(grammar, n, seed=n, conforming=n % 2 == 0).tobytes()
+                      for n in (0, 1, 17, 300, 5000, 60000, 1 << 20)] + [long_tok]
+            for mode in ("lexer", "both"):
+                for x in inputs:
+                    got = be.run_case(mode, x)
+                    exp = pyoracle.record(grammar, mode, x, index32=index32, length=length)
+                    assert first_diff(got, exp) is None, (grammar, mode, index32, length, len(x))
+                got = be.run_batch(mode, inputs)
+                exp = b"".join(pyoracle.record(grammar, mode, x, index32=index32, length=length) for x in inputs)
+                assert first_diff(got, exp) is None, (grammar, mode, "batch")
+        finally:
+            be.close()
+    cli = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "alpacc_amd", "lib", "alpacc-hip")
+    flags = (["--index32"] if index32 else []) + (["--length", str(length)] if length else [])
+    out = subprocess.run([cli, "-g", "json", "--layout"] + flags, capture_output=True, text=True, timeout=60).stdout
+    assert f"index_t={4 if index32 else 8}" in out
+    assert f"length_t={length // 8 if length else 2}" in out
+
+
 def test_batch_alone_and_in_mix(backends):
     """Each result is the same alone and in a mix: every input of a batch
     against run_case on that input."""

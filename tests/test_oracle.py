@@ -357,3 +357,60 @@ def test_parents_closed_form(analysed, name):
         assert r.nodes.tolist() == nodes
         assert r.parents.tolist() == tpar
         assert r.token_parents.tolist() == tokp
+
+
+# ---- width variants (--index32, length) ------------------------------------
+
+WIDTH_VARIANTS = [(True, 0), (False, 8), (True, 64)]
+
+
+def variant_grammar(name, index32, length):
+    g = tables.load(os.path.join(ROOT, "grammars", name + ".alp"), index32=index32)
+    if length:
+        g.length_bits = length
+    return g
+
+
+@pytest.mark.parametrize("grammar", ["json", "arithmetic"])
+@pytest.mark.parametrize("mode", ["lexer", "both"])
+@pytest.mark.parametrize("index32,length", WIDTH_VARIANTS)
+def test_width_variant_records(grammar, mode, index32, length):
+    """A width variant's record is the default build's record with index
+    and length fields re-encoded in the variant's widths, and a reject when
+    a kept token does not fit the variant's length_t (the rule of
+    backends/c/lexer.c:63-72 with length_t from Generator/C/Generator.hs:
+    36-53): checked against the default oracle, which the known-answer tests
+    above pin."""
+    from alpacc_amd import datagen
+    gd = variant_grammar(grammar, False, 0)
+    gv = variant_grammar(grammar, index32, length)
+    assert gv.index_bytes == (4 if index32 else 8)
+    assert gv.length_bytes == (length // 8 if length else (gd.length_bits // 8 if gd.length_bits else gv.index_bytes))
+    ld, lv = lex_layout(gd, mode), lex_layout(gv, mode)
+    max_len = (1 << (8 * lv.length)) - 1
+    inputs = [datagen.generate(grammar, n, seed=n, conforming=n % 2 == 0).tobytes() for n in (0, 1, 17, 300, 5000, 60000)]
+    long_tok = b'"' + b"a" * 300 + b'"' if grammar == "json" else b"x" * 300 + b"+1"
+    inputs += [long_tok, b"[" + long_tok + b"]" if grammar == "json" else b"(" + long_tok + b")"]
+    for x in inputs:
+        d = pyoracle.record(grammar, mode, x)
+        v = pyoracle.record(grammar, mode, x, index32=index32, length=length)
+        rec, end = wire.decode_record(d, 0, ld)
+        assert end == len(d)
+        if not rec.valid:
+            # the default rejects: the same text with wider lengths may not,
+            # but narrower ones never accept it
+            if lv.length <= ld.length:
+                assert v == b"\x00", (grammar, mode, len(x))
+            continue
+        exp = b"\x00" if len(rec.lengths) and int(rec.lengths.max()) > max_len else wire.encode_record(rec, lv)
+        assert v == exp, (grammar, mode, index32, length, len(x))
+
+
+def test_encode_record_inverts_decode():
+    for grammar, mode in (("json", "lexer"), ("json", "both"), ("arithmetic", "both"), ("sexp", "parser")):
+        g = variant_grammar(grammar, False, 0)
+        lay = lex_layout(g, mode)
+        payload = b"[1, 2, {}]" if grammar == "json" else (b"1+(2*a)" if grammar == "arithmetic" else bytes([0, 1]))
+        rec_bytes = pyoracle.record(grammar, mode, payload)
+        rec, _ = wire.decode_record(rec_bytes, 0, lay)
+        assert wire.encode_record(rec, lay) == rec_bytes
