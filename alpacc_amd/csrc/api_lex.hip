@@ -22,7 +22,7 @@ namespace alpacc {
 constexpr uint64_t LEX_MAX_INPUT = (1ull << 31) - 1;
 
 int lex_batch_on_device(Context& c, int mode, const uint8_t* d_in, const uint64_t* h_off, uint64_t count,
-                        hipStream_t st, const BatchCaps* caps, bool enqueue) {
+                        hipStream_t st, const BatchCaps* caps, bool enqueue, const LexStreamStep* stream) {
   const GrammarDesc* g = c.g;
   if (count == 0 || count > 0xFFFFFFFEull) return fail(ALPACC_ERR_ARG, "batch of %llu inputs", (unsigned long long)count);
   const uint64_t total = h_off[count] - h_off[0];
@@ -114,6 +114,14 @@ int lex_batch_on_device(Context& c, int mode, const uint8_t* d_in, const uint64_
   p.ntiles = (uint32_t)ntiles;
   p.ipt = c.lex_ipt;
   p.lb_win = c.lex_lb;
+  p.is_final = 1;
+  if (stream) {
+    p.carry_endo = stream->carry_endo;
+    p.carry_start = stream->carry_start;
+    p.start_bias = stream->start_bias;
+    p.is_final = stream->is_final;
+    p.stream_out = stream->out;
+  }
   if (!c.lex_grid) HIP_TRY(lex_grid_size(p, c.lex_wpc, &c.lex_grid, &c.lex_grid2));
   c.cp_meta_up = Context::Copy{dm, hm, L.up_bytes, hipMemcpyHostToDevice};
   if (!c.capturing) HIP_TRY(hipMemcpyAsync(dm, hm, L.up_bytes, hipMemcpyHostToDevice, st));
@@ -247,6 +255,87 @@ int alpacc_hip_lex_device(alpacc_hip_ctx* h, int mode, const uint8_t* d_str, uin
   d_out->starts = c.lx_start.as<int64_t>();
   d_out->lengths = c.lx_len.ptr;
   return r;
+}
+
+int alpacc_hip_lex_stream_begin(alpacc_hip_ctx* h, int mode) {
+  if (!h) return fail(ALPACC_ERR_ARG, "null argument");
+  if (mode != ALPACC_MODE_LEXER && mode != ALPACC_MODE_BOTH) return fail(ALPACC_ERR_ARG, "bad lexer mode %d", mode);
+  Context& c = h->c;
+  c.lstream = Context::LexStream{};
+  c.lstream.on = true;
+  c.lstream.mode = mode;
+  return 0;
+}
+
+int alpacc_hip_lex_stream_push(alpacc_hip_ctx* h, const uint8_t* d_chunk, uint64_t n, int is_last_chunk,
+                               alpacc_hip_lexemes* d_out, void* stream) {
+  if (!h || !d_out || (n && !d_chunk)) return fail(ALPACC_ERR_ARG, "null argument");
+  Context& c = h->c;
+  Context::LexStream& ls = c.lstream;
+  if (!ls.on) return fail(ALPACC_ERR_ARG, "no lexer stream begun on this context");
+  HIP_TRY(hipSetDevice(c.device));
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c.stream;
+  const GrammarDesc* g = c.g;
+  const bool lex_ids = ls.mode == ALPACC_MODE_LEXER;
+  const int lb = g->length_bytes;
+  const uint64_t max_len = lb >= 8 ? ~0ull : ((1ull << (8 * lb)) - 1);
+  d_out->count = 0;
+  d_out->terminals = c.lx_term.ptr;
+  d_out->starts = c.lx_start.as<int64_t>();
+  d_out->lengths = c.lx_len.ptr;
+  int r;
+  if (n) {
+    // the chunk as one input that continues the stream: its lanes start
+    // from the stream's state, its first token from the open one
+    if ((r = c.lx_stream.reserve(16))) return r;
+    uint32_t* words = c.lx_stream.as<uint32_t>();
+    const LexStreamStep step{ls.endo, ls.open_start - (int64_t)ls.pos, (int64_t)ls.pos, is_last_chunk ? 1 : 0, words};
+    const uint64_t off[2] = {0, n};
+    if ((r = lex_batch_on_device(c, ls.mode, d_chunk, off, 1, st, nullptr, true, &step)) < 0) return r;
+    uint32_t* hw = reinterpret_cast<uint32_t*>(c.host_small + 12);  // pinned
+    HIP_TRY(hipMemcpyAsync(hw, words, 12, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    ls.flags |= c.lb_flags[0];
+    ls.endo = ls.endo ? g->compose[(size_t)ls.endo * g->nendo + hw[0]] : hw[0];
+    if (hw[2]) ls.open_start = (int64_t)ls.pos + (int64_t)hw[2] - 1;
+    ls.pos += n;
+    d_out->count = c.lb_tok[1] - c.lb_tok[0];
+    d_out->terminals = c.lx_term.ptr;
+    d_out->starts = c.lx_start.as<int64_t>();
+    d_out->lengths = c.lx_len.ptr;
+  } else if (is_last_chunk && ls.pos) {
+    // the stream ended with the previous chunk: its open token and the
+    // final-state check, from the state its bytes lead to
+    const int state = g->endo_init[ls.endo];
+    const int t = (lex_ids ? g->token_lex : g->token_both)[state];
+    const int dead = lex_ids ? g->dead_lex : g->dead_both, ignore = lex_ids ? g->ignore_lex : g->ignore_both;
+    if (t == dead) {
+      ls.flags |= LEX_REJECT_FINAL;
+    } else if (t != ignore) {
+      const uint64_t len = ls.pos - (uint64_t)ls.open_start;
+      if (len > max_len) ls.flags |= LEX_REJECT_OVERFLOW;
+      if ((r = c.lx_term.reserve(16)) || (r = c.lx_start.reserve(16)) || (r = c.lx_len.reserve(16))) return r;
+      uint8_t* hb = reinterpret_cast<uint8_t*>(c.host_small + 12);  // pinned
+      const uint8_t term = (uint8_t)t;
+      const int64_t start = ls.open_start;
+      memcpy(hb, &term, 1);
+      memcpy(hb + 8, &start, 8);
+      memcpy(hb + 16, &len, (size_t)lb);  // little-endian: the low lb bytes
+      HIP_TRY(hipMemcpyAsync(c.lx_term.ptr, hb, 1, hipMemcpyHostToDevice, st));
+      HIP_TRY(hipMemcpyAsync(c.lx_start.ptr, hb + 8, 8, hipMemcpyHostToDevice, st));
+      HIP_TRY(hipMemcpyAsync(c.lx_len.ptr, hb + 16, (size_t)lb, hipMemcpyHostToDevice, st));
+      HIP_TRY(hipStreamSynchronize(st));
+      d_out->count = 1;
+      d_out->terminals = c.lx_term.ptr;
+      d_out->starts = c.lx_start.as<int64_t>();
+      d_out->lengths = c.lx_len.ptr;
+    }
+  }
+  if (is_last_chunk) {
+    ls.on = false;
+    return ls.flags ? ALPACC_REJECT : ALPACC_VALID;
+  }
+  return (ls.flags & LEX_REJECT_OVERFLOW) ? ALPACC_REJECT : ALPACC_VALID;
 }
 
 int alpacc_hip_set_lex_shape(alpacc_hip_ctx* h, const alpacc_hip_lex_shape* shape) {

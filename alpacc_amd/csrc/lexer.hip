@@ -259,7 +259,11 @@ __global__ __launch_bounds__(WG) void lex_fold(LexParams p) {
     // the lane's prefix from the tile start, or (bit 15) from its input's
     // start when that lies inside the tile
     p.lane_prefix[(uint64_t)tile * WG + tid] = (uint16_t)(ls.first ? 0x8000u : (xf ? 0x8000u | xe : xe));
-    if (tid == WG - 1) p.tile_endo[tile] = (wf | fi) << 31 | (fi ? ei : comp(we, ei));
+    if (tid == WG - 1) {
+      const uint32_t agg = (wf | fi) << 31 | (fi ? ei : comp(we, ei));
+      p.tile_endo[tile] = agg;
+      if (p.stream_out && tile == nt - 1) p.stream_out[1] = agg;  // for the launch's total
+    }
     __syncthreads();
   }
 }
@@ -342,7 +346,12 @@ __global__ __launch_bounds__(TSB) void lex_tile_scan3(LexParams p, uint32_t nblk
   const uint32_t t = blockIdx.x * TSB + threadIdx.x;
   if (t >= *p.nt_dev) return;
   const uint32_t w = p.tile_endo[t], carry = p.tile_blk[nblk + t / TSB];
-  p.tile_prefix[t] = (uint16_t)((w >> 31) ? (w & 0x7FFFFFFFu) : p.compose[carry * p.nendo + w]);
+  const uint32_t pre = (w >> 31) ? (w & 0x7FFFFFFFu) : p.compose[carry * p.nendo + w];
+  p.tile_prefix[t] = (uint16_t)pre;
+  if (p.stream_out && t == *p.nt_dev - 1) {  // the endomorphism of the launch's bytes
+    const uint32_t a = p.stream_out[1];
+    p.stream_out[0] = (a >> 31) ? (a & 0x7FFFFFFFu) : p.compose[pre * p.nendo + (a & 0x7FFFFFFFu)];
+  }
 }
 
 // ------------------------------------------------------------------- L3
@@ -380,9 +389,12 @@ __device__ __forceinline__ uint32_t lds_load16(uint32_t a) { return *(const lds_
 __device__ __forceinline__ void lds_store8(uint32_t a, uint32_t v) { *(lds_u8*)a = (uint8_t)v; }
 
 // DFA state of the lane's first byte.
+// DFA state of the lane's first byte (after the bytes of earlier launches
+// of a stream, carry_endo, when there are any).
 __device__ __forceinline__ uint32_t lane_state(const LexParams& p, uint32_t tile, uint32_t tid) {
   const uint32_t lp = p.lane_prefix[(uint64_t)tile * WG + tid];
-  const uint32_t pre = (lp & 0x8000u) ? lp & 0x7FFFu : p.compose[(uint32_t)p.tile_prefix[tile] * p.nendo + lp];
+  uint32_t pre = (lp & 0x8000u) ? lp & 0x7FFFu : p.compose[(uint32_t)p.tile_prefix[tile] * p.nendo + lp];
+  if (p.carry_endo) pre = p.compose[p.carry_endo * p.nendo + pre];
   return p.endo_init[pre];
 }
 
@@ -502,7 +514,8 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
 #pragma unroll
     for (int q = 0; q < IPT / 32; ++q) cnt += __builtin_popcount(km[q]);
     const bool wave_over = __ballot(cnt > rcap) != 0;
-    const bool owns_end = valid > 0 && base + valid == ls.seg_hi;
+    // the input's last byte (a launch that a stream continues has none)
+    const bool owns_end = valid > 0 && base + valid == ls.seg_hi && p.is_final;
     uint32_t fin_t = 0;
     bool fin_kept = false;
     if (owns_end) {
@@ -517,7 +530,7 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
     uint32_t last = 0;
 #pragma unroll
     for (int q = 0; q < IPT / 32; ++q)
-      if (bm[q]) last = (uint32_t)base + 32 * q + 31 - __builtin_clz(bm[q]);
+      if (bm[q]) last = (uint32_t)base + 32 * q + 32 - __builtin_clz(bm[q]);  // position + 1: 0 = none
 
     // block scan of (count, last boundary)
     uint32_t icnt = cnt, ilast = last;
@@ -598,9 +611,10 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
     // where the lane's first token starts: the last boundary of an earlier
     // lane, else (NO_START) the last boundary before the tile, known after
     // the lookback; never before the token's input's first byte (applied at
-    // the write-out, where a boundary of an earlier input gives way to it)
+    // the write-out, where a boundary of an earlier input gives way to it).
+    // Boundaries travel as position + 1, 0 meaning none.
     constexpr uint32_t NO_START = 0xFFFFFFFFu;
-    const uint32_t start_in = elast ? elast : NO_START;
+    const uint32_t start_in = elast ? elast - 1 : NO_START;
     uint64_t slot0 = 0;
     uint32_t carried = 0;
     bool overflow = false;
@@ -669,20 +683,28 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
         slot0 = w.pre_cnt;     // output slot of the tile's first kept token
         carried = w.pre_last;  // the last boundary before the tile
         if (ls.first) p.seg_tok[ls.seg] = slot0 + ecnt;
-        if (tid == 0 && tile == nt - 1) p.seg_tok[p.nseg] = slot0 + tc;
+        if (tid == 0 && tile == nt - 1) {
+          p.seg_tok[p.nseg] = slot0 + tc;
+          if (p.stream_out) p.stream_out[2] = carried > tl ? carried : tl;  // the launch's last boundary + 1
+        }
       }
       const uint32_t m = tc - lo < STAGE ? tc - lo : STAGE;
       const uint64_t max_len = p.max_len;
       if (!ts.multi()) {
-        // one input: a carried start never lies before its first byte
-        const uint32_t cs = carried > (uint32_t)ts.seg_lo ? carried : (uint32_t)ts.seg_lo;
+        // one input: a carried start never lies before its first byte; with
+        // no boundary since it, the token started there or, in a launch that
+        // continues a stream, where the stream's open token started
+        const int64_t lo_in = (int64_t)ts.seg_lo;
+        const int64_t cs = carried ? ((int64_t)carried - 1 > lo_in ? (int64_t)carried - 1 : lo_in)
+                                   : lo_in + p.carry_start;
         for (uint32_t i = tid; i < m; i += WG) {
           const uint64_t slot = slot0 + lo + i;
-          const uint32_t b0 = s_start[i], b = b0 == NO_START ? cs : b0;
-          const uint64_t len = tbase + s_end[i] - b;
+          const uint32_t b0 = s_start[i];
+          const int64_t b = b0 == NO_START ? cs : (int64_t)b0;
+          const uint64_t len = (uint64_t)((int64_t)tbase + s_end[i] - b);
           overflow |= len > max_len;
           p.terminals[slot] = s_term[i];
-          p.starts[slot] = (int64_t)(b - ts.seg_lo);
+          p.starts[slot] = b - lo_in + p.start_bias;
           store_length(p, slot, len);
         }
       } else {
@@ -699,7 +721,7 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
             seg_lo = tbase + s_istart[d];
           }
           const uint32_t b0 = s_start[i];
-          uint64_t b = b0 == NO_START ? carried : b0;
+          uint64_t b = b0 == NO_START ? (carried ? carried - 1 : 0u) : b0;
           b = b > seg_lo ? b : seg_lo;
           const uint64_t len = tbase + e - b;
           if (len > max_len) atomicOr(p.seg_flags + seg, LEX_REJECT_OVERFLOW);

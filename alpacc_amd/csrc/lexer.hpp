@@ -63,6 +63,17 @@ struct LexParams {
   // shape: input bytes per lane (64 or 128: the chunk) and predecessor
   // tiles per lookback probe (1..64)
   int ipt, lb_win;
+  // a launch of one input that continues a stream (zero / one / null
+  // otherwise): carry_endo = the endomorphism of the stream's earlier bytes,
+  // carry_start = where its open token started (relative to this launch,
+  // <= 0), start_bias added to every start written, is_final = the stream
+  // ends with this launch (its last token and the final-state check),
+  // stream_out[0] = the endomorphism of this launch's bytes, [2] = its last
+  // boundary + 1 (0: none), [1] scratch
+  uint32_t carry_endo;
+  int64_t carry_start, start_bias;
+  int is_final;
+  uint32_t* stream_out;
 };
 
 __device__ __forceinline__ void store_length(const LexParams& p, uint64_t slot, uint64_t len) {

@@ -135,6 +135,17 @@ struct Context {
   uint64_t lex_launches = 0, parse_launches = 0;
   KernelTimer timer;
 
+  // a lexer stream (alpacc_hip_lex_stream_*): the endomorphism of its bytes
+  // so far, their count, where its open token started, its LEX_REJECT_* bits
+  struct LexStream {
+    bool on = false;
+    int mode = 0;
+    uint32_t endo = 0;
+    uint64_t pos = 0;
+    int64_t open_start = 0;
+    uint32_t flags = 0;
+  } lstream;
+  DevBuf lx_stream;  // [4] words a stream's launch reports
   // the combined lex -> parse -> tree sequence captured for a batch
   // capacity (graph_caps), replayed for batches within it that read the
   // same inputs on the same stream while its buffers stay where they were
@@ -169,8 +180,8 @@ struct Context {
   uint64_t graph_captures = 0, graph_replays = 0;
 
   std::vector<DevBuf*> all_buffers() {
-    return {&small, &lx_term, &lx_start, &lx_len, &lx_status, &in_copy, &ps_a,      &ps_b,      &ps_c,   &ps_d,
-            &ps_e,  &ps_f,    &ps_g,     &ps_h,   &ps_i,      &tr_prod, &tr_parent, &tr_tokpar, &lx_meta, &pmeta};
+    return {&small, &lx_term, &lx_start, &lx_len,    &lx_status, &in_copy, &ps_a,      &ps_b,     &ps_c,  &ps_d, &ps_e,
+            &ps_f,  &ps_g,    &ps_h,     &ps_i,      &tr_prod,   &tr_parent, &tr_tokpar, &lx_meta, &pmeta, &lx_stream};
   }
   // allocations of the buffers that the captured sequence uses
   uint64_t graph_buffers_gen() const {
@@ -197,8 +208,18 @@ const char* const* grammar_names();
 // caps, the host copies are written when the stream reaches them.  Without
 // `enqueue` only the host metadata of the batch is written (the copy that
 // a captured sequence uploads).
+// One launch of a stream (alpacc_hip_lex_stream_push): the fields of
+// LexParams that carry the stream's earlier bytes.
+struct LexStreamStep {
+  uint32_t carry_endo;
+  int64_t carry_start, start_bias;
+  int is_final;
+  uint32_t* out;  // [4] device words
+};
+
 int lex_batch_on_device(Context& c, int mode, const uint8_t* d_in, const uint64_t* h_off, uint64_t count,
-                        hipStream_t st, const BatchCaps* caps = nullptr, bool enqueue = true);
+                        hipStream_t st, const BatchCaps* caps = nullptr, bool enqueue = true,
+                        const LexStreamStep* stream = nullptr);
 
 // Wire records of a batch of host payloads, concatenated into one malloc'd
 // buffer; rec_off (if not null) receives the start of every input's record.

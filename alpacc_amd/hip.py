@@ -107,6 +107,8 @@ def load_library(path: str = LIB_PATH):
         "alpacc_hip_graph_stats": (i32, [vp, C.POINTER(u64), C.POINTER(u64)]),
         "alpacc_hip_server_set_graphs": (i32, [vp, i32]),
         "alpacc_hip_set_lex_shape": (i32, [vp, C.POINTER(LexShape)]),
+        "alpacc_hip_lex_stream_begin": (i32, [vp, i32]),
+        "alpacc_hip_lex_stream_push": (i32, [vp, vp, u64, i32, C.POINTER(Lexemes), vp]),
         "alpacc_hip_open_variant": (i32, [C.c_char_p, i32, i32, i32, C.POINTER(vp)]),
         "alpacc_hip_variant_layout": (i32, [C.c_char_p, i32, i32, i32, C.c_char_p, C.c_size_t]),
         "alpacc_hip_server_open_variant": (i32, [C.POINTER(C.c_char_p), i32, i32, i32, u64, u64, i32, i32,
@@ -153,6 +155,22 @@ def _take(ptr, nbytes: int, dtype) -> np.ndarray:
     arr = np.frombuffer(bytes(buf), dtype=dtype).copy()
     load_library().alpacc_hip_free(ptr)
     return arr
+
+
+class _DevBytes:
+    """A device byte range as a __cuda_array_interface__ object."""
+
+    def __init__(self, ptr: int, n: int):
+        self.__cuda_array_interface__ = {"shape": (n,), "typestr": "|u1", "data": (ptr, False), "version": 2,
+                                         "strides": None}
+
+
+def _take_dev(ptr: int, nbytes: int) -> bytes:
+    """Bytes of device memory at ptr (copied to the host)."""
+    import torch
+    if nbytes == 0:
+        return b""
+    return torch.as_tensor(_DevBytes(ptr, nbytes), device="cuda").cpu().numpy().tobytes()
 
 
 class Backend:
@@ -272,6 +290,39 @@ class Backend:
             parents=_take(out.parents, N * 8, "<i8"),
             token_parents=_take(out.token_parents, T * 8, "<i8"),
         )
+
+    # -- lexing a stream of chunks -------------------------------------------
+    def lex_stream(self, chunks, mode: str = "lexer") -> bytes:
+        """Lex one input given as chunks (staged one at a time in device
+        memory) with alpacc_hip_lex_stream_*; returns the lexer-only wire
+        record of the whole input built from the pushes' tokens."""
+        import torch
+        lay = self.layouts["lexer" if mode == "lexer" else "both"]
+        _check(self.lib.alpacc_hip_lex_stream_begin(self.h, MODES[mode]) >= 0 and 1 or -1)
+        terms, starts, lens = [], [], []
+        ok = True
+        chunks = list(chunks)
+        for i, ch in enumerate(chunks):
+            last = i == len(chunks) - 1
+            dev = torch.frombuffer(bytearray(ch + b"\0"), dtype=torch.uint8).to("cuda")
+            torch.cuda.synchronize()
+            out = Lexemes()
+            rc = self.lib.alpacc_hip_lex_stream_push(self.h, C.c_void_p(dev.data_ptr()), len(ch), int(last),
+                                                     C.byref(out), None)
+            ok = _check(rc)
+            T = out.count
+            if T:
+                terms.append(_take_dev(out.terminals, T * lay.terminal))
+                starts.append(_take_dev(out.starts, T * 8))
+                lens.append(_take_dev(out.lengths, T * lay.length))
+            del dev
+        if not ok:
+            return b"\x00"
+        t = np.frombuffer(b"".join(terms), dtype=np.uint8)
+        s = np.frombuffer(b"".join(starts), dtype="<i8")
+        l = np.frombuffer(b"".join(lens), dtype=f"<u{lay.length}")
+        rec = wire.Record(True, terminals=t, starts=s, lengths=l)
+        return wire.encode_record(rec, wire.Layout("lexer", lay.terminal, lay.index, lay.length))
 
     # -- lexer launch shape ----------------------------------------------------
     def set_lex_shape(self, items_per_thread: int = 128, workgroups_per_cu: int = 0, lookback_window: int = 16):

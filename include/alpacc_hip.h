@@ -165,6 +165,23 @@ int alpacc_hip_parse_device(alpacc_hip_ctx *ctx, const void *d_tokens, uint64_t 
 int alpacc_hip_lex_parse_device(alpacc_hip_ctx *ctx, const uint8_t *d_str, uint64_t n,
                                 alpacc_hip_tree *d_out, void *stream);
 
+/* Lexing one input that arrives in chunks (the reference's chunked CUDA
+ * lexer, LexerCtx and its update between chunks, backends/cuda/lexer.cu:
+ * 224-431, and lexer.fut:75-196): begin, then push the chunks in order, the
+ * last with is_last_chunk = 1 (it may be empty).  Each push lexes its chunk
+ * in device memory from the state the earlier chunks left and returns, in
+ * d_out (device arrays of the context, valid until its next call), the
+ * tokens that close inside it; starts are offsets in the whole stream and a
+ * token may start in an earlier chunk.  Concatenated, the pushes' tokens
+ * are lex_string's for the whole input.  Returns 1 while (or, after the
+ * last chunk, if) the input is accepted, 0 once it is rejected (a token
+ * longer than length_t; after the last chunk also a final state that is not
+ * accepting), < 0 on errors.  Chunks take fewer than 2^31 bytes each; the
+ * stream has no limit. */
+int alpacc_hip_lex_stream_begin(alpacc_hip_ctx *ctx, int mode);
+int alpacc_hip_lex_stream_push(alpacc_hip_ctx *ctx, const uint8_t *d_chunk, uint64_t n, int is_last_chunk,
+                               alpacc_hip_lexemes *d_out, void *stream);
+
 /* ---- batches -------------------------------------------------------------
  * `count` inputs packed back to back: input i is bytes [offsets[i],
  * offsets[i + 1]) of the packed buffer (offsets: a HOST array of count + 1

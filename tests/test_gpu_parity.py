@@ -493,6 +493,48 @@ def test_width_variants(index32, length):
     assert f"length_t={length // 8 if length else 2}" in out
 
 
+@pytest.mark.parametrize("grammar", ["json", "arithmetic", "sexp"])
+def test_lex_stream(backends, grammar):
+    """One input pushed as chunks (cut at random points: inside tokens, at
+    boundaries, 1-byte and empty chunks, an empty last chunk) gives the
+    tokens of the whole input: the oracle's lexer record, also for rejected
+    inputs and tokens spanning many chunks."""
+    rng = random.Random(zlib.crc32(f"stream {grammar}".encode()))
+    be = backends[grammar]
+    texts = [datagen.generate(grammar, n, seed=n, conforming=n % 2 == 0).tobytes() for n in (1, 2, 100, 5000, 200000)]
+    texts += [b"", b" " * 5000]
+    if grammar == "json":
+        texts += [b'"' + b"a" * 70000 + b'"', b"[" + b"1," * 30000 + b"1]", b'["' + b"x" * 40000 + b'", 1]']
+    for x in texts:
+        for trial in range(4):
+            cuts = sorted(rng.randrange(0, len(x) + 1) for _ in range(rng.choice([0, 1, 3, 12])))
+            if trial == 3 and len(x) <= 400:
+                cuts = list(range(len(x) + 1))  # one byte per chunk
+            pieces = [x[a:b] for a, b in zip([0] + cuts, cuts + [len(x)])]
+            if trial % 2:
+                pieces.append(b"")  # the end told by an empty last chunk
+            for mode in ("lexer", "both"):
+                got = be.lex_stream(pieces, mode)
+                exp = pyoracle.record(grammar, mode, x) if mode == "lexer" else _lexer_part(grammar, x)
+                if exp is None:  # lexed but not parsed: no combined-numbering tokens to compare
+                    continue
+                assert first_diff(got, exp) is None, (grammar, mode, len(x), [len(p) for p in pieces][:20])
+
+
+def _lexer_part(grammar, x):
+    """The combined-mode tokens of x as a lexer-only record (the combined
+    oracle record's token arrays), or the reject record."""
+    from alpacc_amd import wire
+    g_lay = wire.Layout("both", 1, 8, 2 if grammar in ("json", "sexp") else 8, 1)
+    rec = pyoracle.record(grammar, "both", x)
+    r, _ = wire.decode_record(rec, 0, g_lay)
+    if not r.valid:
+        lex = pyoracle.record(grammar, "lexer", x)
+        return b"\x00" if lex[0] == 0 else None
+    return wire.encode_record(wire.Record(True, terminals=r.terminals, starts=r.starts, lengths=r.lengths),
+                              wire.Layout("lexer", 1, 8, g_lay.length))
+
+
 def test_batch_alone_and_in_mix(backends):
     """Each result is the same alone and in a mix: every input of a batch
     against run_case on that input."""
