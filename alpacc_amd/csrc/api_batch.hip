@@ -3,8 +3,9 @@
 // The reference answers a counted batch one request at a time (the
 // generated run_case per frame, Generator/C/Generator.hs:64-152, driven by
 // backends/c/cli.c:167-189).  Here the inputs of a batch are packed back to
-// back, every stage runs once over all of them with per-input tiles, and
-// the records are cut from the batch's outputs afterwards: each record is
+// back, every stage runs once over all of them (the lexer's tiles shared by
+// the inputs' chunks, the parser's per input), and the records are cut
+// from the batch's outputs afterwards: each record is
 // byte-identical to what alpacc_hip_run_case returns for that input alone.
 #include <hip/hip_runtime.h>
 

@@ -186,7 +186,9 @@ int alpacc_hip_lex_stream_push(alpacc_hip_ctx *ctx, const uint8_t *d_chunk, uint
  * `count` inputs packed back to back: input i is bytes [offsets[i],
  * offsets[i + 1]) of the packed buffer (offsets: a HOST array of count + 1
  * entries; inputs need no alignment).  The whole batch runs as one launch
- * sequence with per-input tiles.  The reference answers a counted batch one
+ * sequence: the lexer packs the inputs into shared tiles (128-byte chunks
+ * per input), the parser gives every input tiles of its own.  The reference
+ * answers a counted batch one
  * request at a time (generated run_case per frame, Generator/C/
  * Generator.hs:64-152, driven by backends/c/cli.c:167-189); these calls
  * replace that loop. */
