@@ -34,9 +34,8 @@
 //                    boundary before it), stages (terminal, start, end) in
 //                    LDS by slot, and the workgroup writes the
 //                    structure-of-arrays output with consecutive lanes on
-//                    consecutive slots.  A tile with a lane of more kept
-//                    tokens than the LDS holds terminals for re-walks its
-//                    bytes instead.
+//                    consecutive slots.  A lane with more kept tokens than
+//                    the LDS holds terminals for re-walks its bytes instead.
 #include "device.hpp"
 #include "lexer.hpp"
 
@@ -403,7 +402,7 @@ __device__ __forceinline__ uint32_t lane_state(const LexParams& p, uint32_t tile
 // row offset is an LDS address).
 struct EmitWords {
   uint32_t tile, pre_cnt, pre_last, pad;
-  uint32_t wave_cnt[NW], wave_last[NW], wave_over[NW];
+  uint32_t wave_cnt[NW], wave_last[NW];
 };
 
 // LDS layout of lex_emit: DFA rows, the staged slots, the lanes' kept-token
@@ -513,7 +512,8 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
     uint32_t cnt = 0;  // kept tokens closed inside the lane
 #pragma unroll
     for (int q = 0; q < IPT / 32; ++q) cnt += __builtin_popcount(km[q]);
-    const bool wave_over = __ballot(cnt > rcap) != 0;
+    // more kept tokens than walk 1 kept terminals for: this lane re-walks
+    const bool lane_over = cnt > rcap;
     // the input's last byte (a launch that a stream continues has none)
     const bool owns_end = valid > 0 && base + valid == ls.seg_hi && p.is_final;
     uint32_t fin_t = 0;
@@ -538,7 +538,6 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
     if (lane == WAVE - 1) {
       w.wave_cnt[wid] = icnt;
       w.wave_last[wid] = ilast;
-      w.wave_over[wid] = wave_over;
     }
     __syncthreads();
     // walk 1 is over, its sink rows free until the next tile: the byte
@@ -546,7 +545,6 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
     // round's write-out on
     if (ts.multi()) input_starts(p, ts, s_istart);
     uint32_t wc = 0, wl = 0, tc = 0, tl = 0;
-    bool over = false;
     for (uint32_t i = 0; i < NW; ++i) {
       if (i < wid) {
         wc += w.wave_cnt[i];
@@ -554,7 +552,6 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
       }
       tc += w.wave_cnt[i];
       tl = tl > w.wave_last[i] ? tl : w.wave_last[i];
-      over |= w.wave_over[i] != 0;
     }
     const uint32_t pc = __shfl_up(icnt, 1, WAVE), pl = __shfl_up(ilast, 1, WAVE);
     const uint32_t ecnt = wc + (lane ? pc : 0u);
@@ -629,7 +626,7 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
       if (cnt > 0 && ecnt < hi && ecnt + cnt > lo) {
         const uint32_t d = ecnt - lo;  // staging index of the lane's first token (mod 2^32)
         uint32_t k = 0, start = start_in;
-        if (!over) {
+        if (!lane_over) {
           // kept token k closes before byte j (bit j of km) and starts at the
           // lane's last boundary before j, else where the lane's first token starts
 #pragma unroll
@@ -651,22 +648,29 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
             start = pm ? qb + 31 - __builtin_clz(pm) : start;
           }
         } else {
-          // some lane of the tile closes more than rcap kept tokens: re-walk
-          // the bytes through the DFA
+          // the lane closes more kept tokens than walk 1 kept terminals
+          // for: re-walk its bytes through the DFA, four per dword load
+          // (only the dwords holding its bytes, as in walk 1)
           uint32_t r = row0;
-          for (uint32_t j = 0; j < valid; ++j) {
-            const uint32_t x = lds_load16(r + 2 * p.in[base + j]);
-            if ((x >> RC::KEPT) & 1u) {
-              const uint32_t i = d + k;
-              if (i < STAGE) {
-                s_term[i] = (uint8_t)lds_load16(r + 512);
-                s_start[i] = start;
-                s_end[i] = (uint16_t)(lane_off + j);
+          const uint32_t* src = reinterpret_cast<const uint32_t*>(p.in + base - ls.shift);
+          for (uint32_t k4 = 0; 4 * k4 < valid; ++k4) {
+            const uint32_t lo_w = src[k4], hi_w = 4 * k4 + 4 < valid + ls.shift ? src[k4 + 1] : 0u;
+            const uint32_t wk = __builtin_amdgcn_alignbyte(hi_w, lo_w, ls.shift);
+            for (uint32_t b = 0; b < 4 && 4 * k4 + b < valid; ++b) {
+              const uint32_t j = 4 * k4 + b;
+              const uint32_t x = lds_load16(r + 2 * ((wk >> (8 * b)) & 0xFFu));
+              if ((x >> RC::KEPT) & 1u) {
+                const uint32_t i = d + k;
+                if (i < STAGE) {
+                  s_term[i] = (uint8_t)lds_load16(r + 512);
+                  s_start[i] = start;
+                  s_end[i] = (uint16_t)(lane_off + j);
+                }
+                ++k;
               }
-              ++k;
+              start = ((x >> RC::PROD) & 1u) ? (uint32_t)base + j : start;
+              r = RC::next_row(x, lds0);
             }
-            start = ((x >> RC::PROD) & 1u) ? (uint32_t)base + j : start;
-            r = RC::next_row(x, lds0);
           }
         }
         if (fin_kept) {
