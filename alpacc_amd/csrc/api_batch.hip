@@ -9,6 +9,7 @@
 // byte-identical to what alpacc_hip_run_case returns for that input alone.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -62,6 +63,15 @@ int lex_parse_batch_graph(Context& c, const uint8_t* d_in, const uint64_t* off, 
   if (c.graph) {  // a larger capacity already captured serves this batch too
     caps.count = caps.count > c.graph_caps.count ? caps.count : c.graph_caps.count;
     caps.bytes = caps.bytes > c.graph_caps.bytes ? caps.bytes : c.graph_caps.bytes;
+  }
+  // a capacity beyond the launch limits (lexer bytes, the parser's 31-bit
+  // depths at worst-case positions) runs eagerly instead
+  const GrammarDesc* g = c.g;
+  const uint64_t per_pos = (uint64_t)std::max(1, std::max(g->max_bpush, g->max_spush));
+  if (caps.bytes > 0x7FFFFFFFull || (caps.bytes + 2 * caps.count) * per_pos + caps.count > 0x7FFFFFFFull) {
+    int r = lex_batch_on_device(c, ALPACC_MODE_BOTH, d_in, off, count, st);
+    if (r < 0) return r;
+    return parse_batch_on_device(c, c.lx_term.as<uint8_t>(), c.d_lb_tok, c.lb_tok, c.lb_flags, count, true, st);
   }
   std::vector<uint64_t> padded(caps.count + 1);
   for (uint64_t i = 0; i <= caps.count; ++i) padded[i] = off[i < count ? i : count];
