@@ -318,7 +318,7 @@ def main(argv=None):
                     help="summarise rocprofv3 FETCH_SIZE/WRITE_SIZE runs into OUT (no GPU)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported by this build "
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported "
               f"(the lex/parse path runs on a single MI355X)", file=sys.stderr)
         return 2
     if a.pmc_summary:
