@@ -535,6 +535,27 @@ def _lexer_part(grammar, x):
                               wire.Layout("lexer", 1, 8, g_lay.length))
 
 
+def test_sexp_256mib_and_flip_variants(backends):
+    """The sexp workload at full size: one 256 MiB input (geometric nesting,
+    1% of lists 10,000 deep) lexed and parsed as the oracle does, and 1,000
+    variants of a 64 KiB sample with one random byte changed, as one batch:
+    every verdict and tree as the oracle's."""
+    n = 256 << 20
+    data = datagen.generate("sexp", n, seed=2, conforming=True).tobytes()
+    print(f"case sexp/both n={n} full size", flush=True)
+    check(backends, "sexp", "both", data, "256 MiB")
+    rng = random.Random(2)
+    sample = data[:1 << 16]
+    flips = []
+    for _ in range(1000):
+        b = bytearray(sample)
+        b[rng.randrange(len(b))] = rng.randrange(256)
+        flips.append(bytes(b))
+    got = backends["sexp"].run_batch("both", flips)
+    exp = b"".join(pyoracle.record("sexp", "both", x) for x in flips)
+    assert first_diff(got, exp) is None, "sexp flip variants"
+
+
 def test_batch_alone_and_in_mix(backends):
     """Each result is the same alone and in a mix: every input of a batch
     against run_case on that input."""
