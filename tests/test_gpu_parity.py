@@ -556,6 +556,17 @@ def test_sexp_256mib_and_flip_variants(backends):
     assert first_diff(got, exp) is None, "sexp flip variants"
 
 
+def test_json_1gib_lexer_only(backends):
+    """The JSON lexer-only workload at full size: 1 GiB (seed 1) lexed with
+    the lexer-only numbering, record identical to the oracle's (1.5 GB)."""
+    n = 1 << 30
+    data = datagen.generate("json", n, seed=1, conforming=True).tobytes()
+    print(f"case json/lexer n={n} full size", flush=True)
+    got = backends["json"].run_case("lexer", data)
+    exp = pyoracle.record("json", "lexer", data)
+    assert len(got) == len(exp) and got == exp, "json 1 GiB lexer-only record"
+
+
 def test_batch_alone_and_in_mix(backends):
     """Each result is the same alone and in a mix: every input of a batch
     against run_case on that input."""
