@@ -60,7 +60,8 @@ def record(grammar: str, mode: str, payload: bytes, n: int = None, index32: bool
     buf = C.create_string_buffer(payload, len(payload) + 1)
     if lib.oracle_response(n, buf, C.byref(rec), C.byref(ln)) != 0:
         raise RuntimeError("oracle_response failed")
-    out = C.string_at(rec.value, ln.value)
+    # a buffer copy: string_at takes a C int size (records can pass 2 GiB)
+    out = bytes((C.c_char * ln.value).from_address(rec.value)) if ln.value else b""
     lib.oracle_free(rec)
     return out
 

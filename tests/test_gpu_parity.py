@@ -567,6 +567,18 @@ def test_json_1gib_lexer_only(backends):
     assert len(got) == len(exp) and got == exp, "json 1 GiB lexer-only record"
 
 
+def test_json_1gib_lex_parse(backends):
+    """The measured workload at full size: 1 GiB of JSON (seed 1) lexed and
+    parsed, tokens and tree identical to the oracle's (about 3.8 GB of
+    record)."""
+    n = 1 << 30
+    data = datagen.generate("json", n, seed=1, conforming=True, wrap=True).tobytes()
+    print(f"case json/both n={n} full size", flush=True)
+    got = backends["json"].run_case("both", data)
+    exp = pyoracle.record("json", "both", data)
+    assert got[0] == 1 and len(got) == len(exp) and got == exp, "json 1 GiB lex+parse record"
+
+
 def test_batch_alone_and_in_mix(backends):
     """Each result is the same alone and in a mix: every input of a batch
     against run_case on that input."""
