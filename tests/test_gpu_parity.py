@@ -579,6 +579,21 @@ def test_json_1gib_lex_parse(backends):
     assert got[0] == 1 and len(got) == len(exp) and got == exp, "json 1 GiB lex+parse record"
 
 
+def test_batching_workload_full(backends):
+    """The batching workload at full size: 20,000 lognormal JSON inputs
+    (bench.py's batch, seed 3) in one batched call, every record identical
+    to the oracle's for that input alone."""
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    host, offs = bench.batch_inputs(20000, 3)
+    payloads = [host[int(offs[i]):int(offs[i + 1])].tobytes() for i in range(len(offs) - 1)]
+    print(f"batch json/both count={len(payloads)} bytes={int(offs[-1])} full size", flush=True)
+    got = backends["json"].run_batch("both", payloads)
+    exp = b"".join(pyoracle.record("json", "both", x) for x in payloads)
+    d = first_diff(got, exp)
+    assert d is None, f"batching workload differs at byte {d}"
+
+
 def test_batch_alone_and_in_mix(backends):
     """Each result is the same alone and in a mix: every input of a batch
     against run_case on that input."""
