@@ -313,6 +313,9 @@ __global__ __launch_bounds__(AWG) void seg_profile(Seg s, TileRec* rec, uint16_t
 struct ScanOut {
   int64_t np, nt, bend, cend;  // totals: slots, terminal productions, bracket depth, slot depth
   int64_t ob, ub, os, us;      // list lengths
+  // largest per-tile counts: nodes (slots less terminal productions),
+  // terminal productions, slots; they size seg_resolve's LDS staging
+  uint32_t tmax_node, tmax_term, tmax_slot, pad;
 };
 
 // The eight running sums over tiles: slots, terminal productions, bracket
@@ -358,14 +361,24 @@ __device__ __forceinline__ void block_scan_nv(int64_t* v, int64_t* tot) {
 
 __device__ __forceinline__ bool is_first(const Seg& s, uint32_t t) { return s.tile_first[s.tile_seg[t]] == t; }
 
-__global__ __launch_bounds__(256) void tile_block_sums(Seg s, const TileRec* rec, int64_t* bsum) {
+__global__ __launch_bounds__(256) void tile_block_sums(Seg s, const TileRec* rec, int64_t* bsum, ScanOut* out) {
   int64_t v[NV] = {0, 0, 0, 0, 0, 0, 0, 0};
+  uint32_t mx[3] = {0, 0, 0};  // per-tile maxima: nodes, terminal productions, slots
   const uint32_t t0 = blockIdx.x * SB + threadIdx.x * (SB / 256), nt = *s.nt_dev;
   for (int i = 0; i < SB / 256; ++i) {
     if (t0 + i >= nt) break;
+    const TileRec r = rec[t0 + i];
     int64_t x[NV];
-    tile_terms(rec[t0 + i], is_first(s, t0 + i), x);
+    tile_terms(r, is_first(s, t0 + i), x);
     for (int k = 0; k < NV; ++k) v[k] += x[k];
+    mx[0] = max(mx[0], (uint32_t)(r.np - r.nt));
+    mx[1] = max(mx[1], (uint32_t)r.nt);
+    mx[2] = max(mx[2], (uint32_t)r.np);
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    for (int o = WAVE / 2; o > 0; o >>= 1) mx[k] = max(mx[k], (uint32_t)__shfl_xor((int)mx[k], o, WAVE));
+    if ((threadIdx.x & (WAVE - 1)) == 0 && mx[k]) atomicMax(&out->tmax_node + k, mx[k]);
   }
   int64_t tot[NV];
   block_scan_nv(v, tot);
@@ -389,7 +402,16 @@ __global__ __launch_bounds__(256) void tile_block_scan(int64_t* bsum, uint32_t n
     if (threadIdx.x < NV) carry[threadIdx.x] += tot[threadIdx.x];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out = ScanOut{carry[0], carry[1], carry[2], carry[3], carry[4], carry[5], carry[6], carry[7]};
+  if (threadIdx.x == 0) {  // the maxima are tile_block_sums' own
+    out->np = carry[0];
+    out->nt = carry[1];
+    out->bend = carry[2];
+    out->cend = carry[3];
+    out->ob = carry[4];
+    out->ub = carry[5];
+    out->os = carry[6];
+    out->us = carry[7];
+  }
 }
 
 __global__ __launch_bounds__(256) void tile_downsweep(Seg s, const TileRec* rec, const int64_t* bsum, TileInfo* info,
@@ -1035,8 +1057,11 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
     return fail(ALPACC_ERR_ARG, "batch beyond the capacity its launch sequence is sized for");
   const int ncap = PT * g->max_prods, tcap = PT * g->max_nt;
   const int bcap = APL * (g->max_bpop + g->max_bpush), scap = APL * (g->max_spop + g->max_spush);
-  const size_t lds_b = (size_t)bcap * RWG * 2 + ((size_t)scap * RWG + 1) / 2 * 2 + (size_t)(ncap + tcap) * 2 +
-                       (size_t)ncap + 16;
+  // seg_resolve's dynamic LDS with node / terminal staging for n / t entries
+  auto resolve_lds = [&](int n, int t) {
+    return (size_t)bcap * RWG * 2 + ((size_t)scap * RWG + 1) / 2 * 2 + (size_t)(n + t) * 2 + (size_t)n + 16;
+  };
+  const size_t lds_b = resolve_lds(ncap, tcap);
   // tile-local depths are int16, compact indices u16, lane-relative slot
   // owners 7 bits
   if ((int64_t)PT * 8 > 0x7FFF || ncap > 0xFFFF || APL * g->max_prods > 0x7F || lds_b > 96 * 1024)
@@ -1158,7 +1183,7 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
                          reinterpret_cast<uint16_t*>(base + off_pid));
     if (!caps) HIP_TRY(tm.end(st));
     if (!caps) HIP_TRY(tm.begin(st, "tile_scan"));
-    hipLaunchKernelGGL(tile_block_sums, dim3(nsb), dim3(256), 0, st, s, rec, bsum);
+    hipLaunchKernelGGL(tile_block_sums, dim3(nsb), dim3(256), 0, st, s, rec, bsum, sout);
     hipLaunchKernelGGL(tile_block_scan, dim3(1), dim3(256), 0, st, bsum, nsb, sout);
     hipLaunchKernelGGL(tile_downsweep, dim3(nsb), dim3(256), 0, st, s, rec, bsum, info, sb, mb, ms);
     if (!caps) HIP_TRY(tm.end(st));
@@ -1196,8 +1221,15 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
         (r = build_tile_tree(ms, ntiles, reinterpret_cast<int32_t*>(base + off_tsx), st, &ts)))
       return r;
     if (!caps) HIP_TRY(tm.end(st));
+    // staging sized to the largest tile when the scan's maxima are on the
+    // host (fewer LDS bytes, more workgroups per CU), else to the grammar's
+    // per-position maxima
+    const int n_st = caps ? ncap : (int)(tree ? so.tmax_node : so.tmax_slot);
+    const int t_st = caps ? tcap : (tree ? (int)so.tmax_term : 0);
+    if (n_st > ncap || t_st > tcap) return fail(ALPACC_ERR_LIMIT, "tile counts beyond the grammar's maxima");
+    const size_t lds_r = resolve_lds(n_st, t_st);
     Resolve rs{s,         reinterpret_cast<const uint16_t*>(base + off_pid),
-               info,      sb,       bcap,  scap, ncap, tcap,
+               info,      sb,       bcap,  scap, n_st, t_st,
                ob_list,   ub_list,   os_list,  us_list,       c.tr_prod.as<uint8_t>(),
                c.tr_parent.as<int64_t>(), c.tr_tokpar.as<int64_t>(), tree ? nullptr : c.ps_d.as<uint8_t>(),
                tree ? 1 : 0};
@@ -1219,7 +1251,7 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
       c.resolve_lds_set = lds_b;
     }
     if (!caps) HIP_TRY(tm.begin(st, "seg_resolve"));
-    hipLaunchKernelGGL(kern[caps ? 1 : 0], dim3(g_res), dim3(RWG), lds_b, st, rs);
+    hipLaunchKernelGGL(kern[caps ? 1 : 0], dim3(g_res), dim3(RWG), lds_r, st, rs);
     if (!caps) HIP_TRY(tm.end(st));
     Far fr{s, info, sb, tb, ts, ob_list, ub_list, os_list, us_list, c.tr_parent.as<int64_t>(),
            c.tr_tokpar.as<int64_t>(), tree ? 1 : 0};
