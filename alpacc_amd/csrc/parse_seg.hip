@@ -523,7 +523,8 @@ __device__ __forceinline__ void block_scan4(int32_t* v, int32_t* tot, int32_t (*
 
 // MBPOP..MSPUSH bound the window effects of the grammar (bracket pops and
 // pushes, productions, surviving child slots); the host picks the smallest
-// instantiation that covers them.
+// instantiation that covers them.  BT holds a bracket symbol: u8 when every
+// symbol of the grammar is below 256 (half the stack's LDS), else u16.
 //
 // Every lane runs its APL positions' window effects in order on two short
 // stacks in LDS -- bracket symbols, and open child slots as lane-relative
@@ -533,9 +534,9 @@ __device__ __forceinline__ void block_scan4(int32_t* v, int32_t* tot, int32_t (*
 // consecutive levels just above its minimum, so a pop left over by a later
 // lane at tile-relative level D reads entry D - min - 1 of the last earlier
 // lane whose minimum is below D (a sparse table over the lanes' minima).
-template <int MBPOP, int MBPUSH, int MNP, int MSPUSH>
+template <int MBPOP, int MBPUSH, int MNP, int MSPUSH, typename BT>
 __device__ __forceinline__ void resolve_tile(const Resolve& r, uint32_t tile, uint32_t tid) {
-  // dynamic LDS: bracket stacks u16 [bcap][RWG], slot stacks u8 [scap][RWG],
+  // dynamic LDS: bracket stacks BT [bcap][RWG], slot stacks u8 [scap][RWG],
   // staged parents of nodes u16 [ncap] and of terminal productions u16
   // [tcap] (tile-local compact indices), staged productions u8 [ncap]
   extern __shared__ __align__(16) uint16_t dyn[];
@@ -547,9 +548,11 @@ __device__ __forceinline__ void resolve_tile(const Resolve& r, uint32_t tile, ui
   const Seg& s = r.s;
   const uint32_t lane = tid & (WAVE - 1), wid = tid / WAVE;
   const int bcap = r.bcap, scap = r.scap;
-  uint16_t* const bst = dyn + tid;
-  uint8_t* const sst = reinterpret_cast<uint8_t*>(dyn + bcap * RWG) + tid;
-  uint16_t* const s_par = dyn + bcap * RWG + (scap * RWG + 1) / 2;
+  BT* const ball = reinterpret_cast<BT*>(dyn);
+  BT* const bst = ball + tid;
+  uint8_t* const sall = reinterpret_cast<uint8_t*>(ball + bcap * RWG);
+  uint8_t* const sst = sall + tid;
+  uint16_t* const s_par = reinterpret_cast<uint16_t*>(sall + (scap * RWG + 1) / 2 * 2);
   uint16_t* const s_tpar = s_par + r.ncap;
   uint8_t* const s_prod = reinterpret_cast<uint8_t*>(s_tpar + r.tcap);
   const uint32_t seg = s.tile_seg[tile];
@@ -605,7 +608,7 @@ __device__ __forceinline__ void resolve_tile(const Resolve& r, uint32_t tile, ui
 #pragma unroll
     for (int e = 0; e < MBPOP; ++e) {
       if (e < nbpop) {
-        const uint16_t sym = (uint16_t)rec_br(r0, r1, e);
+        const BT sym = (BT)rec_br(r0, r1, e);
         if (bsp > 0) {
           bad |= bst[--bsp * RWG] != sym;
         } else {
@@ -618,7 +621,7 @@ __device__ __forceinline__ void resolve_tile(const Resolve& r, uint32_t tile, ui
     bmin = bdep < bmin ? bdep : bmin;
 #pragma unroll
     for (int e = 0; e < MBPUSH; ++e)
-      if (e < nbpush) bst[(bsp + e) * RWG] = (uint16_t)rec_br(r0, r1, 4 + e);
+      if (e < nbpush) bst[(bsp + e) * RWG] = (BT)rec_br(r0, r1, 4 + e);
     bsp += nbpush;
     bdep += nbpush;
     const int32_t cbk = slot - tinc;  // tile-local compact index of the position's first node
@@ -725,13 +728,13 @@ __device__ __forceinline__ void resolve_tile(const Resolve& r, uint32_t tile, ui
 #pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
   for (int u = 0; u < bun; ++u) {
     const int32_t D = bS - u;
-    const uint16_t sym = bst[(bcap - 1 - u) * RWG];
+    const BT sym = bst[(bcap - 1 - u) * RWG];
     while (l >= 0 && (int32_t)(int16_t)(w >> 16) >= D) {
       l = (int16_t)(w & 0xFFFFu);
       if (l >= 0) w = pkb[l];
     }
     if (l >= 0)
-      bad |= dyn[(D - (int32_t)(int16_t)(w >> 16) - 1) * RWG + l] != sym;
+      bad |= ball[(D - (int32_t)(int16_t)(w >> 16) - 1) * RWG + l] != sym;
     else
       r.ub_list[ti.ub_off + (-D)] = sym;
   }
@@ -742,7 +745,6 @@ __device__ __forceinline__ void resolve_tile(const Resolve& r, uint32_t tile, ui
 #pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
   for (int e = 0; e < nob; ++e) r.ob_list[ti.ob_off + (lbm + e - ti.bmin)] = bst[e * RWG];
   if (r.tree) {
-    const uint8_t* const sall = reinterpret_cast<const uint8_t*>(dyn + bcap * RWG);
     l = (int)tid - 1;
     w = l >= 0 ? pks[l] : 0u;
 #pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
@@ -793,17 +795,27 @@ __device__ __forceinline__ void resolve_tile(const Resolve& r, uint32_t tile, ui
 }
 
 // One tile per workgroup, as seg_profile.
-template <int MBPOP, int MBPUSH, int MNP, int MSPUSH, bool STRIDE>
+template <int MBPOP, int MBPUSH, int MNP, int MSPUSH, bool STRIDE, typename BT>
 __global__ __launch_bounds__(RWG) void seg_resolve(Resolve r) {
   const uint32_t nt = *r.s.nt_dev;
   if (!STRIDE) {
-    if (blockIdx.x < nt) resolve_tile<MBPOP, MBPUSH, MNP, MSPUSH>(r, blockIdx.x, threadIdx.x);
+    if (blockIdx.x < nt) resolve_tile<MBPOP, MBPUSH, MNP, MSPUSH, BT>(r, blockIdx.x, threadIdx.x);
     return;
   }
   for (uint32_t tile = blockIdx.x; tile < nt; tile += gridDim.x) {
-    resolve_tile<MBPOP, MBPUSH, MNP, MSPUSH>(r, tile, tid_per_tile());
+    resolve_tile<MBPOP, MBPUSH, MNP, MSPUSH, BT>(r, tile, tid_per_tile());
     __syncthreads();
   }
+}
+
+// The single-tile and striding kernels of one instantiation, bracket
+// stacks u8 (br8) or u16.
+template <int MBPOP, int MBPUSH, int MNP, int MSPUSH>
+void resolve_kernels(bool br8, void (**kern)(Resolve)) {
+  kern[0] = br8 ? seg_resolve<MBPOP, MBPUSH, MNP, MSPUSH, false, uint8_t>
+                : seg_resolve<MBPOP, MBPUSH, MNP, MSPUSH, false, uint16_t>;
+  kern[1] = br8 ? seg_resolve<MBPOP, MBPUSH, MNP, MSPUSH, true, uint8_t>
+                : seg_resolve<MBPOP, MBPUSH, MNP, MSPUSH, true, uint16_t>;
 }
 
 // ---------------------------------------------------------------- pass C
@@ -1057,11 +1069,20 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
     return fail(ALPACC_ERR_ARG, "batch beyond the capacity its launch sequence is sized for");
   const int ncap = PT * g->max_prods, tcap = PT * g->max_nt;
   const int bcap = APL * (g->max_bpop + g->max_bpush), scap = APL * (g->max_spop + g->max_spush);
+  if (c.resolve_br8 < 0) {  // bracket symbols: bytes 8..23 of each pair record, u16
+    uint32_t mx = 0;
+    for (int i = 0; i <= g->npairs; ++i)
+      for (int j = 8; j < 24; j += 2)
+        mx = std::max<uint32_t>(mx, g->pair_rec[(size_t)i * 64 + j] | (uint32_t)g->pair_rec[(size_t)i * 64 + j + 1] << 8);
+    c.resolve_br8 = mx < 256 ? 1 : 0;
+  }
+  const bool br8 = c.resolve_br8 == 1;
   // seg_resolve's dynamic LDS with node / terminal staging for n / t entries
-  auto resolve_lds = [&](int n, int t) {
-    return (size_t)bcap * RWG * 2 + ((size_t)scap * RWG + 1) / 2 * 2 + (size_t)(n + t) * 2 + (size_t)n + 16;
+  // (bracket stacks of bw bytes an entry)
+  auto resolve_lds = [&](int n, int t, int bw) {
+    return (size_t)bcap * RWG * bw + ((size_t)scap * RWG + 1) / 2 * 2 + (size_t)(n + t) * 2 + (size_t)n + 16;
   };
-  const size_t lds_b = resolve_lds(ncap, tcap);
+  const size_t lds_b = resolve_lds(ncap, tcap, br8 ? 1 : 2);
   // tile-local depths are int16, compact indices u16, lane-relative slot
   // owners 7 bits
   if ((int64_t)PT * 8 > 0x7FFF || ncap > 0xFFFF || APL * g->max_prods > 0x7F || lds_b > 96 * 1024)
@@ -1227,23 +1248,19 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
     const int n_st = caps ? ncap : (int)(tree ? so.tmax_node : so.tmax_slot);
     const int t_st = caps ? tcap : (tree ? (int)so.tmax_term : 0);
     if (n_st > ncap || t_st > tcap) return fail(ALPACC_ERR_LIMIT, "tile counts beyond the grammar's maxima");
-    const size_t lds_r = resolve_lds(n_st, t_st);
+    const size_t lds_r = resolve_lds(n_st, t_st, br8 ? 1 : 2);
     Resolve rs{s,         reinterpret_cast<const uint16_t*>(base + off_pid),
                info,      sb,       bcap,  scap, n_st, t_st,
                ob_list,   ub_list,   os_list,  us_list,       c.tr_prod.as<uint8_t>(),
                c.tr_parent.as<int64_t>(), c.tr_tokpar.as<int64_t>(), tree ? nullptr : c.ps_d.as<uint8_t>(),
                tree ? 1 : 0};
     void (*kern[2])(Resolve);  // one tile per workgroup, striding
-    if (g->max_bpop <= 2 && g->max_bpush <= 3 && g->max_prods <= 4 && g->max_spush <= 3) {
-      kern[0] = seg_resolve<2, 3, 4, 3, false>;
-      kern[1] = seg_resolve<2, 3, 4, 3, true>;
-    } else if (g->max_bpop <= 3 && g->max_bpush <= 4 && g->max_prods <= 4 && g->max_spush <= 4) {
-      kern[0] = seg_resolve<3, 4, 4, 4, false>;
-      kern[1] = seg_resolve<3, 4, 4, 4, true>;
-    } else {
-      kern[0] = seg_resolve<4, 4, 8, 24, false>;
-      kern[1] = seg_resolve<4, 4, 8, 24, true>;
-    }
+    if (g->max_bpop <= 2 && g->max_bpush <= 3 && g->max_prods <= 4 && g->max_spush <= 3)
+      resolve_kernels<2, 3, 4, 3>(br8, kern);
+    else if (g->max_bpop <= 3 && g->max_bpush <= 4 && g->max_prods <= 4 && g->max_spush <= 4)
+      resolve_kernels<3, 4, 4, 4>(br8, kern);
+    else
+      resolve_kernels<4, 4, 8, 24>(br8, kern);
     if (!c.resolve_lds_set || c.resolve_lds_set < lds_b) {
       for (auto k : kern)
         HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
