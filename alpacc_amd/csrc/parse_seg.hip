@@ -375,15 +375,22 @@ __global__ __launch_bounds__(256) void tile_block_sums(Seg s, const TileRec* rec
     mx[1] = max(mx[1], (uint32_t)r.nt);
     mx[2] = max(mx[2], (uint32_t)r.np);
   }
+  __shared__ uint32_t wmx[3][256 / WAVE];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     for (int o = WAVE / 2; o > 0; o >>= 1) mx[k] = max(mx[k], (uint32_t)__shfl_xor((int)mx[k], o, WAVE));
-    if ((threadIdx.x & (WAVE - 1)) == 0 && mx[k]) atomicMax(&out->tmax_node + k, mx[k]);
+    if ((threadIdx.x & (WAVE - 1)) == 0) wmx[k][threadIdx.x / WAVE] = mx[k];
   }
   int64_t tot[NV];
-  block_scan_nv(v, tot);
-  if (threadIdx.x == 0)
+  block_scan_nv(v, tot);  // its barriers publish wmx
+  if (threadIdx.x == 0) {
     for (int k = 0; k < NV; ++k) bsum[(uint64_t)blockIdx.x * NV + k] = tot[k];
+    for (int k = 0; k < 3; ++k) {  // one atomic a block and count
+      uint32_t m = 0;
+      for (int w = 0; w < 256 / WAVE; ++w) m = max(m, wmx[k][w]);
+      if (m) atomicMax(&out->tmax_node + k, m);
+    }
+  }
 }
 
 __global__ __launch_bounds__(256) void tile_block_scan(int64_t* bsum, uint32_t nblk, ScanOut* out) {
