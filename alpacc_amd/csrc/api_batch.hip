@@ -382,6 +382,22 @@ int alpacc_hip_set_graphs(alpacc_hip_ctx* h, int on) {
   return 0;
 }
 
+int alpacc_hip_set_bracket_width(alpacc_hip_ctx* h, int bytes) {
+  if (!h) return fail(ALPACC_ERR_ARG, "null argument");
+  if (bytes < 0 || bytes > 2) return fail(ALPACC_ERR_ARG, "bracket width %d is not 0, 1 or 2", bytes);
+  Context& c = h->c;
+  if (bytes == 1 && bracket_symbol_max(c.g) >= 256)
+    return fail(ALPACC_ERR_ARG, "bracket symbols of %s reach 256: one-byte stacks cannot hold them", c.g->name);
+  HIP_TRY(hipSetDevice(c.device));
+  c.resolve_bw = bytes;
+  c.resolve_lds_set = 0;  // the other kernels have not been granted their LDS
+  if (c.graph) {          // captured with the old kernels
+    (void)hipGraphExecDestroy(c.graph);
+    c.graph = nullptr;
+  }
+  return 0;
+}
+
 int alpacc_hip_graph_stats(const alpacc_hip_ctx* h, uint64_t* captures, uint64_t* replays) {
   if (!h || !captures || !replays) return fail(ALPACC_ERR_ARG, "null argument");
   *captures = h->c.graph_captures;

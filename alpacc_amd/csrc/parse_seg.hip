@@ -1067,6 +1067,14 @@ __global__ __launch_bounds__(PLAN_WG) void parse_plan(const uint64_t* tok_off, c
 
 }  // namespace
 
+uint32_t bracket_symbol_max(const GrammarDesc* g) {  // bytes 8..23 of each pair record, u16
+  uint32_t mx = 0;
+  for (int i = 0; i <= g->npairs; ++i)
+    for (int j = 8; j < 24; j += 2)
+      mx = std::max<uint32_t>(mx, g->pair_rec[(size_t)i * 64 + j] | (uint32_t)g->pair_rec[(size_t)i * 64 + j + 1] << 8);
+  return mx;
+}
+
 int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_tok_off, const uint64_t* h_tok_off,
                           const uint32_t* h_skip, uint64_t count, bool tree, hipStream_t st, const BatchCaps* caps,
                           const uint32_t* d_skip, const uint32_t* d_live) {
@@ -1076,14 +1084,8 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
     return fail(ALPACC_ERR_ARG, "batch beyond the capacity its launch sequence is sized for");
   const int ncap = PT * g->max_prods, tcap = PT * g->max_nt;
   const int bcap = APL * (g->max_bpop + g->max_bpush), scap = APL * (g->max_spop + g->max_spush);
-  if (c.resolve_br8 < 0) {  // bracket symbols: bytes 8..23 of each pair record, u16
-    uint32_t mx = 0;
-    for (int i = 0; i <= g->npairs; ++i)
-      for (int j = 8; j < 24; j += 2)
-        mx = std::max<uint32_t>(mx, g->pair_rec[(size_t)i * 64 + j] | (uint32_t)g->pair_rec[(size_t)i * 64 + j + 1] << 8);
-    c.resolve_br8 = mx < 256 ? 1 : 0;
-  }
-  const bool br8 = c.resolve_br8 == 1;
+  if (c.resolve_br8 < 0) c.resolve_br8 = bracket_symbol_max(g) < 256 ? 1 : 0;
+  const bool br8 = c.resolve_bw ? c.resolve_bw == 1 : c.resolve_br8 == 1;
   // seg_resolve's dynamic LDS with node / terminal staging for n / t entries
   // (bracket stacks of bw bytes an entry)
   auto resolve_lds = [&](int n, int t, int bw) {

@@ -38,6 +38,9 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
 
 // One string of n ids: returns 1 (valid) / 0 (rejected) / < 0 (error) and
 // the derivation length in *num_prods (trees: nodes = *num_prods - n).
+// Largest bracket symbol of the grammar's pair records.
+uint32_t bracket_symbol_max(const GrammarDesc* g);
+
 int parse_on_device(Context& c, const uint8_t* d_tok, uint64_t n, bool tree, hipStream_t st, uint64_t* num_prods);
 
 }  // namespace alpacc

@@ -84,6 +84,7 @@ struct Context {
   int lex_grid = 0, lex_grid2 = 0;
   size_t resolve_lds_set = 0;        // dynamic LDS granted to the parser's resolve kernel
   int resolve_br8 = -1;              // 1: every bracket symbol below 256 (-1: not checked yet)
+  int resolve_bw = 0;                // bracket stack bytes an entry (0: 1 when resolve_br8, else 2)
   // resident tables
   uint8_t* d_class = nullptr;
   uint16_t* d_step = nullptr;

@@ -104,6 +104,7 @@ def load_library(path: str = LIB_PATH):
                                            C.POINTER(u64)]),
         "alpacc_hip_server_stats": (i32, [vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64)]),
         "alpacc_hip_set_graphs": (i32, [vp, i32]),
+        "alpacc_hip_set_bracket_width": (i32, [vp, i32]),
         "alpacc_hip_graph_stats": (i32, [vp, C.POINTER(u64), C.POINTER(u64)]),
         "alpacc_hip_server_set_graphs": (i32, [vp, i32]),
         "alpacc_hip_set_lex_shape": (i32, [vp, C.POINTER(LexShape)]),
@@ -348,6 +349,10 @@ class Backend:
     def set_graphs(self, on: bool = True):
         """Run combined-mode batches through a captured HIP graph."""
         _check(self.lib.alpacc_hip_set_graphs(self.h, int(on)) >= 0 and 1 or -1)
+
+    def set_bracket_width(self, nbytes: int = 0):
+        """Element width of the parser's LDS bracket stacks: 0 automatic, 1 or 2 bytes."""
+        _check(self.lib.alpacc_hip_set_bracket_width(self.h, int(nbytes)))
 
     def graph_stats(self):
         """(captures, replays) of the batch graph so far."""

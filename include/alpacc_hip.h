@@ -234,6 +234,14 @@ int alpacc_hip_lex_parse_batch_device(alpacc_hip_ctx *ctx, const uint8_t *d_pack
 int alpacc_hip_set_graphs(alpacc_hip_ctx *ctx, int on);
 int alpacc_hip_graph_stats(const alpacc_hip_ctx *ctx, uint64_t *captures, uint64_t *replays);
 
+/* Element width of the parser's per-lane bracket stacks in LDS (a variant of
+ * the kernels, with the same results; no reference counterpart, as the lexer
+ * shape above): 0 = automatic (1 byte when every bracket symbol of the
+ * grammar is below 256, else 2), 1 = one byte (ALPACC_ERR_ARG when a symbol
+ * is 256 or more), 2 = two bytes.  One-byte stacks take less LDS, so more
+ * tiles are resolved at once per CU. */
+int alpacc_hip_set_bracket_width(alpacc_hip_ctx *ctx, int bytes);
+
 /* ---- serving ---------------------------------------------------------------
  * Continuous batching over resident grammars (no reference counterpart: the
  * reference's server answers one request at a time, backends/c/cli.c:

@@ -461,6 +461,36 @@ def test_autotune_lexer():
         be.close()
 
 
+def test_bracket_widths():
+    """Every bracket stack width of the parser's resolve kernels gives the
+    oracle's records: one-byte and two-byte stacks, eager single inputs and
+    batches, and the graph path's striding kernels; a width the grammar's
+    symbols do not fit, or not 0/1/2, is refused."""
+    from alpacc_amd import hip
+    for grammar in ("arithmetic", "json", "sexp"):  # the grammars with an input generator
+        inputs = [datagen.generate(grammar, n, seed=n + 5, conforming=n % 3 != 1).tobytes()
+                  for n in (0, 3, 1000, 4097, 70000, 1 << 20, 3 << 20)]
+        exp = [pyoracle.record(grammar, "both", x) for x in inputs]
+        be = hip.Backend(grammar)
+        try:
+            for width in (2, 1, 0):
+                print(f"bracket width {grammar} {width}", flush=True)
+                be.set_bracket_width(width)
+                for x, e in zip(inputs, exp):
+                    assert first_diff(be.run_case("both", x), e) is None, (grammar, width, len(x))
+                be.set_graphs(False)
+                assert first_diff(be.run_batch("both", inputs), b"".join(exp)) is None, (grammar, width, "batch")
+                be.set_graphs(True)
+                for _ in range(2):
+                    assert first_diff(be.run_batch("both", inputs), b"".join(exp)) is None, (grammar, width, "graph")
+                be.set_graphs(False)
+            for bad in (3, -1):
+                with pytest.raises(hip.HipError):
+                    be.set_bracket_width(bad)
+        finally:
+            be.close()
+
+
 @pytest.mark.parametrize("index32,length", [(True, 0), (False, 8), (True, 64)])
 def test_width_variants(index32, length):
     """Contexts opened as width variants (--index32, length) answer every
