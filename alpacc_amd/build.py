@@ -35,7 +35,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
-HIP_SOURCES = ["lexer.hip", "parse_seg.hip", "runtime.hip", "api_lex.hip", "api_parse.hip", "api_case.hip", "api_batch.hip", "serve.hip"]
+HIP_SOURCES = ["lexer.hip", "parse_seg.hip", "runtime.hip", "api_lex.hip", "api_parse.hip", "api_case.hip", "api_batch.hip", "serve.hip", "tables_file.cpp"]
 
 LIB_NAME = "libalpacc_hip.so"
 CLI_NAME = "alpacc-hip"
@@ -73,12 +73,12 @@ def generate() -> list:
     reg.append("static const GrammarDesc* const ALL[] = {" + ", ".join(f"&GRAMMAR_{n.upper()}" for n in names) + "};")
     reg.append("static const char* const NAMES[] = {" + ", ".join(f'"{n}"' for n in names) + ", nullptr};")
     reg += [
-        "const GrammarDesc* find_grammar(const char* name) {",
+        "const GrammarDesc* find_builtin_grammar(const char* name) {",
         "  for (const GrammarDesc* g : ALL)",
         "    if (std::strcmp(g->name, name) == 0) return g;",
         "  return nullptr;",
         "}",
-        "const char* const* grammar_names() { return NAMES; }",
+        "const char* const* builtin_grammar_names() { return NAMES; }",
         "}  // namespace alpacc",
         "",
     ]
@@ -146,12 +146,37 @@ def build(jobs: int = 8, force: bool = True) -> str:
     return lib
 
 
+def emit_test_tables(jobs: int = 8) -> list:
+    """Tables files (python -m alpacc_amd.gen hip) of the parity tests'
+    grammars under tests/golden/grammars/, written next to them as
+    <name>.alphip; the tests load them into the library at run time."""
+    sys.path.insert(0, ROOT)
+    from alpacc_amd.gen import emit_hip, tables
+
+    d = os.path.join(ROOT, "tests", "golden", "grammars")
+    if not os.path.isdir(d):
+        return []
+    names = sorted(f for f in os.listdir(d) if f.endswith(".alp"))
+
+    def one(f):
+        out = os.path.join(d, f[:-4] + ".alphip")
+        blob = emit_hip.emit_tables(tables.load(os.path.join(d, f)))
+        if not os.path.exists(out) or open(out, "rb").read() != blob:
+            with open(out, "wb") as fh:
+                fh.write(blob)
+        return out
+
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        return list(ex.map(one, names))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=8)
     ap.add_argument("--incremental", action="store_true", help="reuse unchanged objects")
     a = ap.parse_args(argv)
     print(build(a.jobs, force=not a.incremental))
+    print(f"{len(emit_test_tables(a.jobs))} test grammar tables")
 
 
 if __name__ == "__main__":

@@ -27,6 +27,9 @@ namespace alpacc {
   } while (0)
 
 int stage_input(Context& c, const void* host, uint64_t bytes, hipStream_t st, const uint8_t** d);
+int lexer_record(Context& c, uint64_t n, const uint8_t* payload, std::vector<uint8_t>& rec);
+int parser_record(Context& c, uint64_t n, const uint8_t* payload, std::vector<uint8_t>& rec);
+int both_record(Context& c, uint64_t n, const uint8_t* payload, std::vector<uint8_t>& rec);
 
 namespace {
 
@@ -67,7 +70,7 @@ int lex_parse_batch_graph(Context& c, const uint8_t* d_in, const uint64_t* off, 
   // a capacity beyond the launch limits (lexer bytes, the parser's 31-bit
   // depths at worst-case positions) runs eagerly instead
   const GrammarDesc* g = c.g;
-  const uint64_t per_pos = (uint64_t)std::max(1, std::max(g->max_bpush, g->max_spush));
+  const uint64_t per_pos = (uint64_t)parse_per_position(g);
   if (caps.bytes > 0x7FFFFFFFull || (caps.bytes + 2 * caps.count) * per_pos + caps.count > 0x7FFFFFFFull) {
     int r = lex_batch_on_device(c, ALPACC_MODE_BOTH, d_in, off, count, st);
     if (r < 0) return r;
@@ -339,6 +342,90 @@ int batch_records(Context& c, int mode, const uint8_t* packed, const uint64_t* o
   return 0;
 }
 
+namespace {
+
+// Records appended to one malloc'd buffer.
+struct Records {
+  uint8_t* buf = nullptr;
+  size_t len = 0, cap = 0;
+  int append(const uint8_t* p, size_t n) {
+    if (len + n > cap) {
+      size_t nc = cap ? cap : 1 << 16;
+      while (nc < len + n) nc *= 2;
+      uint8_t* q = static_cast<uint8_t*>(realloc(buf, nc));
+      if (!q) return fail(ALPACC_ERR_NOMEM, "host allocation failed");
+      buf = q;
+      cap = nc;
+    }
+    memcpy(buf + len, p, n);
+    len += n;
+    return 0;
+  }
+};
+
+// Inputs [a, b) of a batch, answered in launches that each stay within one
+// launch's limits: at most BATCH_LAUNCH_BYTES of input (the lexer's 31-bit
+// positions), and in parser mode at most parse_max_positions framed
+// positions.  In combined mode the token count is known only after the
+// lexer, so a launch the parser refuses for its size is split in halves.
+// An input beyond the lexer's limit alone is answered by run_case (the
+// stream path in lexer mode; a size error in the others).
+constexpr uint64_t BATCH_LAUNCH_BYTES = 1ull << 30;
+
+int records_in_launches(Context& c, int mode, const uint8_t* packed, const uint64_t* off, uint64_t a, uint64_t b,
+                        Records& acc) {
+  const uint64_t max_pos = parse_max_positions(c.g);
+  uint64_t i = a;
+  while (i < b) {
+    if (off[i + 1] - off[i] > BATCH_LAUNCH_BYTES) {  // one input of its own
+      std::vector<uint8_t> rec;
+      int r;
+      switch (mode) {
+        case ALPACC_MODE_LEXER: r = lexer_record(c, off[i + 1] - off[i], packed + off[i], rec); break;
+        case ALPACC_MODE_PARSER: r = parser_record(c, off[i + 1] - off[i], packed + off[i], rec); break;
+        default: r = both_record(c, off[i + 1] - off[i], packed + off[i], rec); break;
+      }
+      if (r < 0) return r;
+      if ((r = acc.append(rec.data(), rec.size()))) return r;
+      ++i;
+      continue;
+    }
+    uint64_t j = i + 1;
+    auto fits = [&](uint64_t e) {
+      if (off[e] - off[i] > BATCH_LAUNCH_BYTES) return false;
+      return mode != ALPACC_MODE_PARSER || (off[e] - off[i]) / c.g->terminal_bytes_both + 2 * (e - i) <= max_pos;
+    };
+    while (j < b && fits(j + 1)) ++j;
+    uint8_t* out = nullptr;
+    size_t len = 0;
+    int r = batch_records(c, mode, packed, off + i, j - i, &out, &len, nullptr);
+    if (r == ALPACC_ERR_LIMIT && j - i > 1) {  // too many tokens for one parser launch
+      free(out);
+      const uint64_t mid = i + (j - i) / 2;
+      if ((r = records_in_launches(c, mode, packed, off, i, mid, acc)) < 0) return r;
+      if ((r = records_in_launches(c, mode, packed, off, mid, j, acc)) < 0) return r;
+      i = j;
+      continue;
+    }
+    if (r < 0) {
+      free(out);
+      return r;
+    }
+    if (!acc.buf) {  // the first launch's records become the buffer
+      acc.buf = out;
+      acc.len = acc.cap = len;
+    } else {
+      r = acc.append(out, len);
+      free(out);
+      if (r) return r;
+    }
+    i = j;
+  }
+  return 0;
+}
+
+}  // namespace
+
 }  // namespace alpacc
 
 using namespace alpacc;
@@ -358,15 +445,14 @@ int alpacc_hip_run_batch(alpacc_hip_ctx* h, int mode, const uint8_t* packed, con
     if (offsets[i + 1] < offsets[i]) return fail(ALPACC_ERR_ARG, "offsets decrease at input %llu", (unsigned long long)i);
   Context& c = h->c;
   HIP_TRY(hipSetDevice(c.device));
-  uint8_t* out = nullptr;
-  size_t len = 0;
-  int r = batch_records(c, mode, packed, offsets, count, &out, &len, nullptr);
+  Records acc;
+  int r = records_in_launches(c, mode, packed, offsets, 0, count, acc);
   if (r < 0) {
-    free(out);
+    free(acc.buf);
     return r;
   }
-  *records = out;
-  *records_len = len;
+  *records = acc.buf;
+  *records_len = acc.len;
   return 0;
 }
 
@@ -390,8 +476,7 @@ int alpacc_hip_set_bracket_width(alpacc_hip_ctx* h, int bytes) {
     return fail(ALPACC_ERR_ARG, "bracket symbols of %s reach 256: one-byte stacks cannot hold them", c.g->name);
   HIP_TRY(hipSetDevice(c.device));
   c.resolve_bw = bytes;
-  c.resolve_lds_set = 0;  // the other kernels have not been granted their LDS
-  if (c.graph) {          // captured with the old kernels
+  if (c.graph) {  // captured with the old kernels
     (void)hipGraphExecDestroy(c.graph);
     c.graph = nullptr;
   }

@@ -18,8 +18,7 @@ namespace alpacc {
     if (e_ != hipSuccess) return fail(ALPACC_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
   } while (0)
 
-// Positions and counts travel in 31-bit fields of the lookback words.
-constexpr uint64_t LEX_MAX_INPUT = (1ull << 31) - 1;
+constexpr uint64_t LEX_MAX_INPUT = LEX_MAX_LAUNCH_BYTES;
 
 int lex_batch_on_device(Context& c, int mode, const uint8_t* d_in, const uint64_t* h_off, uint64_t count,
                         hipStream_t st, const BatchCaps* caps, bool enqueue, const LexStreamStep* stream) {
@@ -197,84 +196,16 @@ int lexemes_to_host(Context& c, int mode, uint64_t count, hipStream_t st, alpacc
   return 0;
 }
 
-// Lexer-only response record: u8 1, u64 count, then (terminal, start,
-// length) per lexeme with native widths (C/Generator.hs:88-105).
-int lexer_record(Context& c, uint64_t n, const uint8_t* payload, std::vector<uint8_t>& rec) {
-  hipStream_t st = c.stream;
-  const uint8_t* d_in = nullptr;
-  int r;
-  if ((r = stage_input(c, payload, n, st, &d_in))) return r;
-  uint64_t count = 0;
-  r = lex_on_device(c, ALPACC_MODE_LEXER, d_in, n, st, &count);
-  if (r < 0) return r;
-  if (r == ALPACC_REJECT) {
-    rec.assign(1, 0);
-    return ALPACC_REJECT;
-  }
-  alpacc_hip_lexemes lx{};
-  if ((r = lexemes_to_host(c, ALPACC_MODE_LEXER, count, st, &lx))) return r;
-  const GrammarDesc* g = c.g;
-  const int tb = g->terminal_bytes_lex, ib = g->index_bytes, lb = g->length_bytes;
-  const size_t per = tb + ib + lb;
-  rec.resize(1 + 8 + count * per);
-  rec[0] = 1;
-  memcpy(&rec[1], &count, 8);
-  uint8_t* o = rec.data() + 9;
-  const uint8_t* t = static_cast<const uint8_t*>(lx.terminals);
-  const uint8_t* l = static_cast<const uint8_t*>(lx.lengths);
-  for (uint64_t i = 0; i < count; ++i) {
-    memcpy(o, t + i * tb, tb);
-    memcpy(o + tb, reinterpret_cast<const uint8_t*>(lx.starts + i), ib);
-    memcpy(o + tb + ib, l + i * lb, lb);
-    o += per;
-  }
-  free(lx.terminals);
-  free(lx.starts);
-  free(lx.lengths);
-  return ALPACC_VALID;
-}
-
-}  // namespace alpacc
-
-using namespace alpacc;
-
-extern "C" {
-
-int alpacc_hip_lex_device(alpacc_hip_ctx* h, int mode, const uint8_t* d_str, uint64_t n, alpacc_hip_lexemes* d_out,
-                          void* stream) {
-  if (!h || !d_out || (n && !d_str)) return fail(ALPACC_ERR_ARG, "null argument");
-  if (mode != ALPACC_MODE_LEXER && mode != ALPACC_MODE_BOTH) return fail(ALPACC_ERR_ARG, "bad lexer mode %d", mode);
-  Context& c = h->c;
-  HIP_TRY(hipSetDevice(c.device));
-  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c.stream;
-  uint64_t count = 0;
-  int r = lex_on_device(c, mode, d_str, n, st, &count);
-  if (r < 0) return r;
-  d_out->count = count;
-  d_out->terminals = c.lx_term.ptr;
-  d_out->starts = c.lx_start.as<int64_t>();
-  d_out->lengths = c.lx_len.ptr;
-  return r;
-}
-
-int alpacc_hip_lex_stream_begin(alpacc_hip_ctx* h, int mode) {
-  if (!h) return fail(ALPACC_ERR_ARG, "null argument");
-  if (mode != ALPACC_MODE_LEXER && mode != ALPACC_MODE_BOTH) return fail(ALPACC_ERR_ARG, "bad lexer mode %d", mode);
-  Context& c = h->c;
+void lex_stream_begin(Context& c, int mode) {
   c.lstream = Context::LexStream{};
   c.lstream.on = true;
   c.lstream.mode = mode;
-  return 0;
 }
 
-int alpacc_hip_lex_stream_push(alpacc_hip_ctx* h, const uint8_t* d_chunk, uint64_t n, int is_last_chunk,
-                               alpacc_hip_lexemes* d_out, void* stream) {
-  if (!h || !d_out || (n && !d_chunk)) return fail(ALPACC_ERR_ARG, "null argument");
-  Context& c = h->c;
+// One chunk of the stream begun on c (alpacc_hip_lex_stream_push).
+int lex_stream_push(Context& c, const uint8_t* d_chunk, uint64_t n, int is_last_chunk, alpacc_hip_lexemes* d_out,
+                    hipStream_t st) {
   Context::LexStream& ls = c.lstream;
-  if (!ls.on) return fail(ALPACC_ERR_ARG, "no lexer stream begun on this context");
-  HIP_TRY(hipSetDevice(c.device));
-  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c.stream;
   const GrammarDesc* g = c.g;
   const bool lex_ids = ls.mode == ALPACC_MODE_LEXER;
   const int lb = g->length_bytes;
@@ -336,6 +267,87 @@ int alpacc_hip_lex_stream_push(alpacc_hip_ctx* h, const uint8_t* d_chunk, uint64
     return ls.flags ? ALPACC_REJECT : ALPACC_VALID;
   }
   return (ls.flags & LEX_REJECT_OVERFLOW) ? ALPACC_REJECT : ALPACC_VALID;
+}
+
+
+int lexer_record_stream(Context& c, uint64_t n, const uint8_t* payload, std::vector<uint8_t>& rec);
+
+// Lexer-only response record: u8 1, u64 count, then (terminal, start,
+// length) per lexeme with native widths (C/Generator.hs:88-105).  An input
+// beyond one launch's limit goes through the stream path in chunks.
+int lexer_record(Context& c, uint64_t n, const uint8_t* payload, std::vector<uint8_t>& rec) {
+  if (n > LEX_MAX_INPUT) return lexer_record_stream(c, n, payload, rec);
+  hipStream_t st = c.stream;
+  const uint8_t* d_in = nullptr;
+  int r;
+  if ((r = stage_input(c, payload, n, st, &d_in))) return r;
+  uint64_t count = 0;
+  r = lex_on_device(c, ALPACC_MODE_LEXER, d_in, n, st, &count);
+  if (r < 0) return r;
+  if (r == ALPACC_REJECT) {
+    rec.assign(1, 0);
+    return ALPACC_REJECT;
+  }
+  alpacc_hip_lexemes lx{};
+  if ((r = lexemes_to_host(c, ALPACC_MODE_LEXER, count, st, &lx))) return r;
+  const GrammarDesc* g = c.g;
+  const int tb = g->terminal_bytes_lex, ib = g->index_bytes, lb = g->length_bytes;
+  const size_t per = tb + ib + lb;
+  rec.resize(1 + 8 + count * per);
+  rec[0] = 1;
+  memcpy(&rec[1], &count, 8);
+  uint8_t* o = rec.data() + 9;
+  const uint8_t* t = static_cast<const uint8_t*>(lx.terminals);
+  const uint8_t* l = static_cast<const uint8_t*>(lx.lengths);
+  for (uint64_t i = 0; i < count; ++i) {
+    memcpy(o, t + i * tb, tb);
+    memcpy(o + tb, reinterpret_cast<const uint8_t*>(lx.starts + i), ib);
+    memcpy(o + tb + ib, l + i * lb, lb);
+    o += per;
+  }
+  free(lx.terminals);
+  free(lx.starts);
+  free(lx.lengths);
+  return ALPACC_VALID;
+}
+
+}  // namespace alpacc
+
+using namespace alpacc;
+
+extern "C" {
+
+int alpacc_hip_lex_device(alpacc_hip_ctx* h, int mode, const uint8_t* d_str, uint64_t n, alpacc_hip_lexemes* d_out,
+                          void* stream) {
+  if (!h || !d_out || (n && !d_str)) return fail(ALPACC_ERR_ARG, "null argument");
+  if (mode != ALPACC_MODE_LEXER && mode != ALPACC_MODE_BOTH) return fail(ALPACC_ERR_ARG, "bad lexer mode %d", mode);
+  Context& c = h->c;
+  HIP_TRY(hipSetDevice(c.device));
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c.stream;
+  uint64_t count = 0;
+  int r = lex_on_device(c, mode, d_str, n, st, &count);
+  if (r < 0) return r;
+  d_out->count = count;
+  d_out->terminals = c.lx_term.ptr;
+  d_out->starts = c.lx_start.as<int64_t>();
+  d_out->lengths = c.lx_len.ptr;
+  return r;
+}
+
+int alpacc_hip_lex_stream_begin(alpacc_hip_ctx* h, int mode) {
+  if (!h) return fail(ALPACC_ERR_ARG, "null argument");
+  if (mode != ALPACC_MODE_LEXER && mode != ALPACC_MODE_BOTH) return fail(ALPACC_ERR_ARG, "bad lexer mode %d", mode);
+  lex_stream_begin(h->c, mode);
+  return 0;
+}
+
+int alpacc_hip_lex_stream_push(alpacc_hip_ctx* h, const uint8_t* d_chunk, uint64_t n, int is_last_chunk,
+                               alpacc_hip_lexemes* d_out, void* stream) {
+  if (!h || !d_out || (n && !d_chunk)) return fail(ALPACC_ERR_ARG, "null argument");
+  Context& c = h->c;
+  if (!c.lstream.on) return fail(ALPACC_ERR_ARG, "no lexer stream begun on this context");
+  HIP_TRY(hipSetDevice(c.device));
+  return lex_stream_push(c, d_chunk, n, is_last_chunk, d_out, stream ? static_cast<hipStream_t>(stream) : c.stream);
 }
 
 int alpacc_hip_set_lex_shape(alpacc_hip_ctx* h, const alpacc_hip_lex_shape* shape) {
@@ -425,3 +437,60 @@ int alpacc_hip_lex_string(alpacc_hip_ctx* h, int mode, const uint8_t* str, uint6
 }
 
 }  // extern "C"
+
+namespace alpacc {
+
+// The lexer-only record of an input of any size: its bytes are staged and
+// lexed in chunks of LEX_STREAM_CHUNK through the stream entry points, the
+// tokens of every chunk appended to the record as they close.
+constexpr uint64_t LEX_STREAM_CHUNK = 1ull << 30;
+
+int lexer_record_stream(Context& c, uint64_t n, const uint8_t* payload, std::vector<uint8_t>& rec) {
+  hipStream_t st = c.stream;
+  const GrammarDesc* g = c.g;
+  const int tb = g->terminal_bytes_lex, ib = g->index_bytes, lb = g->length_bytes;
+  const size_t per = tb + ib + lb;
+  lex_stream_begin(c, ALPACC_MODE_LEXER);
+  int r;
+  rec.assign(9, 0);
+  uint64_t count = 0;
+  std::vector<uint8_t> ht, hs, hl;
+  for (uint64_t at = 0; at < n;) {
+    const uint64_t m = n - at < LEX_STREAM_CHUNK ? n - at : LEX_STREAM_CHUNK;
+    const bool last = at + m == n;
+    const uint8_t* d_in = nullptr;
+    if ((r = stage_input(c, payload + at, m, st, &d_in))) return r;
+    alpacc_hip_lexemes d{};
+    r = lex_stream_push(c, d_in, m, last ? 1 : 0, &d, st);
+    if (r < 0) return r;
+    if (r == ALPACC_REJECT) {
+      c.lstream.on = false;
+      rec.assign(1, 0);
+      return ALPACC_REJECT;
+    }
+    if (d.count) {
+      ht.resize(d.count * tb);
+      hs.resize(d.count * 8);
+      hl.resize(d.count * lb);
+      HIP_TRY(hipMemcpyAsync(ht.data(), d.terminals, d.count * tb, hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipMemcpyAsync(hs.data(), d.starts, d.count * 8, hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipMemcpyAsync(hl.data(), d.lengths, d.count * lb, hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+      const size_t o0 = rec.size();
+      rec.resize(o0 + d.count * per);
+      uint8_t* o = rec.data() + o0;
+      for (uint64_t i = 0; i < d.count; ++i, o += per) {
+        memcpy(o, &ht[i * tb], tb);
+        memcpy(o + tb, &hs[i * 8], ib);
+        memcpy(o + tb + ib, &hl[i * lb], lb);
+      }
+      count += d.count;
+    }
+    at += m;
+  }
+  rec[0] = 1;
+  memcpy(&rec[1], &count, 8);
+  return ALPACC_VALID;
+}
+
+}  // namespace alpacc

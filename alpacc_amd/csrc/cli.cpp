@@ -3,15 +3,22 @@
 //
 //   alpacc-hip -g GRAMMAR [--lexer|--parser] [-i FILE] [-o FILE]
 //              [--timeit] [--server] [--layout] [--device N]
-//              [--index32] [--length N]
+//              [--index32] [--length N] [--names]
+//   alpacc-hip --tables FILE [...]   a grammar generated at run time
+//              (python -m alpacc_amd.gen hip GRAMMAR.alp -o FILE)
 //
+// --names prints the id -> name tables of the mode (the names the
+// reference's Futhark target gives its terminal and production types) and
+// exits.
 // Invoked through a link named GRAMMAR or GRAMMAR_MODE (MODE = lexer, parser,
 // both) it needs no -g / mode flag, so it can stand in for a generated
 // program: batch mode reads one counted batch, --server loops over counted
 // batches until EOF and flushes after every batch.  The frames of a batch go
 // to the GPU together (alpacc_hip_run_batch).  Framing errors go to stderr
 // with exit status 1, as in the reference.
+#include <algorithm>
 #include <chrono>
+#include <exception>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -52,11 +59,27 @@ int read_frame(Prog& p, FILE* in, std::vector<uint8_t>& packed, std::vector<uint
     fprintf(stderr, "error: truncated frame\n");
     return -1;
   }
+  // the payload is read in pieces of at most 64 MiB, so a bogus frame
+  // length runs into the end of the input (a truncated frame) instead of
+  // into one huge allocation
   const size_t at = packed.size();
-  packed.resize(at + (flen - 8));
-  if (fread(packed.data() + at, 1, flen - 8, in) != flen - 8) {
-    fprintf(stderr, "error: truncated frame\n");
-    return -1;
+  uint64_t left = flen - 8;
+  while (left > 0) {
+    const size_t piece = (size_t)std::min<uint64_t>(left, 64ull << 20), pos = packed.size();
+    try {
+      packed.resize(pos + piece);
+    } catch (const std::exception&) {
+      packed.resize(at);
+      fprintf(stderr, "error: truncated frame\n");
+      return -1;
+    }
+    const size_t got = fread(packed.data() + pos, 1, piece, in);
+    if (got != piece) {
+      packed.resize(at);
+      fprintf(stderr, "error: truncated frame\n");
+      return -1;
+    }
+    left -= piece;
   }
   if (flen != 8 + alpacc_hip_grammar_payload_bytes(p.grammar.c_str(), p.mode, n)) {
     fprintf(stderr, "error: frame length mismatch\n");
@@ -174,6 +197,8 @@ void usage(const char* prog) {
           "  --device N GPU ordinal (default 0)\n"
           "  --index32  32-bit index_t (and length_t when the grammar sets no length)\n"
           "  --length N length_t of N bits (8, 16, 32 or 64)\n"
+          "  --tables FILE  load a grammar's tables file (alpacc_amd.gen hip)\n"
+          "  --names    print the terminal and production names and exit\n"
           "  -h/--help  show this message\n",
           prog);
 }
@@ -198,7 +223,7 @@ int main(int argc, char** argv) {
     }
   }
   const char *inp = nullptr, *outp = nullptr;
-  bool timeit = false, server = false, layout = false;
+  bool timeit = false, server = false, layout = false, names = false;
   int device = 0;
   for (int i = 1; i < argc; ++i) {
     std::string a = argv[i];
@@ -223,6 +248,15 @@ int main(int argc, char** argv) {
       outp = argv[++i];
     } else if (a == "--device" && i + 1 < argc) {
       device = atoi(argv[++i]);
+    } else if (a == "--names") {
+      names = true;
+    } else if (a == "--tables" && i + 1 < argc) {
+      char nm[256];
+      if (alpacc_hip_load_grammar(argv[++i], nm, sizeof nm) < 0) {
+        fprintf(stderr, "error: %s\n", alpacc_hip_last_error());
+        return 1;
+      }
+      if (grammar.empty()) grammar = nm;
     } else if (a == "--index32") {
       p.index32 = 1;
     } else if (a == "--length" && i + 1 < argc) {
@@ -249,6 +283,23 @@ int main(int argc, char** argv) {
       fputs(buf, stdout);
       return 0;
     }
+  }
+  if (names) {
+    const int kinds[2] = {p.mode == ALPACC_MODE_LEXER ? ALPACC_NAMES_TERMINAL_LEXER : ALPACC_NAMES_TERMINAL,
+                          ALPACC_NAMES_PRODUCTION};
+    const char* label[2] = {"terminal", "production"};
+    for (int k = 0; k < (p.mode == ALPACC_MODE_LEXER ? 1 : 2); ++k) {
+      const int n = alpacc_hip_name_count(grammar.c_str(), kinds[k]);
+      for (int id = 0; id < n; ++id) {
+        char nm[256];
+        if (alpacc_hip_name(grammar.c_str(), kinds[k], (uint64_t)id, nm, sizeof nm) < 0) {
+          fprintf(stderr, "error: %s\n", alpacc_hip_last_error());
+          return 1;
+        }
+        printf("%s %d %s\n", label[k], id, nm);
+      }
+    }
+    return 0;
   }
   FILE* in = inp ? fopen(inp, "rb") : stdin;
   FILE* out = outp ? fopen(outp, "wb") : stdout;

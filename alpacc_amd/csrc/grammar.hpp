@@ -1,7 +1,10 @@
 // Device-table descriptor of one grammar, filled by the sources that
 // alpacc_amd/gen/emit_hip.py generates (one per grammar).
 #pragma once
+#include <cstddef>
 #include <cstdint>
+#include <string>
+#include <vector>
 
 namespace alpacc {
 
@@ -48,8 +51,25 @@ struct GrammarDesc {
   int max_spop, max_spush;
   int max_nt;                    // terminal productions per window
   int length_param;              // the grammar's `length` (bits), 0 when it has none: length_t follows index_t
+  int has_parser;                // 0: terminal rules only, a lexer-only grammar (its parser fields are empty)
+  // names by id, NUL-separated: terminals in lexer-only and in combined
+  // numbering (the dead terminal last), productions
+  const uint8_t* terminal_names_lex;
+  const uint8_t* terminal_names_both;
+  const uint8_t* production_names;
 };
 
+// A grammar by name: one loaded at run time (alpacc_hip_load_grammar), else
+// one compiled into the library.
 const GrammarDesc* find_grammar(const char* name);
+// The compiled-in grammars (generated registry).
+const GrammarDesc* find_builtin_grammar(const char* name);
+// Names of every grammar, compiled-in and loaded, null-terminated.
+const char* const* grammar_names();
+const char* const* builtin_grammar_names();
+// Parses a tables file (alpacc_amd/gen/emit_hip.py emit_tables) into a
+// descriptor whose arrays live in *storage; returns 0 or an error.
+int parse_grammar_tables(const uint8_t* data, size_t len, GrammarDesc* out, std::vector<std::vector<uint8_t>>* storage,
+                         std::string* name);
 
 }  // namespace alpacc

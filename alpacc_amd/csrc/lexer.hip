@@ -38,6 +38,7 @@
 //                    the LDS holds terminals for re-walks its bytes instead.
 #include "device.hpp"
 #include "lexer.hpp"
+#include "runtime.hpp"
 
 namespace alpacc {
 
@@ -750,7 +751,7 @@ hipError_t occupancy_grid(K k, size_t lds, int wpc, int* grid) {
   if (e != hipSuccess) return e;
   e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  e = grant_lds(reinterpret_cast<const void*>(k), lds);
   if (e != hipSuccess) return e;
   e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(k), WG, lds);
   if (e != hipSuccess) return e;

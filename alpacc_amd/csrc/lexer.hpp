@@ -12,6 +12,10 @@ constexpr int LEX_WG = 256;   // lanes per workgroup (4 waves)
 constexpr int LEX_IPT = 128;  // input bytes per lane (a chunk) by default; 64 is the other shape
 constexpr int LEX_LB_WIN = 16;  // predecessor tiles per lookback probe by default
 
+// Input bytes of one lexer launch: positions and counts travel in 31-bit
+// fields of the lookback words.
+constexpr uint64_t LEX_MAX_LAUNCH_BYTES = (1ull << 31) - 1;
+
 constexpr uint32_t LEX_REJECT_FINAL = 1u;     // last state not accepting
 constexpr uint32_t LEX_REJECT_OVERFLOW = 2u;  // a kept token is longer than length_t
 

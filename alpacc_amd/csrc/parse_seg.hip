@@ -1067,6 +1067,14 @@ __global__ __launch_bounds__(PLAN_WG) void parse_plan(const uint64_t* tok_off, c
 
 }  // namespace
 
+int parse_per_position(const GrammarDesc* g) {
+  return std::max(std::max(8, std::max(g->max_bpush, g->max_spush)), std::max(g->max_bpop, g->max_spop));
+}
+
+uint64_t parse_max_positions(const GrammarDesc* g) {
+  return (0x7FFFFFFFull - 1) / (uint64_t)(parse_per_position(g) + 1);
+}
+
 uint32_t bracket_symbol_max(const GrammarDesc* g) {  // bytes 8..23 of each pair record, u16
   uint32_t mx = 0;
   for (int i = 0; i <= g->npairs; ++i)
@@ -1079,6 +1087,7 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
                           const uint32_t* h_skip, uint64_t count, bool tree, hipStream_t st, const BatchCaps* caps,
                           const uint32_t* d_skip, const uint32_t* d_live) {
   const GrammarDesc* g = c.g;
+  if (!g->has_parser) return fail(ALPACC_ERR_ARG, "grammar '%s' has no production rules: lexer mode only", g->name);
   if (count == 0 || count > 0xFFFFFFFEull) return fail(ALPACC_ERR_ARG, "batch of %llu inputs", (unsigned long long)count);
   if (caps && (count != caps->count || !d_skip || !d_live))
     return fail(ALPACC_ERR_ARG, "batch beyond the capacity its launch sequence is sized for");
@@ -1124,11 +1133,12 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
     h_tf[count] = tt;
     h_tf[count + 1] = tt;
   }
-  // depths grow by at most the pushes of a window per position (at
-  // capacity that bound itself; else 8 per position, the bound the tiles
-  // were laid out for)
-  const uint64_t per_pos = caps ? (uint64_t)std::max(1, std::max(g->max_bpush, g->max_spush)) : 8;
-  if (ntiles > 0x7FFFFFFFull || positions * per_pos + (caps ? count : 0) > 0x7FFFFFFFull)
+  // depths, list offsets and the tiles' depth minima move by at most one
+  // window's pushes or pops per position: positions times the largest of
+  // those (and 8, the bound the tiles were laid out for) bounds every 31-bit
+  // field of the scans, accepted input or not
+  const uint64_t per_pos = (uint64_t)parse_per_position(g);
+  if (ntiles > 0x7FFFFFFFull || positions * per_pos + count > 0x7FFFFFFFull)
     return fail(ALPACC_ERR_LIMIT, "batch of %llu framed positions exceeds the parser's 31-bit depths",
                 (unsigned long long)positions);
 
@@ -1233,7 +1243,9 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
     so = *c.h_pres.as<ScanOut>();
   }
   const uint64_t np = (uint64_t)so.np, nterm = (uint64_t)so.nt;
-  if (so.np > 0x7FFFFFFF && !caps) return fail(ALPACC_ERR_LIMIT, "derivation beyond 2^31 entries");
+  if (!caps && (so.np > 0x7FFFFFFF || so.ob > 0x7FFFFFFF || so.ub > 0x7FFFFFFF || so.os > 0x7FFFFFFF ||
+                so.us > 0x7FFFFFFF))
+    return fail(ALPACC_ERR_LIMIT, "derivation or bracket lists beyond 2^31 entries");
   const uint64_t nodes = caps ? np : np - nterm;
   if ((r = c.ps_b.reserve((size_t)(so.ob + so.ub) * 2 + 64)) || (r = c.ps_c.reserve((size_t)(so.os + so.us) * 8 + 64)) ||
       (tree && ((r = c.tr_prod.reserve(nodes + 16)) || (r = c.tr_parent.reserve(nodes * 8 + 16)) ||
@@ -1270,12 +1282,7 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
       resolve_kernels<3, 4, 4, 4>(br8, kern);
     else
       resolve_kernels<4, 4, 8, 24>(br8, kern);
-    if (!c.resolve_lds_set || c.resolve_lds_set < lds_b) {
-      for (auto k : kern)
-        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)lds_b));
-      c.resolve_lds_set = lds_b;
-    }
+    for (auto k : kern) HIP_TRY(grant_lds(reinterpret_cast<const void*>(k), lds_b));
     if (!caps) HIP_TRY(tm.begin(st, "seg_resolve"));
     hipLaunchKernelGGL(kern[caps ? 1 : 0], dim3(g_res), dim3(RWG), lds_r, st, rs);
     if (!caps) HIP_TRY(tm.end(st));

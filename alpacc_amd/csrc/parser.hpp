@@ -36,11 +36,19 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
                           const BatchCaps* caps = nullptr, const uint32_t* d_skip = nullptr,
                           const uint32_t* d_live = nullptr);
 
-// One string of n ids: returns 1 (valid) / 0 (rejected) / < 0 (error) and
-// the derivation length in *num_prods (trees: nodes = *num_prods - n).
+// Bound on how far one framed position moves any depth or list offset of
+// the parser's scans: the largest of a window's bracket / slot pushes and
+// pops, and 8.  Positions times this must stay below 2^31 in one launch.
+int parse_per_position(const GrammarDesc* g);
+// Framed positions one parser launch takes for sure (any count of inputs
+// up to this many): positions * per_position + inputs < 2^31.
+uint64_t parse_max_positions(const GrammarDesc* g);
+
 // Largest bracket symbol of the grammar's pair records.
 uint32_t bracket_symbol_max(const GrammarDesc* g);
 
+// One string of n ids: returns 1 (valid) / 0 (rejected) / < 0 (error) and
+// the derivation length in *num_prods (trees: nodes = *num_prods - n).
 int parse_on_device(Context& c, const uint8_t* d_tok, uint64_t n, bool tree, hipStream_t st, uint64_t* num_prods);
 
 }  // namespace alpacc

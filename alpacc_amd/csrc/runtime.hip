@@ -6,12 +6,16 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/alpacc_hip.h"
 #include "grammar.hpp"
 #include "lexer.hpp"
+#include "parser.hpp"
 #include "runtime.hpp"
 
 namespace alpacc {
@@ -203,6 +207,8 @@ void alpacc_hip_close(alpacc_hip_ctx* h) {
 }
 
 static int layout_text(const GrammarDesc* g, int mode, char* buf, size_t cap) {
+  if (mode != ALPACC_MODE_LEXER && !g->has_parser)
+    return fail(ALPACC_ERR_ARG, "grammar '%s' has no production rules: lexer mode only", g->name);
   std::string s;
   char line[64];
   int tb = mode == ALPACC_MODE_LEXER ? g->terminal_bytes_lex : g->terminal_bytes_both;
@@ -278,6 +284,48 @@ int alpacc_hip_open_variant(const char* grammar, int device, int index32, int le
   return 0;
 }
 
+int alpacc_hip_grammar_limits(const char* grammar, uint64_t* launch_bytes, uint64_t* parser_positions) {
+  const GrammarDesc* g = grammar ? find_grammar(grammar) : nullptr;
+  if (!g || !launch_bytes || !parser_positions) return fail(ALPACC_ERR_ARG, "unknown grammar or null argument");
+  *launch_bytes = LEX_MAX_LAUNCH_BYTES;
+  *parser_positions = g->has_parser ? parse_max_positions(g) : 0;
+  return 0;
+}
+
+// The NUL-separated names of one kind.
+static int names_of(const char* grammar, int kind, const uint8_t** names, int* count) {
+  const GrammarDesc* g = grammar ? find_grammar(grammar) : nullptr;
+  if (!g) return fail(ALPACC_ERR_ARG, "unknown grammar '%s'", grammar ? grammar : "(null)");
+  switch (kind) {
+    case ALPACC_NAMES_TERMINAL_LEXER: *names = g->terminal_names_lex; *count = g->dead_lex + 1; break;
+    case ALPACC_NAMES_TERMINAL: *names = g->terminal_names_both; *count = g->dead_both + 1; break;
+    case ALPACC_NAMES_PRODUCTION: *names = g->production_names; *count = g->has_parser ? g->nproductions : 0; break;
+    default: return fail(ALPACC_ERR_ARG, "bad name kind %d", kind);
+  }
+  return 0;
+}
+
+int alpacc_hip_name_count(const char* grammar, int kind) {
+  const uint8_t* names;
+  int count;
+  if (int r = names_of(grammar, kind, &names, &count)) return r;
+  return count;
+}
+
+int alpacc_hip_name(const char* grammar, int kind, uint64_t id, char* buf, size_t cap) {
+  const uint8_t* names;
+  int count;
+  if (!buf) return fail(ALPACC_ERR_ARG, "null argument");
+  if (int r = names_of(grammar, kind, &names, &count)) return r;
+  if (id >= (uint64_t)count) return fail(ALPACC_ERR_ARG, "id %llu of %d", (unsigned long long)id, count);
+  const char* p = reinterpret_cast<const char*>(names);
+  for (uint64_t i = 0; i < id; ++i) p += strlen(p) + 1;
+  const size_t n = strlen(p);
+  if (n + 1 > cap) return fail(ALPACC_ERR_ARG, "name buffer too small");
+  memcpy(buf, p, n + 1);
+  return (int)n;
+}
+
 uint64_t alpacc_hip_grammar_payload_bytes(const char* grammar, int mode, uint64_t n) {
   const GrammarDesc* g = grammar ? find_grammar(grammar) : nullptr;
   if (!g) return 0;
@@ -336,6 +384,20 @@ const char* const* alpacc_hip_grammars(void) { return grammar_names(); }
 }  // extern "C"
 
 namespace alpacc {
+
+hipError_t grant_lds(const void* k, size_t bytes) {
+  static std::mutex mu;
+  static std::map<std::pair<int, const void*>, size_t> granted;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lock(mu);
+  size_t& g = granted[{dev, k}];
+  if (g >= bytes) return hipSuccess;
+  e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e == hipSuccess) g = bytes;
+  return e;
+}
 
 hipError_t KernelTimer::begin(hipStream_t st, const char* name) {
   if (!on) return hipSuccess;

@@ -82,7 +82,6 @@ struct Context {
   // not computed yet)
   int lex_ipt = LEX_IPT, lex_lb = LEX_LB_WIN, lex_wpc = 0;
   int lex_grid = 0, lex_grid2 = 0;
-  size_t resolve_lds_set = 0;        // dynamic LDS granted to the parser's resolve kernel
   int resolve_br8 = -1;              // 1: every bracket symbol below 256 (-1: not checked yet)
   int resolve_bw = 0;                // bracket stack bytes an entry (0: 1 when resolve_br8, else 2)
   // resident tables
@@ -201,7 +200,12 @@ namespace alpacc {
 
 int fail(int code, const char* fmt, ...);
 int zero_words(void* p, size_t bytes, hipStream_t st);
-const char* const* grammar_names();
+// Raises kernel k's dynamic-LDS limit on the current device to at least
+// `bytes`.  The limit belongs to the kernel for the whole process, so the
+// grants of every context are kept in one mutex-guarded table per (device,
+// kernel) and only ever raised: a context of a smaller grammar never lowers
+// what another context's launches rely on.
+hipError_t grant_lds(const void* k, size_t bytes);
 
 // Lexer over `count` inputs packed in device memory at d_in, input i being
 // bytes [h_off[i], h_off[i + 1]) (host array).  Returns 0 or an error;

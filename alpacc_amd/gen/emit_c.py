@@ -33,6 +33,8 @@ def _b(v: bool) -> str:
 def emit(g: Grammar, mode: str) -> str:
     """mode: 'lexer' | 'parser' | 'both'."""
     assert mode in ("lexer", "parser", "both")
+    if mode != "lexer" and not g.has_parser:
+        raise ValueError(f"{g.name}: CFG has no production rules (lexer mode only)")
     out: List[str] = [f"/* generated by alpacc_amd.gen for grammar '{g.name}', mode {mode} */",
                       "#include <stdbool.h>", "#include <stdint.h>"]
     tbytes = g.terminal_bytes_lex if mode == "lexer" else g.terminal_bytes_both

@@ -85,6 +85,29 @@ def device_tables(g: Grammar) -> dict:
             out.append(t)
         return out
 
+    d = dict(
+        nstate=nstate, nendo=ne, nclass=nc, byte_class=classes, step=step, compose=compose,
+        endo_init=endo_init, delta_lex=delta(lo), delta_both=delta(lb), token_lex=tokens(lo), token_both=tokens(lb),
+        ignore_lex=-1 if lo.ignore_id is None else lo.ignore_id,
+        ignore_both=-1 if lb.ignore_id is None else lb.ignore_id,
+        dead_lex=lo.dead_id, dead_both=lb.dead_id,
+    )
+    d.update(_parser_tables(g) if g.has_parser else _no_parser(g))
+    return d
+
+
+def _no_parser(g: Grammar) -> dict:
+    """Parser fields of a lexer-only grammar: no windows, no pairs; record 0
+    (the empty record) and one production slot keep every array non-empty."""
+    return dict(
+        has_parser=0, empty_t=g.lex_both.dead_id, start_t=g.lex_both.dead_id + 1, end_t=g.lex_both.dead_id + 2,
+        window=[], base=g.lex_both.dead_id + 3, npairs=0, spans=[], stacks=[], productions=[], n_productions=0,
+        production_bytes=1, arity=[], prod_is_term=[], profile=[], max_slot_items=1, pair_rec=[0] * 64,
+        prod_info=[], max_brackets=0, max_prods=0, max_bpop=0, max_bpush=0, max_spop=0, max_spush=0, max_nt=0,
+    )
+
+
+def _parser_tables(g: Grammar) -> dict:
     p = g.parser
     if p.production_bytes != 1:
         raise EmitError(f"{g.name}: {p.n_productions} productions exceed the u8 production field")
@@ -203,66 +226,115 @@ def device_tables(g: Grammar) -> dict:
     max_slot_items = max([(p1 - p0) + sum(p.arity[x] for x in p.productions[p0:p1])
                           for (_, _, p0, p1) in p.spans] + [1])
     return dict(
-        nstate=nstate, nendo=ne, nclass=nc, byte_class=classes, step=step, compose=compose,
-        endo_init=endo_init, delta_lex=delta(lo), delta_both=delta(lb), token_lex=tokens(lo), token_both=tokens(lb),
-        ignore_lex=-1 if lo.ignore_id is None else lo.ignore_id,
-        ignore_both=-1 if lb.ignore_id is None else lb.ignore_id,
-        dead_lex=lo.dead_id, dead_both=lb.dead_id,
-        window=window, base=base, spans=spans, stacks=stacks16, productions=p.productions,
+        has_parser=1, empty_t=p.empty_t, start_t=p.start_t, end_t=p.end_t,
+        window=window, base=base, npairs=len(p.keys), spans=spans, stacks=stacks16, productions=p.productions,
+        n_productions=p.n_productions, production_bytes=p.production_bytes,
         arity=p.arity, prod_is_term=[1 if t is not None else 0 for t in p.prod_terminal],
         profile=profile, max_slot_items=max_slot_items, pair_rec=rec, prod_info=prod_info,
+        max_brackets=p.max_brackets, max_prods=p.max_prods,
         max_bpop=eff["bpop"], max_bpush=eff["bpush"], max_spop=eff["spop"], max_spush=eff["spush"],
         max_nt=eff["nt"],
     )
 
 
-def emit(g: Grammar) -> str:
-    d = device_tables(g)
-    p = g.parser
-    ns = f"grammar_{g.name}"
-    src = [
-        f"// generated by alpacc_amd.gen.emit_hip for grammar '{g.name}'; do not edit",
-        '#include "../grammar.hpp"',
-        "namespace alpacc {",
-        f"namespace {ns} {{",
-        _arr("uint8_t", "BYTE_CLASS", d["byte_class"]),
-        _arr("uint16_t", "STEP", d["step"]),
-        _arr("uint16_t", "COMPOSE", d["compose"]),
-        _arr("uint8_t", "ENDO_INIT", d["endo_init"]),
-        _arr("uint16_t", "DELTA_LEX", d["delta_lex"]),
-        _arr("uint16_t", "DELTA_BOTH", d["delta_both"]),
-        _arr("uint8_t", "TOKEN_LEX", d["token_lex"]),
-        _arr("uint8_t", "TOKEN_BOTH", d["token_both"]),
-        _arr("uint16_t", "WINDOW", d["window"]),
-        _arr("int32_t", "SPANS", d["spans"]),
-        _arr("uint8_t", "PROFILE", d["profile"]),
-        "__attribute__((aligned(64)))\n" + _arr("uint8_t", "PAIR_REC", d["pair_rec"]),
-        _arr("uint8_t", "PROD_INFO", d["prod_info"]),
-        _arr("uint16_t", "STACKS", d["stacks"]),
-        _arr("uint8_t", "PRODUCTIONS", d["productions"]),
-        _arr("uint8_t", "ARITY", d["arity"]),
-        _arr("uint8_t", "PROD_IS_TERMINAL", d["prod_is_term"]),
-        "}  // namespace",
-        "",
-        f"extern const GrammarDesc GRAMMAR_{g.name.upper()} = {{",
-        f'  "{g.name}", {g.q}, {g.k},',
-        f"  {d['nstate']}, {d['nendo']}, {d['nclass']},",
-        f"  {ns}::BYTE_CLASS, {ns}::STEP, {ns}::COMPOSE, {ns}::ENDO_INIT,",
-        f"  {ns}::DELTA_LEX, {ns}::DELTA_BOTH, {ns}::TOKEN_LEX, {ns}::TOKEN_BOTH,",
-        f"  {d['ignore_lex']}, {d['ignore_both']}, {d['dead_lex']}, {d['dead_both']},",
-        f"  {g.terminal_bytes_lex}, {g.terminal_bytes_both}, {g.index_bytes}, {g.length_bytes},",
-        f"  {p.empty_t}, {p.start_t}, {p.end_t}, {d['base']},",
-        f"  {len(d['window'])}, {ns}::WINDOW, {len(p.keys)}, {ns}::SPANS, {ns}::PROFILE,",
-        f"  {max(1, len(p.stacks))}, {ns}::STACKS,",
-        f"  {max(1, len(p.productions))}, {ns}::PRODUCTIONS,",
-        f"  {p.n_productions}, {p.production_bytes}, {ns}::ARITY, {ns}::PROD_IS_TERMINAL,",
-        f"  {max(1, p.max_brackets)}, {max(1, p.max_prods)}, {d['max_slot_items']},",
-        f"  {ns}::PAIR_REC, {ns}::PROD_INFO,",
-        f"  {max(1, d['max_bpop'])}, {max(1, d['max_bpush'])}, {max(1, d['max_spop'])}, {max(1, d['max_spush'])},",
-        f"  {max(1, d['max_nt'])},",
-        f"  {g.length_bits or 0},",
-        "};",
-        "}  // namespace alpacc",
-        "",
+
+# Fields of GrammarDesc (csrc/grammar.hpp) in declaration order: a scalar
+# ("int"), the name ("str"), or an array of u8 / u16 / i32 entries.  The
+# C++ source and the binary tables file are both written from this list.
+def _fields(g: Grammar, d: dict) -> list:
+    def one(v):  # an array is never empty (the C++ arrays hold at least one entry)
+        return list(v) or [0]
+    return [
+        ("name", "str", g.name), ("q", "int", g.q), ("k", "int", g.k),
+        ("nstate", "int", d["nstate"]), ("nendo", "int", d["nendo"]), ("nclass", "int", d["nclass"]),
+        ("byte_class", "u8", d["byte_class"]), ("step", "u16", d["step"]), ("compose", "u16", d["compose"]),
+        ("endo_init", "u8", d["endo_init"]), ("delta_lex", "u16", d["delta_lex"]),
+        ("delta_both", "u16", d["delta_both"]), ("token_lex", "u8", d["token_lex"]),
+        ("token_both", "u8", d["token_both"]),
+        ("ignore_lex", "int", d["ignore_lex"]), ("ignore_both", "int", d["ignore_both"]),
+        ("dead_lex", "int", d["dead_lex"]), ("dead_both", "int", d["dead_both"]),
+        ("terminal_bytes_lex", "int", g.terminal_bytes_lex), ("terminal_bytes_both", "int", g.terminal_bytes_both),
+        ("index_bytes", "int", g.index_bytes), ("length_bytes", "int", g.length_bytes),
+        ("empty_t", "int", d["empty_t"]), ("start_t", "int", d["start_t"]), ("end_t", "int", d["end_t"]),
+        ("window_base", "int", d["base"]),
+        ("nwindow", "int", len(d["window"])), ("window", "u16", one(d["window"])),
+        ("npairs", "int", d["npairs"]), ("spans", "i32", one(d["spans"])), ("profile", "u8", one(d["profile"])),
+        ("nstacks", "int", max(1, len(d["stacks"]))), ("stacks", "u16", one(d["stacks"])),
+        ("nprods_flat", "int", max(1, len(d["productions"]))), ("productions", "u8", one(d["productions"])),
+        ("nproductions", "int", d["n_productions"]), ("production_bytes", "int", d["production_bytes"]),
+        ("arity", "u8", one(d["arity"])), ("prod_is_terminal", "u8", one(d["prod_is_term"])),
+        ("max_brackets", "int", max(1, d["max_brackets"])), ("max_prods", "int", max(1, d["max_prods"])),
+        ("max_slot_items", "int", d["max_slot_items"]),
+        ("pair_rec", "u8", d["pair_rec"]), ("prod_info", "u8", one(d["prod_info"])),
+        ("max_bpop", "int", max(1, d["max_bpop"])), ("max_bpush", "int", max(1, d["max_bpush"])),
+        ("max_spop", "int", max(1, d["max_spop"])), ("max_spush", "int", max(1, d["max_spush"])),
+        ("max_nt", "int", max(1, d["max_nt"])),
+        ("length_param", "int", g.length_bits or 0),
+        ("has_parser", "int", d["has_parser"]),
+        ("terminal_names_lex", "u8", _names(g.lex_only.names())),
+        ("terminal_names_both", "u8", _names(g.lex_both.names())),
+        ("production_names", "u8", _names(g.parser.prod_names if g.has_parser else [])),
     ]
+
+
+def _names(names) -> list:
+    """Names as one NUL-separated byte array (at least one NUL)."""
+    return list(b"".join(n.encode() + b"\0" for n in names)) or [0]
+
+
+_CTYPE = {"u8": "uint8_t", "u16": "uint16_t", "i32": "int32_t"}
+
+
+def emit(g: Grammar) -> str:
+    """C++ source of the grammar's GrammarDesc (compiled into the library)."""
+    d = device_tables(g)
+    fields = _fields(g, d)
+    ns = f"grammar_{g.name}"
+    src = [f"// generated by alpacc_amd.gen.emit_hip for grammar '{g.name}'; do not edit",
+           '#include "../grammar.hpp"', "namespace alpacc {", f"namespace {ns} {{"]
+    init = []
+    for name, kind, v in fields:
+        if kind == "str":
+            init.append(f'"{v}"')
+        elif kind == "int":
+            init.append(str(int(v)))
+        else:
+            arr = _arr(_CTYPE[kind], name.upper(), v)
+            src.append(("__attribute__((aligned(64)))\n" if name == "pair_rec" else "") + arr)
+            init.append(f"{ns}::{name.upper()}")
+    src += ["}  // namespace", "", f"extern const GrammarDesc GRAMMAR_{g.name.upper()} = {{"]
+    src += ["  " + ", ".join(init[i:i + 6]) + "," for i in range(0, len(init), 6)]
+    src += ["};", "}  // namespace alpacc", ""]
     return "\n".join(src)
+
+
+TABLES_MAGIC = b"ALPHIPT1"
+
+
+def emit_tables(g: Grammar) -> bytes:
+    """The same descriptor as a binary tables file that the library loads at
+    run time (alpacc_hip_load_grammar): the magic, a u32 field count, then
+    every field in order -- a string as u32 length + bytes, a scalar as
+    tag 'I' + i64, an array as tag 'A' + u32 entry size + u64 count + the
+    entries; each field padded to 8 bytes."""
+    import struct
+    d = device_tables(g)
+    fields = _fields(g, d)
+    out = bytearray(TABLES_MAGIC)
+    out += struct.pack("<I", len(fields))
+
+    def pad():
+        out.extend(b"\0" * (-len(out) % 8))
+
+    pad()
+    for name, kind, v in fields:
+        if kind == "str":
+            b = v.encode()
+            out += struct.pack("<I", len(b)) + b
+        elif kind == "int":
+            out += b"I" + b"\0" * 7 + struct.pack("<q", int(v))
+        else:
+            fmt, size = {"u8": ("B", 1), "u16": ("H", 2), "i32": ("i", 4)}[kind]
+            out += b"A" + b"\0" * 3 + struct.pack("<IQ", size, len(v)) + struct.pack(f"<{len(v)}{fmt}", *v)
+        pad()
+    return bytes(out)

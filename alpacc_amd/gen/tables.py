@@ -75,6 +75,16 @@ class LexerTables:
     accept: List[bool]
     identity: int
 
+    def names(self) -> List[str]:
+        """Name of every terminal id, the dead one last (nameTerminals,
+        src/Alpacc/Generator/Analyzer.hs:104-110): a named terminal by its
+        name, a literal as literal_<id>, the dead terminal as empty_<id>."""
+        out = [""] * (self.dead_id + 1)
+        for t, i in self.term_id.items():
+            out[i] = t[1] if t[0] == "T" else f"literal_{i}"
+        out[self.dead_id] = f"empty_{self.dead_id}"
+        return out
+
 
 @dataclasses.dataclass
 class ParserTables:
@@ -115,7 +125,7 @@ class Grammar:
     index32: bool
     lex_both: LexerTables              # combined / parser numbering
     lex_only: LexerTables              # lexer-only numbering
-    parser: ParserTables
+    parser: Optional[ParserTables]     # None: terminal rules only (a lexer-only grammar)
     terminal_bytes_both: int
     terminal_bytes_lex: int
     productions: List[Tuple[int, Tuple[int, ...]]] = dataclasses.field(default_factory=list)
@@ -131,10 +141,18 @@ class Grammar:
             return self.index_bytes
         return self.length_bits // 8
 
+    @property
+    def has_parser(self) -> bool:
+        return self.parser is not None
+
+    def modes(self) -> List[str]:
+        """Build modes this grammar has: every mode, or the lexer alone."""
+        return ["lexer", "parser", "both"] if self.has_parser else ["lexer"]
+
     def digest(self) -> str:
         blob = json.dumps({
             "to_state": self.lex_both.to_state, "n": self.lex_both.dfa.n,
-            "keys": self.parser.keys, "values": self.parser.values,
+            "keys": self.parser.keys if self.parser else [], "values": self.parser.values if self.parser else [],
         }).encode()
         return hashlib.sha256(blob).hexdigest()[:16]
 
@@ -209,8 +227,6 @@ def analyse(name: str, text: str, index32: bool = False, q: Optional[int] = None
             k: Optional[int] = None) -> Grammar:
     """Analyse grammar text; ``q`` / ``k`` override the grammar's params."""
     cfg = G.parse_cfg(text)
-    if not cfg.nt_rules:
-        raise G.GrammarError("CFG has no production rules.")
     if q is not None:
         cfg.params.lookback = q
     if k is not None:
@@ -251,6 +267,17 @@ def analyse(name: str, text: str, index32: bool = False, q: Optional[int] = None
         lex_ids[ignore] = ignore_lex
     dead_lex = len(lex_ids)
     lex_only = _lexer_tables(dfa, endos, lex_ids, ignore_lex, dead_lex, len(lex_ids) + 1)
+
+    if not cfg.nt_rules:
+        # a grammar of terminal rules alone builds a lexer only: the
+        # reference's mkLexer needs no productions (Analyzer.hs:113-149),
+        # while its parser and combined builds stop with "CFG has no
+        # production rules." (CFG.hs:115)
+        return Grammar(
+            name=name, cfg=cfg, terminals=terminals, q=q, k=k, length_bits=cfg.params.length,
+            index32=index32, lex_both=lex_both, lex_only=lex_only, parser=None,
+            terminal_bytes_both=uint_bytes(end_t), terminal_bytes_lex=uint_bytes(dead_lex),
+        )
 
     # ---- augmented grammar (Grammar.hs:191-381) ----
     user_nts: List[str] = []

@@ -116,6 +116,10 @@ def load_library(path: str = LIB_PATH):
                                                  C.POINTER(vp)]),
         "alpacc_hip_get_lex_shape": (i32, [vp, C.POINTER(LexShape)]),
         "alpacc_hip_autotune_lexer": (i32, [vp, i32, vp, u64, i32, C.POINTER(LexShape)]),
+        "alpacc_hip_load_grammar": (i32, [C.c_char_p, C.c_char_p, C.c_size_t]),
+        "alpacc_hip_grammar_limits": (i32, [C.c_char_p, C.POINTER(u64), C.POINTER(u64)]),
+        "alpacc_hip_name_count": (i32, [C.c_char_p, i32]),
+        "alpacc_hip_name": (i32, [C.c_char_p, i32, u64, C.c_char_p, C.c_size_t]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -139,6 +143,41 @@ def grammars():
         out.append(arr[i].decode())
         i += 1
     return out
+
+
+def load_grammar(path: str) -> str:
+    """Register a grammar's tables file (python -m alpacc_amd.gen hip) with
+    the library; returns its name."""
+    lib = load_library()
+    buf = C.create_string_buffer(256)
+    _check(min(lib.alpacc_hip_load_grammar(path.encode(), buf, 256), 0))
+    return buf.value.decode()
+
+
+NAME_KINDS = {"terminal_lexer": 0, "terminal": 1, "production": 2}
+
+
+def names(grammar: str, kind: str):
+    """Names of every id of a kind: terminal_lexer (lexer-only numbering),
+    terminal (parser / combined numbering) or production."""
+    lib = load_library()
+    n = lib.alpacc_hip_name_count(grammar.encode(), NAME_KINDS[kind])
+    _check(min(n, 0))
+    buf = C.create_string_buffer(512)
+    out = []
+    for i in range(n):
+        _check(min(lib.alpacc_hip_name(grammar.encode(), NAME_KINDS[kind], i, buf, 512), 0))
+        out.append(buf.value.decode())
+    return out
+
+
+def limits(grammar: str):
+    """(input bytes one lexer launch takes, framed positions one parser
+    launch takes) for a grammar."""
+    lib = load_library()
+    a, b = C.c_uint64(), C.c_uint64()
+    _check(min(lib.alpacc_hip_grammar_limits(grammar.encode(), C.byref(a), C.byref(b)), 0))
+    return a.value, b.value
 
 
 def _check(rc: int) -> bool:
@@ -187,7 +226,12 @@ class Backend:
         else:
             _check(self.lib.alpacc_hip_open(grammar.encode(), device, C.byref(h)))
         self.h = h
-        self.layouts = {m: self._layout(m) for m in MODES}
+        # the modes the grammar has (a lexer-only grammar: the lexer alone)
+        self.layouts = {}
+        for m in MODES:
+            buf = C.create_string_buffer(512)
+            if self.lib.alpacc_hip_layout(self.h, MODES[m], buf, 512) >= 0:
+                self.layouts[m] = self._layout(m)
 
     def close(self):
         if self.h:

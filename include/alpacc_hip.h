@@ -79,8 +79,41 @@ int alpacc_hip_variant_layout(const char *grammar, int mode, int index32, int le
 void alpacc_hip_close(alpacc_hip_ctx *ctx);
 const char *alpacc_hip_last_error(void);
 
-/* Names of the compiled-in grammars, NULL-terminated. */
+/* Names of the grammars: those compiled in, then those loaded, NULL-terminated. */
 const char *const *alpacc_hip_grammars(void);
+
+/* Load a grammar's tables file at run time (the output of `python -m
+ * alpacc_amd.gen hip GRAMMAR.alp -o FILE`): the counterpart of generating
+ * one program per grammar with `alpacc c|cuda GRAMMAR.alp` (app/Main.hs:
+ * 393-402, src/Alpacc/Generator/C/Generator.hs:161-207).  The grammar is
+ * registered under its name (written to name, if not NULL) for the rest of
+ * the process and opens like a compiled-in one.  Loading the same file
+ * again is a no-op; another grammar of the same name is refused. */
+int alpacc_hip_load_grammar(const char *path, char *name, size_t name_cap);
+
+/* Size limits of one launch for a grammar, without a device: the input
+ * bytes one lexer launch takes (alpacc_hip_run_batch and run_case split
+ * larger batches into several launches; a larger lexer-only input goes
+ * through the stream path, a larger input of the other modes is refused
+ * with ALPACC_ERR_LIMIT) and the framed parser positions (tokens + 2 per
+ * input) one parser launch is sure to take. */
+int alpacc_hip_grammar_limits(const char *grammar, uint64_t *launch_bytes, uint64_t *parser_positions);
+
+/* Names of ids, as the reference's Futhark target names them in its
+ * `terminal` and `production` types (src/Alpacc/Generator/Futhark/
+ * Generator.hs:91-118, names from src/Alpacc/Generator/Analyzer.hs:104-110,
+ * 158-165 and src/Alpacc/Encode.hs:334-341): a named terminal by its name,
+ * a literal as literal_<id>, the dead terminal as empty_<id>; a production
+ * by its [Name] in the grammar, else <lhs>_<id>, Literal_<id> or start_<id>.
+ * name_count returns how many ids there are (or an error); name writes the
+ * id's name into buf and returns its length. */
+enum alpacc_hip_names {
+  ALPACC_NAMES_TERMINAL_LEXER = 0, /* terminal ids of lexer-only mode */
+  ALPACC_NAMES_TERMINAL = 1,       /* terminal ids of parser-only and combined modes */
+  ALPACC_NAMES_PRODUCTION = 2
+};
+int alpacc_hip_name_count(const char *grammar, int kind);
+int alpacc_hip_name(const char *grammar, int kind, uint64_t id, char *buf, size_t cap);
 
 /* `--layout` (print_layout, src/Alpacc/Generator/C/Generator.hs:64-110):
  * key=value lines into buf; returns the length or a negative error. */

@@ -10,6 +10,8 @@ oracle.c twice with gcc into oracle/build/:
 Width variants (the generator's --index32 flag, a `length` other than the
 grammar's) get a suffix: <grammar>_<mode>_i32, ..._l8, ..._i32_l8.
 
+Grammars come from grammars/ and (libraries only) from tests/golden/grammars/.
+
 Usage: python oracle/build.py [grammar ...]
 """
 
@@ -28,6 +30,23 @@ from alpacc_amd.gen import emit_c, tables  # noqa: E402
 MODES = ("lexer", "parser", "both")
 BUILD = os.path.join(HERE, "build")
 GRAMMARS = os.path.join(ROOT, "grammars")
+# grammars of the parity tests only (generated by alpacc_amd.gen random):
+# oracles only, no programs
+TEST_GRAMMARS = os.path.join(ROOT, "tests", "golden", "grammars")
+
+
+def grammar_path(name: str) -> str:
+    for d in (GRAMMARS, TEST_GRAMMARS):
+        p = os.path.join(d, name + ".alp")
+        if os.path.exists(p):
+            return p
+    raise FileNotFoundError(f"no grammar named {name}")
+
+
+def grammar_names(d: str):
+    if not os.path.isdir(d):
+        return []
+    return sorted(os.path.splitext(f)[0] for f in os.listdir(d) if f.endswith(".alp"))
 
 
 def variant_suffix(index32: bool = False, length: int = 0) -> str:
@@ -36,13 +55,15 @@ def variant_suffix(index32: bool = False, length: int = 0) -> str:
 
 def build_grammar(name: str, cc: str = "gcc", index32: bool = False, length: int = 0, modes=MODES,
                   programs: bool = True) -> None:
-    g = tables.load(os.path.join(GRAMMARS, name + ".alp"), index32=index32)
+    g = tables.load(grammar_path(name), index32=index32)
     if length:
         g.length_bits = length
     sfx = variant_suffix(index32, length)
     os.makedirs(BUILD, exist_ok=True)
     src = os.path.join(HERE, "oracle.c")
     for mode in modes:
+        if mode not in g.modes():  # a lexer-only grammar has no parser builds
+            continue
         hdr = os.path.join(BUILD, f"{name}_{mode}{sfx}.h")
         text = emit_c.emit(g, mode)
         old = open(hdr).read() if os.path.exists(hdr) else None
@@ -68,13 +89,19 @@ def build_variant(name: str, index32: bool, length: int) -> None:
 
 
 def main(argv):
-    names = argv or sorted(os.path.splitext(f)[0] for f in os.listdir(GRAMMARS) if f.endswith(".alp"))
+    import concurrent.futures as cf
+    names = argv or grammar_names(GRAMMARS)
+    tests = [] if argv else grammar_names(TEST_GRAMMARS)
     for n in names:
         build_grammar(n)
         if n in VARIANT_GRAMMARS:
             for index32, length in VARIANTS:
                 build_variant(n, index32, length)
         print(f"oracle: built {n}", flush=True)
+    if tests:
+        with cf.ThreadPoolExecutor(max_workers=8) as ex:
+            list(ex.map(lambda n: build_grammar(n, programs=False), tests))
+        print(f"oracle: built {len(tests)} test grammars", flush=True)
 
 
 if __name__ == "__main__":

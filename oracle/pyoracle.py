@@ -22,9 +22,9 @@ def _sfx(index32: bool, length: int) -> str:
     return ("_i32" if index32 else "") + (f"_l{length}" if length else "")
 
 
-def ensure_built(grammar: str, index32: bool = False, length: int = 0) -> None:
+def ensure_built(grammar: str, index32: bool = False, length: int = 0, mode: str = None) -> None:
     sfx = _sfx(index32, length)
-    modes = ("lexer", "both") if sfx else ("lexer", "parser", "both")
+    modes = (mode,) if mode else (("lexer", "both") if sfx else ("lexer", "parser", "both"))
     if all(os.path.exists(os.path.join(BUILD, f"lib{grammar}_{m}{sfx}.so")) for m in modes):
         return
     if sfx:
@@ -38,7 +38,7 @@ def ensure_built(grammar: str, index32: bool = False, length: int = 0) -> None:
 def _lib(grammar: str, mode: str, index32: bool = False, length: int = 0):
     key = (grammar, mode, index32, length)
     if key not in _libs:
-        ensure_built(grammar, index32, length)
+        ensure_built(grammar, index32, length, mode)
         lib = C.CDLL(os.path.join(BUILD, f"lib{grammar}_{mode}{_sfx(index32, length)}.so"))
         lib.oracle_response.restype = C.c_int
         lib.oracle_response.argtypes = [C.c_uint64, C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64)]
@@ -83,5 +83,5 @@ def time_case(grammar: str, mode: str, payload, n: int) -> float:
 
 
 def program(grammar: str, mode: str) -> str:
-    ensure_built(grammar)
+    ensure_built(grammar, mode=mode)
     return os.path.join(BUILD, f"{grammar}_{mode}")
