@@ -35,7 +35,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
-HIP_SOURCES = ["lexer.hip", "parse_seg.hip", "runtime.hip", "api_lex.hip", "api_parse.hip", "api_case.hip", "api_batch.hip", "serve.hip", "tables_file.cpp"]
+HIP_SOURCES = ["lexer.hip", "parse_seg.hip", "runtime.hip", "api_lex.hip", "api_parse.hip", "api_case.hip", "api_batch.hip", "serve.hip", "api_noalloc.hip", "tables_file.cpp"]
 
 LIB_NAME = "libalpacc_hip.so"
 CLI_NAME = "alpacc-hip"

@@ -198,6 +198,23 @@ int alpacc_hip_parse_device(alpacc_hip_ctx *ctx, const void *d_tokens, uint64_t 
 int alpacc_hip_lex_parse_device(alpacc_hip_ctx *ctx, const uint8_t *d_str, uint64_t n,
                                 alpacc_hip_tree *d_out, void *stream);
 
+/* alpacc_hip_lex_device without allocating: no reference counterpart; for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The lexer's scratch memory is a
+ * static device array of ALPACC_NOALLOC_SCRATCH_BYTES on every device the
+ * library runs on (held until the process ends); the outputs are the
+ * caller's device arrays of `cap` entries (cap >= n: a token holds at least
+ * one byte) -- terminal_t, int64 start, length_t -- and *count receives the
+ * number of tokens.  Inputs above ALPACC_NOALLOC_MAX_INPUT bytes are refused
+ * with ALPACC_ERR_LIMIT.  Results, return codes and the context's lexer
+ * shape are those of alpacc_hip_lex_device.  Calls that overlap on one
+ * device share its scratch array and race: serialise them.  The context
+ * must have been opened (opening allocates its tables). */
+#define ALPACC_NOALLOC_SCRATCH_BYTES (3u << 20)
+#define ALPACC_NOALLOC_MAX_INPUT (64ull << 20)
+int alpacc_hip_lex_device_noalloc(alpacc_hip_ctx *ctx, int mode, const uint8_t *d_str, uint64_t n, void *d_terminals,
+                                  int64_t *d_starts, void *d_lengths, uint64_t cap, uint64_t *count, void *stream);
+
 /* Lexing one input that arrives in chunks (the reference's chunked CUDA
  * lexer, LexerCtx and its update between chunks, backends/cuda/lexer.cu:
  * 224-431, and lexer.fut:75-196): begin, then push the chunks in order, the
