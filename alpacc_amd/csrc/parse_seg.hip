@@ -1068,7 +1068,7 @@ __global__ __launch_bounds__(PLAN_WG) void parse_plan(const uint64_t* tok_off, c
 }  // namespace
 
 int parse_per_position(const GrammarDesc* g) {
-  return std::max(std::max(8, std::max(g->max_bpush, g->max_spush)), std::max(g->max_bpop, g->max_spop));
+  return std::max(std::max(1, std::max(g->max_bpush, g->max_spush)), std::max(g->max_bpop, g->max_spop));
 }
 
 uint64_t parse_max_positions(const GrammarDesc* g) {
@@ -1135,8 +1135,7 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
   }
   // depths, list offsets and the tiles' depth minima move by at most one
   // window's pushes or pops per position: positions times the largest of
-  // those (and 8, the bound the tiles were laid out for) bounds every 31-bit
-  // field of the scans, accepted input or not
+  // those bounds every 31-bit field of the scans, accepted input or not
   const uint64_t per_pos = (uint64_t)parse_per_position(g);
   if (ntiles > 0x7FFFFFFFull || positions * per_pos + count > 0x7FFFFFFFull)
     return fail(ALPACC_ERR_LIMIT, "batch of %llu framed positions exceeds the parser's 31-bit depths",

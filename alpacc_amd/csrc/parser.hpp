@@ -38,7 +38,7 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
 
 // Bound on how far one framed position moves any depth or list offset of
 // the parser's scans: the largest of a window's bracket / slot pushes and
-// pops, and 8.  Positions times this must stay below 2^31 in one launch.
+// pops.  Positions times this must stay below 2^31 in one launch.
 int parse_per_position(const GrammarDesc* g);
 // Framed positions one parser launch takes for sure (any count of inputs
 // up to this many): positions * per_position + inputs < 2^31.

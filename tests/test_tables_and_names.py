@@ -215,13 +215,13 @@ def test_dev_reports():
 @needs_lib
 def test_launch_limits():
     """One parser launch takes positions p with p * per_position + inputs <
-    2^31, per_position the largest push or pop count of a window (and 8);
+    2^31, per_position the largest push or pop count of a window;
     bigger batches are split, a bigger single input is refused."""
     from alpacc_amd import hip
     for name in ("json", "arithmetic", "sexp"):
         g = load(os.path.join(ROOT, "grammars", name + ".alp"))
         d = emit_hip.device_tables(g)
-        per = max(8, d["max_bpush"], d["max_spush"], d["max_bpop"], d["max_spop"])
+        per = max(1, d["max_bpush"], d["max_spush"], d["max_bpop"], d["max_spop"])
         lex_bytes, pos = hip.limits(name)
         assert lex_bytes == (1 << 31) - 1
         assert pos * per + pos < (1 << 31) <= (pos + 2) * (per + 1) + 1
