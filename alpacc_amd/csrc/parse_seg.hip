@@ -63,6 +63,7 @@ constexpr uint32_t NO_PAIR = 0xFFFFu;
 // Batch and grammar, shared by the passes.
 struct Seg {
   const uint8_t* tok;          // token ids of all inputs
+  const uint8_t* tok_end;      // the end of the token buffer: reads stay below it
   const uint64_t* tok_off;     // [count + 1] first token of each input
   const uint32_t* tile_first;  // [count + 1] first tile of each input (inputs not parsed have none)
   const uint32_t* tile_seg;    // [ntiles] input of each tile
@@ -208,77 +209,108 @@ __device__ __forceinline__ uint32_t rec_owner(const uint4& r2, const uint4& r3, 
 
 // ---------------------------------------------------------------- pass A
 
-// Both tile passes: APL consecutive positions per lane, PT / APL lanes.
-constexpr int APL = 4;
-constexpr int AWG = PT / APL;
-constexpr int ANW = AWG / WAVE;
+// Pass A: one wave per tile, PPL consecutive positions per lane, so every
+// lane has 16 independent window-index loads and then 16 independent
+// record-head loads in flight, and the tile's sums and depth profiles are
+// wave reductions (no workgroup barrier).
+constexpr int PPL = PT / WAVE;
+constexpr int AWG = 256;  // four tiles per workgroup
+constexpr int AWAVES = AWG / WAVE;
 
-__device__ __forceinline__ void profile_tile(const Seg& s, uint32_t tile, uint32_t tid, TileRec* rec, uint16_t* pid) {
-  const uint32_t lane = tid & 63, wid = tid >> 6;
+// Pair ids of framed positions p0 .. p0 + N - 1 for windows of exactly W
+// = q + k tokens.  Lanes whose windows lie inside the input's tokens (all
+// but the lanes at an input's two ends) load their N + W - 1 tokens
+// directly and roll the window code along; the others take window_ids.
+template <int W, int N>
+__device__ __forceinline__ void window_ids_exact(const Seg& s, const TileOf& t, int32_t p0, uint32_t* ids) {
+  const int32_t f0 = p0 - s.q;  // framed index of the first token the windows read
+  if (f0 > 0 && f0 + N + W - 2 < (int32_t)t.m - 1) {
+    uint32_t v[N + W - 1];
+    const uint8_t* src = t.tk + (f0 - 1);
+    // the tokens' dwords: ND dword-aligned loads, realigned in registers,
+    // when they lie inside the token buffer; else one load per token
+    constexpr int ND = (N + W - 1 + 3) / 4 + 1;
+    const uint32_t shift = (uint32_t)((uintptr_t)src & 3u);
+    const uint8_t* lo = src - shift;
+    if (lo >= s.tok && lo + 4 * ND <= s.tok_end) {
+      typedef uint32_t u32a4 __attribute__((aligned(4)));
+      const u32a4* d = reinterpret_cast<const u32a4*>(lo);
+      uint32_t raw[ND];
+#pragma unroll
+      for (int k = 0; k < ND; ++k) raw[k] = d[k];
+#pragma unroll
+      for (int i = 0; i < N + W - 1; ++i) {
+        const uint32_t wk = __builtin_amdgcn_alignbyte(raw[i / 4 + 1], raw[i / 4], shift);
+        v[i] = (wk >> (8 * (i % 4))) & 0xFFu;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < N + W - 1; ++i) v[i] = src[i];
+    }
+    const uint32_t base = (uint32_t)s.base;
+    uint32_t top = 1;  // base^(W - 1)
+#pragma unroll
+    for (int j = 1; j < W; ++j) top *= base;
+    uint32_t code = 0, bad = 0;
+#pragma unroll
+    for (int j = 0; j < W - 1; ++j) {
+      code = code * base + v[j];
+      bad += v[j] >= base;
+    }
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+      code = code * base + v[r + W - 1];
+      bad += v[r + W - 1] >= base;
+      ids[r] = bad ? NO_PAIR : (uint32_t)s.window[code];
+      code -= v[r] * top;
+      bad -= v[r] >= base;
+    }
+  } else {
+    window_ids<W, N>(s, t, p0, ids);
+  }
+}
+
+template <int W>
+__device__ __forceinline__ void profile_tile(const Seg& s, uint32_t tile, uint32_t lane, TileRec* rec, uint16_t* pid) {
   const TileOf t = tile_of(s, tile);
-  const int64_t p0 = (int64_t)t.lt * PT + (int64_t)tid * APL;
+  const int32_t p0 = (int32_t)t.lt * PT + (int32_t)lane * PPL;
+  uint32_t ids[PPL];
+  if (W > 0) {
+    window_ids_exact<(W > 0 ? W : 1), PPL>(s, t, p0, ids);
+  } else if (s.q + s.k <= MAXW) {
+    window_ids<MAXW, PPL>(s, t, p0, ids);
+  } else {
+#pragma unroll
+    for (int r = 0; r < PPL; ++r) ids[r] = p0 + r < t.m ? pair_at(s, t, p0 + r) : NO_PAIR;
+  }
+  uint2 h[PPL];
+#pragma unroll
+  for (int r = 0; r < PPL; ++r)  // positions past the input's end read the empty record
+    h[r] = *reinterpret_cast<const uint2*>(s.rec + 4 * (ids[r] == NO_PAIR ? (uint32_t)s.npairs : ids[r]));
   int32_t np = 0, nt = 0;
   SM b{0, 0}, sl{0, 0};
   bool bad = false;
-  uint32_t ids[APL];
-  if (s.q + s.k <= MAXW) {
-    window_ids<MAXW, APL>(s, t, (int32_t)p0, ids);
-  } else {
 #pragma unroll
-    for (int r = 0; r < APL; ++r) ids[r] = p0 + r < t.m ? pair_at(s, t, p0 + r) : NO_PAIR;
+  for (int r = 0; r < PPL; ++r) {
+    bad |= p0 + r < t.m && ids[r] == NO_PAIR;
+    const uint2 x = h[r];
+    const int32_t bpop = x.x & 0xFF, bpush = (x.x >> 8) & 0xFF, spop = x.y & 0xFF, spush = (x.y >> 8) & 0xFF;
+    np += (x.x >> 16) & 0xFF;
+    nt += x.x >> 24;
+    b = sm_cat(b, SM{bpush - bpop, -bpop});
+    sl = sm_cat(sl, SM{spush - spop, -spop});
   }
+  uint4* dst = reinterpret_cast<uint4*>(pid + (uint64_t)tile * PT + (uint64_t)lane * PPL);
 #pragma unroll
-  for (int r = 0; r < APL; ++r) {
-    const int64_t p = p0 + r;
-    const uint32_t id = ids[r];
-    if (p < t.m) {
-      if (id == NO_PAIR) {
-        bad = true;
-      } else {
-        const uint2 h = *reinterpret_cast<const uint2*>(s.rec + 4 * id);
-        const int32_t bpop = h.x & 0xFF, bpush = (h.x >> 8) & 0xFF, spop = h.y & 0xFF, spush = (h.y >> 8) & 0xFF;
-        np += (h.x >> 16) & 0xFF;
-        nt += h.x >> 24;
-        b = sm_cat(b, SM{bpush - bpop, -bpop});
-        sl = sm_cat(sl, SM{spush - spop, -spop});
-      }
-    }
-    ids[r] = id;
-  }
-  uint2 v;
-  v.x = ids[0] | (ids[1] << 16);
-  v.y = ids[2] | (ids[3] << 16);
-  *reinterpret_cast<uint2*>(pid + (uint64_t)tile * PT + (uint64_t)tid * APL) = v;
-  __shared__ int32_t wc[2][ANW];
-  __shared__ SM wb[ANW], ws[ANW];
-  __shared__ uint32_t s_bad;
-  if (tid == 0) s_bad = 0;
+  for (int v = 0; v < PPL / 8; ++v)
+    dst[v] = uint4{ids[8 * v] | ids[8 * v + 1] << 16, ids[8 * v + 2] | ids[8 * v + 3] << 16,
+                   ids[8 * v + 4] | ids[8 * v + 5] << 16, ids[8 * v + 6] | ids[8 * v + 7] << 16};
   const uint32_t cp = wave_scan_add32((uint32_t)np), ct = wave_scan_add32((uint32_t)nt);
   const SM ib = wave_scan_sm(b), is = wave_scan_sm(sl);
-  if (lane == 63) {
-    wc[0][wid] = (int32_t)cp;
-    wc[1][wid] = (int32_t)ct;
-    wb[wid] = ib;
-    ws[wid] = is;
-  }
-  __syncthreads();
-  if (bad) s_bad = 1;
-  __syncthreads();
-  if (tid == 0) {
-    TileRec r{0, 0, 0, 0, 0, 0, 0, 0};
-    SM tb{0, 0}, tsm{0, 0};
-    for (int w = 0; w < ANW; ++w) {
-      r.np += wc[0][w];
-      r.nt += wc[1][w];
-      tb = sm_cat(tb, wb[w]);
-      tsm = sm_cat(tsm, ws[w]);
-    }
-    r.bs = tb.s;
-    r.bm = tb.m;
-    r.ss = tsm.s;
-    r.sm = tsm.m;
-    rec[tile] = r;
-    if (s_bad) atomicOr(s.flags + t.seg, BAD_PAIR);
+  const uint64_t anybad = __ballot(bad);
+  if (lane == WAVE - 1) {
+    rec[tile] = TileRec{(int32_t)cp, (int32_t)ct, ib.s, ib.m, is.s, is.m, 0, 0};
+    if (anybad) atomicOr(s.flags + t.seg, BAD_PAIR);
   }
 }
 
@@ -291,20 +323,32 @@ __device__ __forceinline__ uint32_t tid_per_tile() {
   return tid;
 }
 
-// One tile per workgroup: workgroup t takes tile t, or (STRIDE, for a grid
-// sized before the tile count is known) the grid strides over the tiles.
-// The loop is a separate instantiation because values the compiler hoists
-// out of it cost the single-tile form its occupancy.
-template <bool STRIDE>
+// One tile per wave: wave w of workgroup g takes tile AWAVES * g + w, or
+// (STRIDE, for a grid sized before the tile count is known) the waves
+// stride over the tiles.
+// W = the grammar's window width q + k (1..6), or 0 for any width.
+template <bool STRIDE, int W>
 __global__ __launch_bounds__(AWG) void seg_profile(Seg s, TileRec* rec, uint16_t* pid) {
-  const uint32_t nt = *s.nt_dev;
+  const uint32_t nt = *s.nt_dev, lane = threadIdx.x & (WAVE - 1);
+  const uint32_t w0 = blockIdx.x * AWAVES + threadIdx.x / WAVE;
   if (!STRIDE) {
-    if (blockIdx.x < nt) profile_tile(s, blockIdx.x, threadIdx.x, rec, pid);
+    if (w0 < nt) profile_tile<W>(s, w0, lane, rec, pid);
     return;
   }
-  for (uint32_t tile = blockIdx.x; tile < nt; tile += gridDim.x) {
-    profile_tile(s, tile, tid_per_tile(), rec, pid);
-    __syncthreads();
+  for (uint32_t tile = w0; tile < nt; tile += gridDim.x * AWAVES) profile_tile<W>(s, tile, lane, rec, pid);
+}
+
+typedef void (*ProfileKernel)(Seg, TileRec*, uint16_t*);
+template <bool STRIDE>
+ProfileKernel profile_kernel(int w) {
+  switch (w) {
+    case 1: return seg_profile<STRIDE, 1>;
+    case 2: return seg_profile<STRIDE, 2>;
+    case 3: return seg_profile<STRIDE, 3>;
+    case 4: return seg_profile<STRIDE, 4>;
+    case 5: return seg_profile<STRIDE, 5>;
+    case 6: return seg_profile<STRIDE, 6>;
+    default: return seg_profile<STRIDE, 0>;
   }
 }
 
@@ -462,7 +506,8 @@ __global__ __launch_bounds__(256) void tile_downsweep(Seg s, const TileRec* rec,
 
 // ---------------------------------------------------------------- pass B
 
-constexpr int RWG = AWG;          // lanes per tile
+constexpr int APL = 4;            // positions per lane in pass B
+constexpr int RWG = PT / APL;     // lanes per tile
 constexpr int RNW = RWG / WAVE;   // waves per tile
 constexpr int WLV = 6;            // in-wave sparse-table levels: blocks of 1..32 lanes
 constexpr uint8_t TERM_SLOT = 0x80u;
@@ -1168,6 +1213,8 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
   uint32_t* d_tf = reinterpret_cast<uint32_t*>(base + off_tf);
   Seg s{};
   s.tok = d_tok;
+  // the token buffer holds the batch's tokens (at capacity: its bytes)
+  s.tok_end = d_tok + (caps ? caps->bytes : h_tok_off[count]);
   s.tok_off = d_tok_off;
   s.tile_first = d_tf;
   s.tile_seg = reinterpret_cast<const uint32_t*>(base + off_ts);
@@ -1205,7 +1252,7 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
   // grids: one workgroup per tile when the tile count is known, else as
   // many as the device holds at once, striding over the tiles
   const uint32_t g_map = (uint32_t)((ntiles + 255) / 256);
-  const uint32_t g_prof = (uint32_t)(caps && ntiles > 4096 ? 4096 : ntiles);
+  const uint32_t g_prof = (uint32_t)(caps && ntiles > 4096 * AWAVES ? 4096 : (ntiles + AWAVES - 1) / AWAVES);
   const uint32_t g_res = (uint32_t)(caps && ntiles > 2048 ? 2048 : ntiles);
   // far_resolve strides: 2048 workgroups of 4 tile-waves measured faster
   // than a wave per tile (0.136 against 0.143 ms on 1 GiB of JSON)
@@ -1214,12 +1261,8 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
     hipLaunchKernelGGL(pseg_map, dim3(g_map), dim3(256), 0, st, d_tf, s.count, s.nt_dev,
                        const_cast<uint32_t*>(s.tile_seg));
     if (!caps) HIP_TRY(tm.begin(st, "seg_profile"));
-    if (caps)
-      hipLaunchKernelGGL(seg_profile<true>, dim3(g_prof), dim3(AWG), 0, st, s, rec,
-                         reinterpret_cast<uint16_t*>(base + off_pid));
-    else
-      hipLaunchKernelGGL(seg_profile<false>, dim3(g_prof), dim3(AWG), 0, st, s, rec,
-                         reinterpret_cast<uint16_t*>(base + off_pid));
+    hipLaunchKernelGGL(caps ? profile_kernel<true>(g->q + g->k) : profile_kernel<false>(g->q + g->k), dim3(g_prof),
+                       dim3(AWG), 0, st, s, rec, reinterpret_cast<uint16_t*>(base + off_pid));
     if (!caps) HIP_TRY(tm.end(st));
     if (!caps) HIP_TRY(tm.begin(st, "tile_scan"));
     hipLaunchKernelGGL(tile_block_sums, dim3(nsb), dim3(256), 0, st, s, rec, bsum, sout);
