@@ -1,5 +1,12 @@
 // Serving runtime: continuous batching over resident grammars.
 //
+// Admission: a batch takes queued requests while their estimated device
+// memory (64 bytes per input byte) fits both the server's budget (64 GiB by
+// default) and the lane: what its grow-only workspaces already hold, plus
+// the device's free memory (hipMemGetInfo) less what other lanes' admitted
+// batches may still allocate and a 1 GiB reserve.  A request larger than
+// that is a batch of its own.
+//
 // The reference serves one request at a time over the wire protocol
 // (backends/c/cli.c:167-189, docs/wire-protocols.md): every request is a
 // separate run_case.  Here every resident grammar has `lanes` contexts, each
@@ -57,6 +64,18 @@ uint64_t estimate(uint64_t n) { return n * 64 + 4096; }
 // workspace growth.
 constexpr uint64_t kBatchBytes = 8ull << 20;
 
+// Device memory left free for everything else (other processes, the HIP
+// runtime) when admission looks at the device's free memory.
+constexpr uint64_t kReserve = 1ull << 30;
+
+// Device bytes a lane's context already holds in its grow-only workspaces:
+// a batch within them allocates nothing.
+uint64_t lane_capacity(alpacc_hip_ctx* h) {
+  uint64_t cap = 0;
+  for (alpacc::DevBuf* b : h->c.all_buffers()) cap += b->cap;
+  return cap;
+}
+
 }  // namespace
 
 struct alpacc_hip_server {
@@ -68,12 +87,13 @@ struct alpacc_hip_server {
   std::condition_variable cv_work, cv_done;
   std::vector<std::deque<Req*>> q;  // [grammar * 3 + mode]
   uint64_t in_use = 0, seq = 0;
+  uint64_t growing = 0;  // device bytes that admitted batches may still allocate
   bool stop = false;
   std::vector<std::thread> workers;
   std::vector<alpacc_hip_ctx*> ctxs;
   std::unordered_map<uint64_t, std::unique_ptr<Req>> tickets;
   uint64_t next_ticket = 1;
-  uint64_t n_batches = 0, n_reqs = 0, biggest = 0;
+  uint64_t n_batches = 0, n_reqs = 0, biggest = 0, peak_in_use = 0;
   std::atomic<bool> graphs{false};  // combined batches through each lane's captured graph
 };
 
@@ -100,7 +120,9 @@ void worker(alpacc_hip_server* s, uint32_t g, alpacc_hip_ctx* ctx) {
   std::vector<size_t> rec_off;
   for (;;) {
     int mode;
-    uint64_t est = 0;
+    uint64_t est = 0, grow = 0;
+    // what the lane holds already; beyond it a batch needs free device memory
+    const uint64_t held = lane_capacity(ctx);
     {
       std::unique_lock<std::mutex> lk(s->mu);
       s->cv_work.wait(lk, [&] { return s->stop || pick(s, g) >= 0; });
@@ -108,16 +130,33 @@ void worker(alpacc_hip_server* s, uint32_t g, alpacc_hip_ctx* ctx) {
       if (mode < 0) return;  // stopping with nothing left for this grammar
       auto& d = s->q[g * 3 + mode];
       batch.clear();
-      uint64_t bytes = 0;
+      uint64_t bytes = 0, room = held;
+      bool asked = false;
       while (!d.empty() && (s->max_batch == 0 || batch.size() < s->max_batch)) {
         Req* r = d.front();
         if (!batch.empty() && (s->in_use + est + r->est > s->budget || bytes + r->n > kBatchBytes)) break;
+        if (!batch.empty() && est + r->est > room) {
+          // admission by free HBM: the device's free memory, less what
+          // other lanes' admitted batches may still allocate, less a reserve
+          if (!asked) {
+            size_t free_b = 0, total_b = 0;
+            if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+              const uint64_t avail = free_b > s->growing + kReserve ? free_b - s->growing - kReserve : 0;
+              room = held + avail;
+            }
+            asked = true;
+          }
+          if (est + r->est > room) break;
+        }
         batch.push_back(r);
         est += r->est;
         bytes += r->n;
         d.pop_front();
       }
+      grow = est > held ? est - held : 0;
       s->in_use += est;
+      s->growing += grow;
+      if (s->in_use > s->peak_in_use) s->peak_in_use = s->in_use;
     }
     packed.clear();
     off.assign(1, 0);
@@ -149,6 +188,7 @@ void worker(alpacc_hip_server* s, uint32_t g, alpacc_hip_ctx* ctx) {
         r->done = now;
       }
       s->in_use -= est;
+      s->growing -= grow;
       s->n_batches++;
       s->n_reqs += batch.size();
       if (batch.size() > s->biggest) s->biggest = batch.size();

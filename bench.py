@@ -34,6 +34,17 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def host_cpu():
+    """The host's CPU model and logical CPU count (nproc)."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count()}
+
+
 def cpu_baseline(grammar: str, mode: str, seed: int, conforming: bool, sample_bytes: int, budget_s: float = 20.0):
     """The sequential oracle on `sample_bytes` of the same generator, 1 core."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -53,6 +64,7 @@ def cpu_baseline(grammar: str, mode: str, seed: int, conforming: bool, sample_by
         "kind": "port",
         "sample": f"{grammar} {mode} {'conforming' if conforming else 'literal'} seed {seed}, "
                   f"{sample_bytes} bytes, mean of {runs} run(s), {secs:.3f} s each",
+        **host_cpu(),
     }
 
 
@@ -264,6 +276,41 @@ def serve_workload(rates, seconds: float, seed: int = 4, plain: bool = True):
     return out, items
 
 
+def serve_capacity(rates, seconds: float, seed: int = 5, p99_limit_ms: float = 10.0):
+    """The continuous-batching server's capacity: the same trace shape at
+    rising open-loop rates (each replayed once untimed, then timed) until p99
+    latency reaches p99_limit_ms; the highest rate that stays below it is
+    the sustainable one."""
+    import numpy as np
+    from alpacc_amd import hip
+    names = ["json", "arithmetic"]
+    rows, best = [], None
+    for rate in rates:
+        count = max(1, int(rate * seconds))
+        gidx, payloads, arrival = serve_trace(count, rate, seed + int(rate))
+        total = sum(len(x) for x in payloads)
+        modes = ["both"] * count
+        srv = hip.Server(names, lanes=2, hbm_budget=64 << 30)
+        srv.replay(gidx, modes, payloads, arrival)
+        _, lat = srv.replay(gidx, modes, payloads, arrival)
+        st = srv.stats()
+        srv.close()
+        span_s = float(np.max(arrival + lat)) / 1e9
+        row = {"rate_req_s": rate, "requests": count, "bytes": total,
+               "p50_ms": round(float(np.percentile(lat, 50)) / 1e6, 3),
+               "p99_ms": round(float(np.percentile(lat, 99)) / 1e6, 3),
+               "GBps": round(total / span_s / 1e9, 3), "req_s": round(count / span_s, 1),
+               "max_batch": st["max_batch"]}
+        log(f"serve capacity {rate:.0f} req/s: {row}")
+        rows.append(row)
+        if row["p99_ms"] >= p99_limit_ms:
+            break
+        best = row
+    return {"p99_limit_ms": p99_limit_ms, "trace_seconds": seconds, "rates": rows,
+            "sustained_req_s": best["rate_req_s"] if best else None,
+            "sustained_GBps": best["GBps"] if best else None}
+
+
 def _oracle_worker(args):
     """Seconds and bytes of the sequential oracle over a list of (grammar, payload)."""
     mode, items = args
@@ -278,8 +325,9 @@ def _oracle_worker(args):
 
 def cpu_parallel_baseline(items, mode: str, label: str, cores: int = 0):
     """The oracle over `items` ((grammar, payload) pairs) in `cores` worker
-    processes on this host (default: min(16, cpu count)); aggregate GB/s over
-    the wall-clock of the parallel run."""
+    processes on this host; aggregate GB/s over the wall-clock of the
+    parallel run.  The default is min(16, nproc): a one-GPU box gives its
+    process 16 CPUs whatever nproc reports (the line records both)."""
     import multiprocessing as mp
     cores = cores or max(1, min(16, os.cpu_count() or 1))
     chunks = [(mode, items[i::cores]) for i in range(cores)]
@@ -290,7 +338,8 @@ def cpu_parallel_baseline(items, mode: str, label: str, cores: int = 0):
         wall = time.perf_counter() - t0
     nbytes = sum(b for _, b in res)
     return {"value": round(nbytes / wall / 1e9, 4), "unit": "GB/s", "cores": cores, "kind": "port",
-            "sample": f"{label}: {len(items)} inputs, {nbytes} bytes, {cores} processes, {wall:.2f} s wall"}
+            "sample": f"{label}: {len(items)} inputs, {nbytes} bytes, {cores} processes, {wall:.2f} s wall",
+            **host_cpu()}
 
 
 def main(argv=None):
@@ -309,6 +358,10 @@ def main(argv=None):
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--serve-rates", default="2000,10000,50000", help="request rates of the serving workload ('' = skip)")
     ap.add_argument("--serve-seconds", type=float, default=1.0)
+    ap.add_argument("--serve-capacity", default="100000,150000,200000,300000",
+                    help="rising request rates of the capacity sweep, until p99 >= 10 ms ('' = skip)")
+    ap.add_argument("--capacity-seconds", type=float, default=0.25)
+    ap.add_argument("--no-literal", action="store_true", help="skip the step on the literal (as-written) input")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_json_lexparse_1GiB.json"),
                     help="per-kernel HBM bytes per launch written by --pmc-summary")
     ap.add_argument("--copy-calibration", type=int, metavar="BYTES", default=0,
@@ -405,10 +458,31 @@ def main(argv=None):
         log(f"batched {batched['batched_GBps']} GB/s (graph {batched['graph_GBps']} GB/s, "
             f"{batched['graph_captures']} capture, {batched['graph_replays']} replays), "
             f"plain {batched['plain_GBps']} GB/s, x{batched['speedup_vs_plain']}")
+    literal = None
+    if not a.no_literal and conforming:
+        # the same step on the workload as written (escapes and exponents
+        # the grammar has no rule for): rejected, but every kernel still runs
+        # over the whole input
+        lit = datagen.generate("json", n, seed=1, conforming=False, wrap=True)
+        dev.copy_(torch.from_numpy(lit))
+        del lit
+        step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(a.steps):
+            lok, _ = step()
+        torch.cuda.synchronize()
+        lms = (time.perf_counter() - t0) * 1e3 / a.steps
+        literal = {"variant": "literal", "ms_per_step": round(lms, 3), "GBps": round(n / (lms / 1e3) / 1e9, 3),
+                   "valid": bool(lok)}
+        log(f"literal input: {literal}")
     served = None
     if a.serve_rates:
         rates = [float(x) for x in a.serve_rates.split(",") if x]
         served, serve_items = serve_workload(rates, a.serve_seconds)
+    capacity = None
+    if a.serve_capacity:
+        capacity = serve_capacity([float(x) for x in a.serve_capacity.split(",") if x], a.capacity_seconds)
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline("json", "both", 1, conforming, min(a.cpu_sample, n))
@@ -420,6 +494,12 @@ def main(argv=None):
             for row in served:
                 row["cpu_baseline"] = cpu_srv
     path_bytes = n + 19 * T + 9 * N  # input + tokens + tree (combined-mode outputs)
+    # HBM bytes one step moves by the counters (FETCH_SIZE + WRITE_SIZE of
+    # every kernel, from the same summary as roofline.traffic)
+    measured = None
+    if a.traffic and os.path.exists(a.traffic):
+        with open(a.traffic) as f:
+            measured = sum(k["bytes_per_launch"] for k in json.load(f).get("kernels", {}).values())
     out = {
         "metric": "lex+parse input GB/s (1 GiB, %HBM peak); batched GB/s; served p50/p99 ms",
         "value": round(n / (ms / 1e3) / 1e9, 3),
@@ -439,10 +519,15 @@ def main(argv=None):
                    "lexer_shape": {k: shape[k] for k in ("items_per_thread", "workgroups_per_cu", "lookback_window")}},
         "pct_hbm_peak_input": round(100 * n / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 2),
         "path_algorithmic_GBps": round(path_bytes / (ms / 1e3) / 1e9, 2),
+        "path_frac": round(path_bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        "path_measured_bytes": measured,
+        "path_measured_frac": round(measured / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4) if measured else None,
         "kernels_ms": {k: round(v, 4) for k, v in sorted(kernels.items(), key=lambda x: -x[1])},
         "roofline": roofline,
+        "literal": literal,
         "batched": batched,
         "served": served,
+        "serve_capacity": capacity,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

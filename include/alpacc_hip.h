@@ -298,10 +298,12 @@ int alpacc_hip_set_bracket_width(alpacc_hip_ctx *ctx, int bytes);
  * 167-189).  Every grammar gets `lanes` contexts (own stream, buffers and
  * worker thread); a free lane takes every request queued for its grammar in
  * one mode, up to `hbm_budget` bytes of estimated device memory admitted at
- * once (0 = 64 GiB), `max_batch` requests (0 = no limit) and 8 MiB of
- * payload (a larger request is a batch of its own), and answers them with
- * one batched launch sequence.  Records are byte-identical to
- * alpacc_hip_run_case's for each request alone. */
+ * once (0 = 64 GiB) and within the device's free memory (hipMemGetInfo,
+ * beyond what the lane's workspaces already hold, less a 1 GiB reserve and
+ * what other lanes' admitted batches may still allocate), `max_batch`
+ * requests (0 = no limit) and 8 MiB of payload (a larger request is a batch
+ * of its own), and answers them with one batched launch sequence.  Records
+ * are byte-identical to alpacc_hip_run_case's for each request alone. */
 typedef struct alpacc_hip_server alpacc_hip_server;
 
 int alpacc_hip_server_open(const char *const *grammars, int ngrammars, int device, int lanes, uint64_t hbm_budget,

@@ -597,6 +597,30 @@ def test_json_1gib_lexer_only(backends):
     assert len(got) == len(exp) and got == exp, "json 1 GiB lexer-only record"
 
 
+def test_json_1gib_literal_variant(backends):
+    """The JSON workload as written (string escapes, exponents: the grammar
+    rejects them) at 1 GiB, both modes: the verdicts equal the oracle's."""
+    n = 1 << 30
+    data = datagen.generate("json", n, seed=1, conforming=False, wrap=True).tobytes()
+    for mode in ("lexer", "both"):
+        print(f"case json/{mode} n={n} literal", flush=True)
+        got = backends["json"].run_case(mode, data)
+        exp = pyoracle.record("json", mode, data)
+        assert got == exp, f"json 1 GiB literal {mode}"
+
+
+def test_arithmetic_1mib_seed0(backends):
+    """The plain-method workload exactly: arithmetic, 1 MiB, seed 0, both
+    variants, every mode."""
+    for conforming in (True, False):
+        data = datagen.generate("arithmetic", 1 << 20, seed=0, conforming=conforming).tobytes()
+        for mode in ("lexer", "both"):
+            check(backends, "arithmetic", mode, data, f"seed 0 conf={conforming}")
+        toks = lexed_tokens("arithmetic", data)
+        if toks is not None:
+            check(backends, "arithmetic", "parser", toks, "seed 0 lexed tokens")
+
+
 def test_json_1gib_lex_parse(backends):
     """The measured workload at full size: 1 GiB of JSON (seed 1) lexed and
     parsed, tokens and tree identical to the oracle's (about 3.8 GB of
@@ -759,3 +783,24 @@ def test_server_submit_wait_any_order():
             assert got[t] == pyoracle.record(g, m, x), (g, m, x)
     finally:
         srv.close()
+
+
+def test_server_bench_trace_full_rate():
+    """The benchmark's own serving trace at its highest rate (50,000 req/s
+    for one second, sizes lognormal(ln 2048, 2) clipped to 16 MiB, 70% JSON /
+    30% arithmetic, 3% byte flips), replayed open-loop against the
+    continuous-batching server: every record equals the oracle's."""
+    import bench
+    from alpacc_amd import hip
+    gidx, payloads, arrival = bench.serve_trace(50000, 50000.0, 4 + 50000)
+    names = ["json", "arithmetic"]
+    modes = ["both"] * len(payloads)
+    print(f"case bench serving trace: {len(payloads)} requests, {sum(map(len, payloads))} bytes", flush=True)
+    srv = hip.Server(names, lanes=2, hbm_budget=64 << 30)
+    try:
+        recs, lat = srv.replay(gidx, modes, payloads, arrival)
+    finally:
+        srv.close()
+    exp = b"".join(pyoracle.record(names[g], "both", x) for g, x in zip(gidx, payloads))
+    assert first_diff(recs, exp) is None
+    print("p50 ms", float(np.percentile(lat, 50)) / 1e6, "p99 ms", float(np.percentile(lat, 99)) / 1e6, flush=True)
