@@ -213,3 +213,78 @@ int alpacc_hip_lex_parse(alpacc_hip_ctx* h, const uint8_t* str, uint64_t n, alpa
 }
 
 }  // extern "C"
+
+// ---- parser tile shape and its autotuner --------------------------------
+
+extern "C" {
+
+int alpacc_hip_set_parse_shape(alpacc_hip_ctx* h, const alpacc_hip_parse_shape* shape) {
+  if (!h || !shape) return fail(ALPACC_ERR_ARG, "null argument");
+  if (!parse_shape_ok(shape->positions_per_tile, shape->positions_per_lane))
+    return fail(ALPACC_ERR_ARG, "parser tile shape %d positions per tile, %d per lane is not one of the kernels'",
+                shape->positions_per_tile, shape->positions_per_lane);
+  Context& c = h->c;
+  HIP_TRY(hipSetDevice(c.device));
+  c.parse_pt = shape->positions_per_tile;
+  c.parse_apl = shape->positions_per_lane;
+  if (c.graph) {  // captured with the old kernels
+    (void)hipGraphExecDestroy(c.graph);
+    c.graph = nullptr;
+  }
+  return 0;
+}
+
+int alpacc_hip_get_parse_shape(const alpacc_hip_ctx* h, alpacc_hip_parse_shape* shape) {
+  if (!h || !shape) return fail(ALPACC_ERR_ARG, "null argument");
+  shape->positions_per_tile = h->c.parse_pt;
+  shape->positions_per_lane = h->c.parse_apl;
+  shape->ms = 0;
+  return 0;
+}
+
+int alpacc_hip_autotune_parser(alpacc_hip_ctx* h, const uint8_t* d_sample, uint64_t n, int reps,
+                               alpacc_hip_parse_shape* best) {
+  if (!h || !best || !d_sample || n == 0 || reps <= 0) return fail(ALPACC_ERR_ARG, "null argument or empty sample");
+  Context& c = h->c;
+  HIP_TRY(hipSetDevice(c.device));
+  if (!c.g->has_parser) return fail(ALPACC_ERR_ARG, "grammar '%s' has no production rules", c.g->name);
+  for (hipEvent_t& ev : c.ev)
+    if (!ev) HIP_TRY(hipEventCreate(&ev));
+  const bool timing = c.timing, ktiming = c.timer.on;
+  c.timing = true;  // the parser's stage events time each run
+  c.timer.on = false;
+  static const int shapes[][2] = {{1024, 4}, {512, 4}, {2048, 4}, {1024, 2}, {1024, 8}};
+  alpacc_hip_parse_shape top{c.parse_pt, c.parse_apl, 0};
+  double top_ms = -1;
+  int r = 0;
+  for (const auto& sh : shapes) {
+    const alpacc_hip_parse_shape want{sh[0], sh[1], 0};
+    if ((r = alpacc_hip_set_parse_shape(h, &want)) < 0) break;
+    alpacc_hip_tree t{};
+    if ((r = lex_parse_device(c, d_sample, n, c.stream, &t)) < 0) break;  // warm
+    c.parse_ms = 0;
+    c.parse_launches = 0;
+    for (int i = 0; i < reps && r >= 0; ++i) r = lex_parse_device(c, d_sample, n, c.stream, &t);
+    if (r < 0) break;
+    if (!c.parse_launches) {
+      r = fail(ALPACC_ERR_ARG, "the tuning sample is rejected by the lexer: the parser never ran");
+      break;
+    }
+    const double ms = c.parse_ms / (double)c.parse_launches;
+    if (top_ms < 0 || ms < top_ms) {
+      top_ms = ms;
+      top = want;
+    }
+  }
+  c.timing = timing;
+  c.timer.on = ktiming;
+  c.parse_ms = c.lex_ms = 0;
+  c.parse_launches = c.lex_launches = 0;
+  if (r < 0) return r;
+  top.ms = top_ms;
+  if ((r = alpacc_hip_set_parse_shape(h, &top)) < 0) return r;
+  *best = top;
+  return 0;
+}
+
+}  // extern "C"

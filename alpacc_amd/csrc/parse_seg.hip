@@ -51,7 +51,11 @@ namespace alpacc {
 
 namespace {
 
-constexpr int PT = 1024;         // framed positions per tile
+// Tile shapes: PT framed positions per tile, APL positions per lane in
+// pass B (so PT / APL lanes resolve a tile).  The default is (1024, 4);
+// the context picks one of TILE_SHAPES (alpacc_hip_set_parse_shape and the
+// parser autotuner).
+constexpr int PT_DEFAULT = 1024, APL_DEFAULT = 4;
 // Reject reasons.  parse_test (parser.c:49-98) rejects on pairs and brackets
 // only; the tree checks add the terminal count (compact_tree, parser.c:
 // 126-153) and the derivation-slot structure.
@@ -209,11 +213,11 @@ __device__ __forceinline__ uint32_t rec_owner(const uint4& r2, const uint4& r3, 
 
 // ---------------------------------------------------------------- pass A
 
-// Pass A: one wave per tile, PPL consecutive positions per lane, so every
-// lane has 16 independent window-index loads and then 16 independent
-// record-head loads in flight, and the tile's sums and depth profiles are
-// wave reductions (no workgroup barrier).
-constexpr int PPL = PT / WAVE;
+// Pass A: one wave per tile, PPL consecutive positions per lane (16, or 8
+// for tiles of 512), so every lane has PPL independent window-index loads
+// and then PPL independent record-head loads in flight; a tile of more than
+// 64 * PPL positions takes several such rounds.  The tile's sums and depth
+// profiles are wave reductions (no workgroup barrier).
 constexpr int AWG = 256;  // four tiles per workgroup
 constexpr int AWAVES = AWG / WAVE;
 
@@ -270,10 +274,13 @@ __device__ __forceinline__ void window_ids_exact(const Seg& s, const TileOf& t, 
   }
 }
 
-template <int W>
-__device__ __forceinline__ void profile_tile(const Seg& s, uint32_t tile, uint32_t lane, TileRec* rec, uint16_t* pid) {
-  const TileOf t = tile_of(s, tile);
-  const int32_t p0 = (int32_t)t.lt * PT + (int32_t)lane * PPL;
+// One round of pass A over PPL positions per lane from framed position p0:
+// writes the pair ids, adds the lane's slot / terminal-production counts to
+// np / nt, and returns the round's bracket and slot depth profiles over the
+// whole wave (in every lane).
+template <int W, int PPL>
+__device__ __forceinline__ void profile_round(const Seg& s, const TileOf& t, int32_t p0, uint16_t* pid_at, int32_t& np,
+                                              int32_t& nt, bool& bad, SM& wb, SM& ws) {
   uint32_t ids[PPL];
   if (W > 0) {
     window_ids_exact<(W > 0 ? W : 1), PPL>(s, t, p0, ids);
@@ -287,9 +294,7 @@ __device__ __forceinline__ void profile_tile(const Seg& s, uint32_t tile, uint32
 #pragma unroll
   for (int r = 0; r < PPL; ++r)  // positions past the input's end read the empty record
     h[r] = *reinterpret_cast<const uint2*>(s.rec + 4 * (ids[r] == NO_PAIR ? (uint32_t)s.npairs : ids[r]));
-  int32_t np = 0, nt = 0;
   SM b{0, 0}, sl{0, 0};
-  bool bad = false;
 #pragma unroll
   for (int r = 0; r < PPL; ++r) {
     bad |= p0 + r < t.m && ids[r] == NO_PAIR;
@@ -300,16 +305,36 @@ __device__ __forceinline__ void profile_tile(const Seg& s, uint32_t tile, uint32
     b = sm_cat(b, SM{bpush - bpop, -bpop});
     sl = sm_cat(sl, SM{spush - spop, -spop});
   }
-  uint4* dst = reinterpret_cast<uint4*>(pid + (uint64_t)tile * PT + (uint64_t)lane * PPL);
+  uint4* dst = reinterpret_cast<uint4*>(pid_at);
 #pragma unroll
   for (int v = 0; v < PPL / 8; ++v)
     dst[v] = uint4{ids[8 * v] | ids[8 * v + 1] << 16, ids[8 * v + 2] | ids[8 * v + 3] << 16,
                    ids[8 * v + 4] | ids[8 * v + 5] << 16, ids[8 * v + 6] | ids[8 * v + 7] << 16};
-  const uint32_t cp = wave_scan_add32((uint32_t)np), ct = wave_scan_add32((uint32_t)nt);
   const SM ib = wave_scan_sm(b), is = wave_scan_sm(sl);
+  wb = SM{__shfl(ib.s, WAVE - 1, WAVE), __shfl(ib.m, WAVE - 1, WAVE)};
+  ws = SM{__shfl(is.s, WAVE - 1, WAVE), __shfl(is.m, WAVE - 1, WAVE)};
+}
+
+template <int W, int PT>
+__device__ __forceinline__ void profile_tile(const Seg& s, uint32_t tile, uint32_t lane, TileRec* rec, uint16_t* pid) {
+  constexpr int PPL = PT / WAVE < 16 ? PT / WAVE : 16, ROUNDS = PT / (WAVE * PPL);
+  const TileOf t = tile_of(s, tile);
+  int32_t np = 0, nt = 0;
+  bool bad = false;
+  SM tb, ts;  // the tile's profiles, round after round
+  profile_round<W, PPL>(s, t, (int32_t)t.lt * PT + (int32_t)lane * PPL,
+                        pid + (uint64_t)tile * PT + (uint64_t)lane * PPL, np, nt, bad, tb, ts);
+  for (int round = 1; round < ROUNDS; ++round) {
+    SM rb, rs;
+    profile_round<W, PPL>(s, t, (int32_t)t.lt * PT + round * WAVE * PPL + (int32_t)lane * PPL,
+                          pid + (uint64_t)tile * PT + (uint64_t)(round * WAVE + lane) * PPL, np, nt, bad, rb, rs);
+    tb = sm_cat(tb, rb);
+    ts = sm_cat(ts, rs);
+  }
+  const uint32_t cp = wave_scan_add32((uint32_t)np), ct = wave_scan_add32((uint32_t)nt);
   const uint64_t anybad = __ballot(bad);
   if (lane == WAVE - 1) {
-    rec[tile] = TileRec{(int32_t)cp, (int32_t)ct, ib.s, ib.m, is.s, is.m, 0, 0};
+    rec[tile] = TileRec{(int32_t)cp, (int32_t)ct, tb.s, tb.m, ts.s, ts.m, 0, 0};
     if (anybad) atomicOr(s.flags + t.seg, BAD_PAIR);
   }
 }
@@ -327,29 +352,34 @@ __device__ __forceinline__ uint32_t tid_per_tile() {
 // (STRIDE, for a grid sized before the tile count is known) the waves
 // stride over the tiles.
 // W = the grammar's window width q + k (1..6), or 0 for any width.
-template <bool STRIDE, int W>
+template <bool STRIDE, int W, int PT>
 __global__ __launch_bounds__(AWG) void seg_profile(Seg s, TileRec* rec, uint16_t* pid) {
   const uint32_t nt = *s.nt_dev, lane = threadIdx.x & (WAVE - 1);
   const uint32_t w0 = blockIdx.x * AWAVES + threadIdx.x / WAVE;
   if (!STRIDE) {
-    if (w0 < nt) profile_tile<W>(s, w0, lane, rec, pid);
+    if (w0 < nt) profile_tile<W, PT>(s, w0, lane, rec, pid);
     return;
   }
-  for (uint32_t tile = w0; tile < nt; tile += gridDim.x * AWAVES) profile_tile<W>(s, tile, lane, rec, pid);
+  for (uint32_t tile = w0; tile < nt; tile += gridDim.x * AWAVES) profile_tile<W, PT>(s, tile, lane, rec, pid);
 }
 
 typedef void (*ProfileKernel)(Seg, TileRec*, uint16_t*);
-template <bool STRIDE>
+template <bool STRIDE, int PT>
 ProfileKernel profile_kernel(int w) {
   switch (w) {
-    case 1: return seg_profile<STRIDE, 1>;
-    case 2: return seg_profile<STRIDE, 2>;
-    case 3: return seg_profile<STRIDE, 3>;
-    case 4: return seg_profile<STRIDE, 4>;
-    case 5: return seg_profile<STRIDE, 5>;
-    case 6: return seg_profile<STRIDE, 6>;
-    default: return seg_profile<STRIDE, 0>;
+    case 1: return seg_profile<STRIDE, 1, PT>;
+    case 2: return seg_profile<STRIDE, 2, PT>;
+    case 3: return seg_profile<STRIDE, 3, PT>;
+    case 4: return seg_profile<STRIDE, 4, PT>;
+    case 5: return seg_profile<STRIDE, 5, PT>;
+    case 6: return seg_profile<STRIDE, 6, PT>;
+    default: return seg_profile<STRIDE, 0, PT>;
   }
+}
+template <bool STRIDE>
+ProfileKernel profile_kernel(int w, int pt) {
+  return pt == 512 ? profile_kernel<STRIDE, 512>(w) : (pt == 2048 ? profile_kernel<STRIDE, 2048>(w)
+                                                                  : profile_kernel<STRIDE, 1024>(w));
 }
 
 // ---------------------------------------------------------------- pass S
@@ -506,18 +536,15 @@ __global__ __launch_bounds__(256) void tile_downsweep(Seg s, const TileRec* rec,
 
 // ---------------------------------------------------------------- pass B
 
-constexpr int APL = 4;            // positions per lane in pass B
-constexpr int RWG = PT / APL;     // lanes per tile
-constexpr int RNW = RWG / WAVE;   // waves per tile
 constexpr int WLV = 6;            // in-wave sparse-table levels: blocks of 1..32 lanes
 constexpr uint8_t TERM_SLOT = 0x80u;
 
 struct Resolve {
   Seg s;
-  const uint16_t* pid;  // [ntiles * PT] pair ids from pass A
+  const uint16_t* pid;  // [ntiles * positions per tile] pair ids from pass A
   const TileInfo* info;
   const SegBase* sb;
-  int bcap, scap;      // per-lane stack room: APL * (pops + pushes) of a window, brackets and slots
+  int bcap, scap;      // per-lane stack room: positions per lane * (pops + pushes) of a window, brackets and slots
   int ncap, tcap;      // staged nodes / terminal productions per tile
   uint16_t* ob_list;   // open pushes (bracket symbols)
   uint16_t* ub_list;   // unmatched pops (bracket symbols)
@@ -535,6 +562,7 @@ struct Resolve {
 // l's wave, pw[l] the minimum over [w, l] and wmin[v] the minimum of wave v:
 // the search descends inside from's wave when it holds such a lane, else
 // inside the nearest earlier wave that does, from its last lane.
+template <int RWG>
 __device__ __forceinline__ int prev_below(const int16_t (*tb)[RWG], const int16_t* pw, const int32_t* wmin, int from,
                                           int32_t L) {
   auto descend = [&](int l) {  // a lane below L lies in [first lane of l's wave, l]
@@ -552,6 +580,7 @@ __device__ __forceinline__ int prev_below(const int16_t (*tb)[RWG], const int16_
 }
 
 // Block-wide exclusive scans of four int32 values.
+template <int RNW>
 __device__ __forceinline__ void block_scan4(int32_t* v, int32_t* tot, int32_t (*ws)[RNW]) {
   const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   int32_t inc[4];
@@ -586,8 +615,10 @@ __device__ __forceinline__ void block_scan4(int32_t* v, int32_t* tot, int32_t (*
 // consecutive levels just above its minimum, so a pop left over by a later
 // lane at tile-relative level D reads entry D - min - 1 of the last earlier
 // lane whose minimum is below D (a sparse table over the lanes' minima).
-template <int MBPOP, int MBPUSH, int MNP, int MSPUSH, typename BT>
+template <int MBPOP, int MBPUSH, int MNP, int MSPUSH, typename BT, int PT, int APL>
 __device__ __forceinline__ void resolve_tile(const Resolve& r, uint32_t tile, uint32_t tid) {
+  constexpr int RWG = PT / APL;    // lanes per tile
+  constexpr int RNW = RWG / WAVE;  // waves per tile
   // dynamic LDS: bracket stacks BT [bcap][RWG], slot stacks u8 [scap][RWG],
   // staged parents of nodes u16 [ncap] and of terminal productions u16
   // [tcap] (tile-local compact indices), staged productions u8 [ncap]
@@ -612,8 +643,24 @@ __device__ __forceinline__ void resolve_tile(const Resolve& r, uint32_t tile, ui
   const SegBase in = r.sb[seg];
 
   // phase 0: record heads of the lane's positions, lane sums and offsets
-  const uint2 pv = *reinterpret_cast<const uint2*>(r.pid + (uint64_t)tile * PT + (uint64_t)tid * APL);
-  const uint32_t ids[APL] = {pv.x & 0xFFFFu, pv.x >> 16, pv.y & 0xFFFFu, pv.y >> 16};
+  uint32_t ids[APL];
+  {
+    const uint16_t* pp = r.pid + (uint64_t)tile * PT + (uint64_t)tid * APL;
+    if (APL == 2) {
+      const uint32_t pv = *reinterpret_cast<const uint32_t*>(pp);
+      ids[0] = pv & 0xFFFFu;
+      ids[1 % APL] = pv >> 16;
+    } else {
+#pragma unroll
+      for (int v = 0; v < APL / 4; ++v) {
+        const uint2 pv = reinterpret_cast<const uint2*>(pp)[v];
+        ids[(4 * v) % APL] = pv.x & 0xFFFFu;
+        ids[(4 * v + 1) % APL] = pv.x >> 16;
+        ids[(4 * v + 2) % APL] = pv.y & 0xFFFFu;
+        ids[(4 * v + 3) % APL] = pv.y >> 16;
+      }
+    }
+  }
   const uint4* rp[APL];
   uint4 h[APL];
   int32_t np = 0, nt = 0, bd = 0, sd = 0;
@@ -847,27 +894,35 @@ __device__ __forceinline__ void resolve_tile(const Resolve& r, uint32_t tile, ui
 }
 
 // One tile per workgroup, as seg_profile.
-template <int MBPOP, int MBPUSH, int MNP, int MSPUSH, bool STRIDE, typename BT>
-__global__ __launch_bounds__(RWG) void seg_resolve(Resolve r) {
+template <int MBPOP, int MBPUSH, int MNP, int MSPUSH, bool STRIDE, typename BT, int PT, int APL>
+__global__ __launch_bounds__(PT / APL) void seg_resolve(Resolve r) {
   const uint32_t nt = *r.s.nt_dev;
   if (!STRIDE) {
-    if (blockIdx.x < nt) resolve_tile<MBPOP, MBPUSH, MNP, MSPUSH, BT>(r, blockIdx.x, threadIdx.x);
+    if (blockIdx.x < nt) resolve_tile<MBPOP, MBPUSH, MNP, MSPUSH, BT, PT, APL>(r, blockIdx.x, threadIdx.x);
     return;
   }
   for (uint32_t tile = blockIdx.x; tile < nt; tile += gridDim.x) {
-    resolve_tile<MBPOP, MBPUSH, MNP, MSPUSH, BT>(r, tile, tid_per_tile());
+    resolve_tile<MBPOP, MBPUSH, MNP, MSPUSH, BT, PT, APL>(r, tile, tid_per_tile());
     __syncthreads();
   }
 }
 
-// The single-tile and striding kernels of one instantiation, bracket
-// stacks u8 (br8) or u16.
+// The single-tile and striding kernels of one instantiation and tile shape,
+// bracket stacks u8 (br8) or u16.
+template <int MBPOP, int MBPUSH, int MNP, int MSPUSH, int PT, int APL>
+void resolve_kernels_shape(bool br8, void (**kern)(Resolve)) {
+  kern[0] = br8 ? seg_resolve<MBPOP, MBPUSH, MNP, MSPUSH, false, uint8_t, PT, APL>
+                : seg_resolve<MBPOP, MBPUSH, MNP, MSPUSH, false, uint16_t, PT, APL>;
+  kern[1] = br8 ? seg_resolve<MBPOP, MBPUSH, MNP, MSPUSH, true, uint8_t, PT, APL>
+                : seg_resolve<MBPOP, MBPUSH, MNP, MSPUSH, true, uint16_t, PT, APL>;
+}
 template <int MBPOP, int MBPUSH, int MNP, int MSPUSH>
-void resolve_kernels(bool br8, void (**kern)(Resolve)) {
-  kern[0] = br8 ? seg_resolve<MBPOP, MBPUSH, MNP, MSPUSH, false, uint8_t>
-                : seg_resolve<MBPOP, MBPUSH, MNP, MSPUSH, false, uint16_t>;
-  kern[1] = br8 ? seg_resolve<MBPOP, MBPUSH, MNP, MSPUSH, true, uint8_t>
-                : seg_resolve<MBPOP, MBPUSH, MNP, MSPUSH, true, uint16_t>;
+void resolve_kernels(bool br8, int pt, int apl, void (**kern)(Resolve)) {
+  if (pt == 512) resolve_kernels_shape<MBPOP, MBPUSH, MNP, MSPUSH, 512, 4>(br8, kern);
+  else if (pt == 2048) resolve_kernels_shape<MBPOP, MBPUSH, MNP, MSPUSH, 2048, 4>(br8, kern);
+  else if (apl == 2) resolve_kernels_shape<MBPOP, MBPUSH, MNP, MSPUSH, 1024, 2>(br8, kern);
+  else if (apl == 8) resolve_kernels_shape<MBPOP, MBPUSH, MNP, MSPUSH, 1024, 8>(br8, kern);
+  else resolve_kernels_shape<MBPOP, MBPUSH, MNP, MSPUSH, 1024, 4>(br8, kern);
 }
 
 // ---------------------------------------------------------------- pass C
@@ -1072,8 +1127,8 @@ __global__ void pseg_map(const uint32_t* tile_first, uint32_t count, const uint3
 // per round.
 constexpr int PLAN_WG = 1024, PLAN_IPT = 4;
 __global__ __launch_bounds__(PLAN_WG) void parse_plan(const uint64_t* tok_off, const uint32_t* skip,
-                                                      const uint32_t* live, uint32_t count, uint32_t* tile_first,
-                                                      uint32_t* nt_dev) {
+                                                      const uint32_t* live, uint32_t count, uint32_t PT,
+                                                      uint32_t* tile_first, uint32_t* nt_dev) {
   __shared__ uint32_t s_w[PLAN_WG / WAVE];
   const uint32_t tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE, n_live = *live;
   uint32_t carry = 0;
@@ -1116,6 +1171,10 @@ int parse_per_position(const GrammarDesc* g) {
   return std::max(std::max(1, std::max(g->max_bpush, g->max_spush)), std::max(g->max_bpop, g->max_spop));
 }
 
+bool parse_shape_ok(int pt, int apl) {
+  return (pt == 1024 && (apl == 2 || apl == 4 || apl == 8)) || ((pt == 512 || pt == 2048) && apl == 4);
+}
+
 uint64_t parse_max_positions(const GrammarDesc* g) {
   return (0x7FFFFFFFull - 1) / (uint64_t)(parse_per_position(g) + 1);
 }
@@ -1136,6 +1195,9 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
   if (count == 0 || count > 0xFFFFFFFEull) return fail(ALPACC_ERR_ARG, "batch of %llu inputs", (unsigned long long)count);
   if (caps && (count != caps->count || !d_skip || !d_live))
     return fail(ALPACC_ERR_ARG, "batch beyond the capacity its launch sequence is sized for");
+  // the context's tile shape: positions per tile and per lane of pass B
+  const int PT = c.parse_pt, APL = c.parse_apl, RWG = PT / APL;
+  if (!parse_shape_ok(PT, APL)) return fail(ALPACC_ERR_ARG, "parser tile shape %d / %d", PT, APL);
   const int ncap = PT * g->max_prods, tcap = PT * g->max_nt;
   const int bcap = APL * (g->max_bpop + g->max_bpush), scap = APL * (g->max_spop + g->max_spush);
   if (c.resolve_br8 < 0) c.resolve_br8 = bracket_symbol_max(g) < 256 ? 1 : 0;
@@ -1241,7 +1303,8 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
   ParseResult* res = reinterpret_cast<ParseResult*>(base + off_res);
 
   if (caps) {
-    hipLaunchKernelGGL(parse_plan, dim3(1), dim3(PLAN_WG), 0, st, d_tok_off, d_skip, d_live, s.count, d_tf, d_tf + count + 1);
+    hipLaunchKernelGGL(parse_plan, dim3(1), dim3(PLAN_WG), 0, st, d_tok_off, d_skip, d_live, s.count, (uint32_t)PT, d_tf,
+                       d_tf + count + 1);
   } else {
     HIP_TRY(hipMemcpyAsync(d_tf, h_tf, (count + 2) * 4, hipMemcpyHostToDevice, st));
   }
@@ -1261,8 +1324,8 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
     hipLaunchKernelGGL(pseg_map, dim3(g_map), dim3(256), 0, st, d_tf, s.count, s.nt_dev,
                        const_cast<uint32_t*>(s.tile_seg));
     if (!caps) HIP_TRY(tm.begin(st, "seg_profile"));
-    hipLaunchKernelGGL(caps ? profile_kernel<true>(g->q + g->k) : profile_kernel<false>(g->q + g->k), dim3(g_prof),
-                       dim3(AWG), 0, st, s, rec, reinterpret_cast<uint16_t*>(base + off_pid));
+    hipLaunchKernelGGL(caps ? profile_kernel<true>(g->q + g->k, PT) : profile_kernel<false>(g->q + g->k, PT),
+                       dim3(g_prof), dim3(AWG), 0, st, s, rec, reinterpret_cast<uint16_t*>(base + off_pid));
     if (!caps) HIP_TRY(tm.end(st));
     if (!caps) HIP_TRY(tm.begin(st, "tile_scan"));
     hipLaunchKernelGGL(tile_block_sums, dim3(nsb), dim3(256), 0, st, s, rec, bsum, sout);
@@ -1319,11 +1382,11 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
                tree ? 1 : 0};
     void (*kern[2])(Resolve);  // one tile per workgroup, striding
     if (g->max_bpop <= 2 && g->max_bpush <= 3 && g->max_prods <= 4 && g->max_spush <= 3)
-      resolve_kernels<2, 3, 4, 3>(br8, kern);
+      resolve_kernels<2, 3, 4, 3>(br8, PT, APL, kern);
     else if (g->max_bpop <= 3 && g->max_bpush <= 4 && g->max_prods <= 4 && g->max_spush <= 4)
-      resolve_kernels<3, 4, 4, 4>(br8, kern);
+      resolve_kernels<3, 4, 4, 4>(br8, PT, APL, kern);
     else
-      resolve_kernels<4, 4, 8, 24>(br8, kern);
+      resolve_kernels<4, 4, 8, 24>(br8, PT, APL, kern);
     for (auto k : kern) HIP_TRY(grant_lds(reinterpret_cast<const void*>(k), lds_b));
     if (!caps) HIP_TRY(tm.begin(st, "seg_resolve"));
     hipLaunchKernelGGL(kern[caps ? 1 : 0], dim3(g_res), dim3(RWG), lds_r, st, rs);

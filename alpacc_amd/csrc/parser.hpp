@@ -44,6 +44,10 @@ int parse_per_position(const GrammarDesc* g);
 // up to this many): positions * per_position + inputs < 2^31.
 uint64_t parse_max_positions(const GrammarDesc* g);
 
+// The parser's tile shapes: positions per tile (512, 1024, 2048) and per
+// lane of the resolve pass (4; or 2, 4, 8 at 1024 positions per tile).
+bool parse_shape_ok(int positions_per_tile, int positions_per_lane);
+
 // Largest bracket symbol of the grammar's pair records.
 uint32_t bracket_symbol_max(const GrammarDesc* g);
 

@@ -82,6 +82,7 @@ struct Context {
   // not computed yet)
   int lex_ipt = LEX_IPT, lex_lb = LEX_LB_WIN, lex_wpc = 0;
   int lex_grid = 0, lex_grid2 = 0;
+  int parse_pt = 1024, parse_apl = 4;  // parser tile shape: positions per tile and per resolve lane
   int resolve_br8 = -1;              // 1: every bracket symbol below 256 (-1: not checked yet)
   int resolve_bw = 0;                // bracket stack bytes an entry (0: 1 when resolve_br8, else 2)
   // resident tables

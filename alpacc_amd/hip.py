@@ -49,6 +49,13 @@ class LexShape(C.Structure):
                 ("ms", C.c_double)]
 
 
+class ParseShape(C.Structure):
+    _fields_ = [("positions_per_tile", C.c_int), ("positions_per_lane", C.c_int), ("ms", C.c_double)]
+
+
+PARSE_SHAPES = [(1024, 4), (512, 4), (2048, 4), (1024, 2), (1024, 8)]
+
+
 class HipError(RuntimeError):
     pass
 
@@ -117,6 +124,9 @@ def load_library(path: str = LIB_PATH):
         "alpacc_hip_get_lex_shape": (i32, [vp, C.POINTER(LexShape)]),
         "alpacc_hip_autotune_lexer": (i32, [vp, i32, vp, u64, i32, C.POINTER(LexShape)]),
         "alpacc_hip_load_grammar": (i32, [C.c_char_p, C.c_char_p, C.c_size_t]),
+        "alpacc_hip_set_parse_shape": (i32, [vp, C.POINTER(ParseShape)]),
+        "alpacc_hip_get_parse_shape": (i32, [vp, C.POINTER(ParseShape)]),
+        "alpacc_hip_autotune_parser": (i32, [vp, vp, u64, i32, C.POINTER(ParseShape)]),
         "alpacc_hip_grammar_limits": (i32, [C.c_char_p, C.POINTER(u64), C.POINTER(u64)]),
         "alpacc_hip_name_count": (i32, [C.c_char_p, i32]),
         "alpacc_hip_name": (i32, [C.c_char_p, i32, u64, C.c_char_p, C.c_size_t]),
@@ -388,6 +398,24 @@ class Backend:
                                                   C.byref(sh)) >= 0 and 1 or -1)
         return {"items_per_thread": sh.items_per_thread, "workgroups_per_cu": sh.workgroups_per_cu,
                 "lookback_window": sh.lookback_window, "ms": round(sh.ms, 4)}
+
+    # -- parser tile shape --------------------------------------------------------
+    def set_parse_shape(self, positions_per_tile: int = 1024, positions_per_lane: int = 4):
+        sh = ParseShape(positions_per_tile, positions_per_lane, 0.0)
+        _check(min(self.lib.alpacc_hip_set_parse_shape(self.h, C.byref(sh)), 0))
+
+    def parse_shape(self) -> dict:
+        sh = ParseShape()
+        _check(min(self.lib.alpacc_hip_get_parse_shape(self.h, C.byref(sh)), 0))
+        return {"positions_per_tile": sh.positions_per_tile, "positions_per_lane": sh.positions_per_lane}
+
+    def autotune_parser(self, d_ptr: int, n: int, reps: int = 3) -> dict:
+        """Time the parser in every tile shape on a device-resident sample
+        (lexed in combined mode), keep the fastest; returns it with its time."""
+        sh = ParseShape()
+        _check(min(self.lib.alpacc_hip_autotune_parser(self.h, C.c_void_p(d_ptr), n, reps, C.byref(sh)), 0))
+        return {"positions_per_tile": sh.positions_per_tile, "positions_per_lane": sh.positions_per_lane,
+                "ms": round(sh.ms, 4)}
 
     # -- graphs for combined-mode batches -----------------------------------------
     def set_graphs(self, on: bool = True):

@@ -354,6 +354,8 @@ def main(argv=None):
     ap.add_argument("--no-autotune", action="store_true", help="keep the default lexer shape instead of tuning it")
     ap.add_argument("--lexer-shape", metavar="IPT,WPC,WIN", default="",
                     help="use this lexer shape (bytes per lane, workgroups per CU, lookback window) instead of tuning")
+    ap.add_argument("--parser-shape", metavar="PT,APL", default="",
+                    help="use this parser tile shape (positions per tile, per lane) instead of tuning")
     ap.add_argument("--batch-count", type=int, default=20000, help="inputs of the batching workload (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--serve-rates", default="2000,10000,50000", help="request rates of the serving workload ('' = skip)")
@@ -400,6 +402,11 @@ def main(argv=None):
         be.set_lex_shape(*[int(x) for x in a.lexer_shape.split(",")])
     shape = be.lex_shape() if a.no_autotune or a.lexer_shape else be.autotune_lexer(dev.data_ptr(), n, "both", reps=3)
     log(f"lexer shape: {shape}")
+    # the parser's tile shape, tuned on the same input
+    if a.parser_shape:
+        be.set_parse_shape(*[int(x) for x in a.parser_shape.split(",")])
+    pshape = be.parse_shape() if a.no_autotune or a.parser_shape else be.autotune_parser(dev.data_ptr(), n, reps=3)
+    log(f"parser shape: {pshape}")
 
     def step():
         ok, tree = be.lex_parse_device(dev.data_ptr(), n, stream=stream)
@@ -516,7 +523,8 @@ def main(argv=None):
         "config": {"workload": f"json lex+parse, one {n}-byte input (combined mode, device-resident)",
                    "grammar": "grammars/json.alp", "bytes": n, "tokens": T, "tree_nodes": N,
                    "valid": bool(ok), "variant": a.variant,
-                   "lexer_shape": {k: shape[k] for k in ("items_per_thread", "workgroups_per_cu", "lookback_window")}},
+                   "lexer_shape": {k: shape[k] for k in ("items_per_thread", "workgroups_per_cu", "lookback_window")},
+                   "parser_shape": {k: pshape[k] for k in ("positions_per_tile", "positions_per_lane")}},
         "pct_hbm_peak_input": round(100 * n / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 2),
         "path_algorithmic_GBps": round(path_bytes / (ms / 1e3) / 1e9, 2),
         "path_frac": round(path_bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),

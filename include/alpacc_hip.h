@@ -184,6 +184,26 @@ int alpacc_hip_get_lex_shape(const alpacc_hip_ctx *ctx, alpacc_hip_lex_shape *sh
 int alpacc_hip_autotune_lexer(alpacc_hip_ctx *ctx, int mode, const uint8_t *d_sample, uint64_t n, int reps,
                               alpacc_hip_lex_shape *best);
 
+/* Tile shape of the parser (the reference's parser BLOCK_SIZE /
+ * ITEMS_PER_THREAD, chosen per architecture when its CUDA program is
+ * generated, backends/cuda/parser.cu:45-158, backends/cuda/cli.cu:50-82):
+ * framed positions per tile, and positions per lane of the resolve pass,
+ * so positions_per_tile / positions_per_lane lanes make its workgroup.
+ * Shapes: (1024, 4) by default, (512, 4), (2048, 4), (1024, 2), (1024, 8);
+ * every shape gives the same results.  autotune_parser lexes a device
+ * sample (combined mode) and times the parser in every shape on `reps`
+ * runs, keeps the fastest in the context and returns it (ms = the parser's
+ * time on the sample); the stage timing totals are reset. */
+typedef struct {
+  int positions_per_tile, positions_per_lane;
+  double ms;
+} alpacc_hip_parse_shape;
+
+int alpacc_hip_set_parse_shape(alpacc_hip_ctx *ctx, const alpacc_hip_parse_shape *shape);
+int alpacc_hip_get_parse_shape(const alpacc_hip_ctx *ctx, alpacc_hip_parse_shape *shape);
+int alpacc_hip_autotune_parser(alpacc_hip_ctx *ctx, const uint8_t *d_sample, uint64_t n, int reps,
+                               alpacc_hip_parse_shape *best);
+
 /* ---- device-memory entry points ---------------------------------------- */
 /* Input already in device memory; results stay in device buffers owned by the
  * context, valid until the next call on it.  Only the count and verdict cross

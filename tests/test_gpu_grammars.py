@@ -89,7 +89,7 @@ def batches(g, mode):
     out = []
     alpha = testvec.alphabet(g) if mode != "parser" else \
         sorted(set(g.lex_both.term_id.values()) - {g.lex_both.ignore_id})
-    k = 4 if len(alpha) <= 6 else 3
+    k = 4 if len(alpha) <= 6 else (3 if len(alpha) <= 24 else 2)
     out.append(("exhaustive", testvec.generate(g, mode, exhaustive=k)))
     if mode != "parser":
         out.append(("walks", testvec.generate(g, mode, long=3000, count=12, seed=1)))

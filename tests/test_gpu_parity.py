@@ -804,3 +804,70 @@ def test_server_bench_trace_full_rate():
     exp = b"".join(pyoracle.record(names[g], "both", x) for g, x in zip(gidx, payloads))
     assert first_diff(recs, exp) is None
     print("p50 ms", float(np.percentile(lat, 50)) / 1e6, "p99 ms", float(np.percentile(lat, 99)) / 1e6, flush=True)
+
+
+PARSE_SHAPES = [(1024, 4), (512, 4), (2048, 4), (1024, 2), (1024, 8)]
+
+
+@pytest.mark.parametrize("shape", PARSE_SHAPES, ids=lambda s: f"pt{s[0]}-apl{s[1]}")
+def test_parser_shapes(shape):
+    """Every parser tile shape the tuner can pick gives the oracle's records:
+    generated text around tile edges in all three grammars, deep nesting,
+    parser-mode token strings with bad ids, a batch of many small inputs
+    eagerly and through the graph, and the widest grammar windows."""
+    from alpacc_amd import hip
+    import test_gpu_grammars as TG
+    pt, apl = shape
+    bes = {}
+    try:
+        for g in ("json", "arithmetic", "sexp"):
+            bes[g] = hip.Backend(g)
+            bes[g].set_parse_shape(pt, apl)
+            assert bes[g].parse_shape() == {"positions_per_tile": pt, "positions_per_lane": apl}
+        for g in ("json", "arithmetic", "sexp"):
+            for n in (1, 100, 4000, 8191, 8193, 16385, 100003, 1 << 20):
+                data = datagen.generate(g, n, seed=n + pt, conforming=n % 3 != 0).tobytes()
+                check(bes, g, "both", data, f"parser shape {shape}")
+                toks = lexed_tokens(g, data)
+                if toks is not None:
+                    check(bes, g, "parser", toks, f"parser shape {shape}")
+        check(bes, "sexp", "both", b"(" * 10000 + b"a" + b")" * 10000, f"deep, parser shape {shape}")
+        check(bes, "json", "parser", bytes([4, 1, 2, 1, 14, 5, 0, 13]), f"bad ids, parser shape {shape}")
+        rng = random.Random(zlib.crc32(f"parser shape {shape}".encode()))
+        payloads = [datagen.generate("json", rng.randrange(0, 3000), seed=i, conforming=i % 5 != 0).tobytes()
+                    for i in range(500)]
+        exp = b"".join(pyoracle.record("json", "both", x) for x in payloads)
+        assert first_diff(bes["json"].run_batch("both", payloads), exp) is None, f"batch, parser shape {shape}"
+        bes["json"].set_graphs(True)
+        for rep in range(2):
+            assert first_diff(bes["json"].run_batch("both", payloads), exp) is None, f"graph {rep}, shape {shape}"
+        bes["json"].set_graphs(False)
+    finally:
+        for b in bes.values():
+            b.close()
+    for name in ("wide_windows", "many_symbols"):
+        TG.load_tables(name)
+        g = TG.grammar_of(name)
+        items = TG.testvec.generate(g, "both", derivations=200, seed=5)
+        exp = b"".join(pyoracle.record(name, "both", x) for x in items)
+        with hip.Backend(name) as be:
+            be.set_parse_shape(pt, apl)
+            assert first_diff(be.run_batch("both", items), exp) is None, (name, shape)
+
+
+def test_autotune_parser():
+    """The parser tuner picks one of the shapes on a device-resident sample,
+    keeps it, and records stay the oracle's."""
+    import torch
+    from alpacc_amd import hip
+    data = datagen.generate("json", 8 << 20, seed=9, conforming=True).tobytes()
+    dev = torch.frombuffer(bytearray(data), dtype=torch.uint8).to("cuda")
+    torch.cuda.synchronize()
+    with hip.Backend("json") as be:
+        best = be.autotune_parser(dev.data_ptr(), len(data), reps=2)
+        assert (best["positions_per_tile"], best["positions_per_lane"]) in PARSE_SHAPES and best["ms"] > 0
+        assert be.parse_shape() == {k: best[k] for k in ("positions_per_tile", "positions_per_lane")}
+        small = datagen.generate("json", 200000, seed=10, conforming=True).tobytes()
+        assert be.run_case("both", small) == pyoracle.record("json", "both", small)
+        with pytest.raises(hip.HipError):
+            be.set_parse_shape(4096, 4)
