@@ -441,9 +441,13 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
     s_rows[i] = b == 256 ? (uint16_t)p.token[s] : RC::encode(b == 257 ? s : p.delta[s * 256 + b], lds0);
   }
   const int ignore = p.ignore;
-  // s_lterm[r][lane] is the byte at LDS address lterm0 + r * WG + lane; a
-  // lane's writes past rcap tokens go to its own byte of the sink rows
-  const uint32_t lterm0 = lds0 + L.lterm, sink = lterm0 + rcap * WG + tid;
+  // s_lterm[r][lane] is the byte at LDS address lterm0 + r * WG + swz, swz
+  // = 4 * (lane % 64) + lane / 64: the 64 lanes of a wave write 64
+  // different banks whatever their r (a plain lane-order byte would put
+  // four lanes on each bank).  A lane's writes past rcap tokens go to its
+  // own byte of the sink rows.
+  const uint32_t swz = 4u * (tid & (WAVE - 1)) + tid / WAVE;
+  const uint32_t lterm0 = lds0 + L.lterm, sink = lterm0 + rcap * WG + swz;
   const uint32_t nt = *p.nt_dev;
 
   for (;;) {
@@ -468,7 +472,7 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
     // to the sink once per 4 bytes (so at most 3 rows past it); in the lane
     // holding an input's last bytes, the positions past them take the row's
     // stay transition.
-    uint32_t row = row0, waddr = lterm0 + tid, bm[IPT / 32] = {}, km[IPT / 32] = {};
+    uint32_t row = row0, waddr = lterm0 + swz, bm[IPT / 32] = {}, km[IPT / 32] = {};
     // off2 = 2 * byte: the transition's offset in the row
     auto step1 = [&](uint32_t off2, uint32_t& mb, uint32_t& mk, uint32_t bit) {
       const uint32_t x = lds_load16(row + off2);
@@ -640,7 +644,7 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
               const uint32_t below = pm & ((1u << jb) - 1u);
               const uint32_t i = d + k;
               if (i < STAGE) {
-                s_term[i] = lds[L.lterm + k * WG + tid];
+                s_term[i] = lds[L.lterm + k * WG + swz];
                 s_start[i] = below ? qb + 31 - __builtin_clz(below) : start;
                 s_end[i] = (uint16_t)(lane_off + 32 * q + jb);
               }
