@@ -95,6 +95,7 @@ int lex_parse_batch_graph(Context& c, const uint8_t* d_in, const uint64_t* off, 
       if (e == hipSuccess) e = hipGraphLaunch(c.graph, st);
       if (e == hipSuccess) e = copy(gs.lex_down);
       if (e == hipSuccess) e = copy(gs.parse_down);
+      if (e == hipSuccess) e = copy(gs.scan_down);
       if (e == hipSuccess) e = hipStreamSynchronize(st);
       if (e != hipSuccess) r = fail(ALPACC_ERR_HIP, "graph replay: %s", hipGetErrorString(e));
       c.graph_replays++;
@@ -107,17 +108,47 @@ int lex_parse_batch_graph(Context& c, const uint8_t* d_in, const uint64_t* off, 
       c.d_lb_live = gs.d_lb_live;
       c.pb_count = gs.pb_count;
       c.pb_res = gs.pb_res;
+      if (r >= 0 && parse_scan_overflow(c, caps.count)) {
+        // a tile beyond the staging learned from earlier batches: this
+        // batch runs again eagerly, and the next one captures again with
+        // staging for the grammar's maxima
+        (void)hipGraphExecDestroy(c.graph);
+        c.graph = nullptr;
+        c.cap_stage_n = c.cap_stage_t = 0;
+        c.timer.on = timing;
+        r = lex_batch_on_device(c, ALPACC_MODE_BOTH, d_in, off, count, st);
+        if (r >= 0) r = parse_batch_on_device(c, c.lx_term.as<uint8_t>(), c.d_lb_tok, c.lb_tok, c.lb_flags, count, true, st);
+        return r < 0 ? r : 0;
+      }
     }
   } else {
     if (c.graph) {
       (void)hipGraphExecDestroy(c.graph);
       c.graph = nullptr;
     }
-    // eager run: sizes every buffer and answers this batch
-    r = lex_parse_enqueue_caps(c, d_in, padded.data(), caps, st);
+    // eager run: sizes every buffer and answers this batch (with staging
+    // for the grammar's maxima when the learned one falls short)
+    auto eager = [&]() {
+      int rr = lex_parse_enqueue_caps(c, d_in, padded.data(), caps, st);
+      if (rr >= 0) {
+        hipError_t e = copy(c.cp_scan_down);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) rr = fail(ALPACC_ERR_HIP, "batch: %s", hipGetErrorString(e));
+      }
+      return rr;
+    };
+    r = eager();
+    if (r >= 0 && parse_scan_overflow(c, caps.count)) {
+      c.cap_stage_n = c.cap_stage_t = 0;
+      r = eager();
+    }
     if (r >= 0) {
-      hipError_t e = hipStreamSynchronize(st);
-      if (e != hipSuccess) r = fail(ALPACC_ERR_HIP, "batch: %s", hipGetErrorString(e));
+      // the captured sequence stages a quarter more than this batch's
+      // largest tile needs (a later batch beyond it runs again eagerly)
+      int n = 0, t = 0;
+      parse_scan_maxima(c, caps.count, true, &n, &t);
+      c.cap_stage_n = (n + n / 4 + 64) & ~63;
+      c.cap_stage_t = (t + t / 4 + 64) & ~63;
     }
     if (r >= 0) {
       const uint64_t gen = c.graph_buffers_gen();
@@ -142,6 +173,7 @@ int lex_parse_batch_graph(Context& c, const uint8_t* d_in, const uint64_t* off, 
         gs.up = c.cp_meta_up;
         gs.lex_down = c.cp_lex_down;
         gs.parse_down = c.cp_parse_down;
+        gs.scan_down = c.cp_scan_down;
         gs.lb_count = c.lb_count;
         gs.lb_flags = c.lb_flags;
         gs.lb_tok = c.lb_tok;

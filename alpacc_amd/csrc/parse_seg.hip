@@ -389,7 +389,12 @@ struct ScanOut {
   int64_t ob, ub, os, us;      // list lengths
   // largest per-tile counts: nodes (slots less terminal productions),
   // terminal productions, slots; they size seg_resolve's LDS staging
-  uint32_t tmax_node, tmax_term, tmax_slot, pad;
+  uint32_t tmax_node, tmax_term, tmax_slot;
+  // set by seg_resolve when a tile holds more nodes or terminal productions
+  // than the launch's staging (only a sequence sized for a capacity, with
+  // staging learned from earlier batches, can see it): the launch's results
+  // are then not valid and the batch runs again eagerly
+  uint32_t overflow;
 };
 
 // The eight running sums over tiles: slots, terminal productions, bracket
@@ -555,6 +560,7 @@ struct Resolve {
   int64_t* tok_par;
   uint8_t* deriv;  // flat derivation (parser-only mode) or null
   int tree;
+  uint32_t* overflow;  // ScanOut::overflow of the launch
 };
 
 // Last lane l <= from whose minimum (tb[0][l]) is below L, or -1.  tb[k][l]
@@ -677,6 +683,13 @@ __device__ __forceinline__ void resolve_tile(const Resolve& r, uint32_t tile, ui
   block_scan4(v, tot, ws);
   const int32_t slot0 = v[0], term0 = v[1], bS = v[2], sS = v[3];
   const int32_t tile_np = tot[0], tile_nt = tot[1];
+  // a tile beyond the staging (possible only with staging sized from
+  // earlier batches) flags the launch and does nothing else: no stack, list
+  // or output entry is written for it
+  if ((r.tree ? tile_np - tile_nt : tile_np) > r.ncap || (r.tree ? tile_nt : 0) > r.tcap) {
+    if (tid == 0) atomicOr(r.overflow, 1u);
+    return;
+  }
   const int32_t cb0 = slot0 - term0;  // tile-local compact index of the lane's first node
   l_cb[tid] = (uint16_t)cb0;
   l_tb[tid] = (uint16_t)term0;
@@ -964,6 +977,7 @@ __device__ int64_t tile_prev_less(const TileTree& t, uint64_t i, int32_t v) {
 
 struct Far {
   Seg s;
+  const uint32_t* overflow;  // set: seg_resolve skipped tiles, the lists are not this launch's
   const TileInfo* info;
   const SegBase* sb;
   TileTree tb, ts;
@@ -1019,6 +1033,7 @@ __device__ __forceinline__ void far_tile(const Far& f, uint32_t tile, uint32_t l
 
 // One wave per tile, the grid striding over the launch's tiles.
 __global__ __launch_bounds__(256) void far_resolve(Far f) {
+  if (*f.overflow) return;  // the lists were not written: the batch runs again
   const uint32_t nt = *f.s.nt_dev, lane = threadIdx.x & (WAVE - 1);
   for (uint32_t tile = blockIdx.x * (256 / WAVE) + threadIdx.x / WAVE; tile < nt; tile += gridDim.x * (256 / WAVE))
     far_tile(f, tile, lane);
@@ -1166,6 +1181,17 @@ __global__ __launch_bounds__(PLAN_WG) void parse_plan(const uint64_t* tok_off, c
 }
 
 }  // namespace
+
+bool parse_scan_overflow(const Context& c, uint64_t count) {
+  const ScanOut* so = reinterpret_cast<const ScanOut*>(c.h_pres.as<uint8_t>() + count * sizeof(ParseResult));
+  return so->overflow != 0;
+}
+
+void parse_scan_maxima(const Context& c, uint64_t count, bool tree, int* nodes, int* terms) {
+  const ScanOut* so = reinterpret_cast<const ScanOut*>(c.h_pres.as<uint8_t>() + count * sizeof(ParseResult));
+  *nodes = (int)(tree ? so->tmax_node : so->tmax_slot);
+  *terms = tree ? (int)so->tmax_term : 0;
+}
 
 int parse_per_position(const GrammarDesc* g) {
   return std::max(std::max(1, std::max(g->max_bpush, g->max_spush)), std::max(g->max_bpop, g->max_spop));
@@ -1315,8 +1341,14 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
   // grids: one workgroup per tile when the tile count is known, else as
   // many as the device holds at once, striding over the tiles
   const uint32_t g_map = (uint32_t)((ntiles + 255) / 256);
-  const uint32_t g_prof = (uint32_t)(caps && ntiles > 4096 * AWAVES ? 4096 : (ntiles + AWAVES - 1) / AWAVES);
-  const uint32_t g_res = (uint32_t)(caps && ntiles > 2048 ? 2048 : ntiles);
+  // A sequence sized for a capacity launches a workgroup (pass A: a wave)
+  // per tile the capacity can hold, and those past the actual count return
+  // at once: cheaper than the striding instantiations, which hold more
+  // registers across their loop (fewer waves per SIMD).  Only a capacity
+  // beyond 2^20 tiles strides.
+  const bool stride = caps && ntiles > (1u << 20);
+  const uint32_t g_prof = (uint32_t)(stride ? 4096 : (ntiles + AWAVES - 1) / AWAVES);
+  const uint32_t g_res = (uint32_t)(stride ? 2048 : ntiles);
   // far_resolve strides: 2048 workgroups of 4 tile-waves measured faster
   // than a wave per tile (0.136 against 0.143 ms on 1 GiB of JSON)
   const uint32_t g_far = (uint32_t)((ntiles + 3) / 4 > 2048 ? 2048 : (ntiles + 3) / 4);
@@ -1324,7 +1356,7 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
     hipLaunchKernelGGL(pseg_map, dim3(g_map), dim3(256), 0, st, d_tf, s.count, s.nt_dev,
                        const_cast<uint32_t*>(s.tile_seg));
     if (!caps) HIP_TRY(tm.begin(st, "seg_profile"));
-    hipLaunchKernelGGL(caps ? profile_kernel<true>(g->q + g->k, PT) : profile_kernel<false>(g->q + g->k, PT),
+    hipLaunchKernelGGL(stride ? profile_kernel<true>(g->q + g->k, PT) : profile_kernel<false>(g->q + g->k, PT),
                        dim3(g_prof), dim3(AWG), 0, st, s, rec, reinterpret_cast<uint16_t*>(base + off_pid));
     if (!caps) HIP_TRY(tm.end(st));
     if (!caps) HIP_TRY(tm.begin(st, "tile_scan"));
@@ -1371,15 +1403,19 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
     // staging sized to the largest tile when the scan's maxima are on the
     // host (fewer LDS bytes, more workgroups per CU), else to the grammar's
     // per-position maxima
-    const int n_st = caps ? ncap : (int)(tree ? so.tmax_node : so.tmax_slot);
-    const int t_st = caps ? tcap : (tree ? (int)so.tmax_term : 0);
+    // (at capacity: the staging the context learned from an eager run of
+    // the sequence, c.cap_stage_*, else the grammar's maxima)
+    const int n_st = caps ? (c.cap_stage_n > 0 && c.cap_stage_n < ncap ? c.cap_stage_n : ncap)
+                          : (int)(tree ? so.tmax_node : so.tmax_slot);
+    const int t_st = caps ? (c.cap_stage_t > 0 && c.cap_stage_t < tcap ? c.cap_stage_t : tcap)
+                          : (tree ? (int)so.tmax_term : 0);
     if (n_st > ncap || t_st > tcap) return fail(ALPACC_ERR_LIMIT, "tile counts beyond the grammar's maxima");
     const size_t lds_r = resolve_lds(n_st, t_st, br8 ? 1 : 2);
     Resolve rs{s,         reinterpret_cast<const uint16_t*>(base + off_pid),
                info,      sb,       bcap,  scap, n_st, t_st,
                ob_list,   ub_list,   os_list,  us_list,       c.tr_prod.as<uint8_t>(),
                c.tr_parent.as<int64_t>(), c.tr_tokpar.as<int64_t>(), tree ? nullptr : c.ps_d.as<uint8_t>(),
-               tree ? 1 : 0};
+               tree ? 1 : 0, &sout->overflow};
     void (*kern[2])(Resolve);  // one tile per workgroup, striding
     if (g->max_bpop <= 2 && g->max_bpush <= 3 && g->max_prods <= 4 && g->max_spush <= 3)
       resolve_kernels<2, 3, 4, 3>(br8, PT, APL, kern);
@@ -1389,9 +1425,9 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
       resolve_kernels<4, 4, 8, 24>(br8, PT, APL, kern);
     for (auto k : kern) HIP_TRY(grant_lds(reinterpret_cast<const void*>(k), lds_b));
     if (!caps) HIP_TRY(tm.begin(st, "seg_resolve"));
-    hipLaunchKernelGGL(kern[caps ? 1 : 0], dim3(g_res), dim3(RWG), lds_r, st, rs);
+    hipLaunchKernelGGL(kern[stride ? 1 : 0], dim3(g_res), dim3(RWG), lds_r, st, rs);
     if (!caps) HIP_TRY(tm.end(st));
-    Far fr{s, info, sb, tb, ts, ob_list, ub_list, os_list, us_list, c.tr_parent.as<int64_t>(),
+    Far fr{s, &sout->overflow, info, sb, tb, ts, ob_list, ub_list, os_list, us_list, c.tr_parent.as<int64_t>(),
            c.tr_tokpar.as<int64_t>(), tree ? 1 : 0};
     if (!caps) HIP_TRY(tm.begin(st, "far_resolve"));
     hipLaunchKernelGGL(far_resolve, dim3(g_far), dim3(256), 0, st, fr);
@@ -1402,6 +1438,9 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
   HIP_TRY(hipGetLastError());
   if (timed) HIP_TRY(hipEventRecord(c.ev[3], st));
   c.cp_parse_down = Context::Copy{c.h_pres.ptr, res, count * sizeof(ParseResult), hipMemcpyDeviceToHost};
+  // the scan's totals (maxima, overflow) land after the per-input results
+  c.cp_scan_down = Context::Copy{c.h_pres.as<uint8_t>() + count * sizeof(ParseResult), sout, sizeof(ScanOut),
+                                 hipMemcpyDeviceToHost};
   if (!c.capturing) HIP_TRY(hipMemcpyAsync(c.h_pres.ptr, res, count * sizeof(ParseResult), hipMemcpyDeviceToHost, st));
   if (!caps) {
     HIP_TRY(hipStreamSynchronize(st));

@@ -83,6 +83,10 @@ struct Context {
   int lex_ipt = LEX_IPT, lex_lb = LEX_LB_WIN, lex_wpc = 0;
   int lex_grid = 0, lex_grid2 = 0;
   int parse_pt = 1024, parse_apl = 4;  // parser tile shape: positions per tile and per resolve lane
+  // seg_resolve's staging (nodes, terminal productions per tile) in a
+  // sequence sized for a capacity, learned from its eager run (0: the
+  // grammar's maxima)
+  int cap_stage_n = 0, cap_stage_t = 0;
   int resolve_br8 = -1;              // 1: every bracket symbol below 256 (-1: not checked yet)
   int resolve_bw = 0;                // bracket stack bytes an entry (0: 1 when resolve_br8, else 2)
   // resident tables
@@ -162,10 +166,10 @@ struct Context {
     hipMemcpyKind kind = hipMemcpyHostToDevice;
   };
   bool capturing = false;
-  Copy cp_meta_up, cp_lex_down, cp_parse_down;
+  Copy cp_meta_up, cp_lex_down, cp_parse_down, cp_scan_down;
   // what a replay restores (calls between replays reuse the fields)
   struct GraphState {
-    Copy up, lex_down, parse_down;
+    Copy up, lex_down, parse_down, scan_down;
     uint64_t lb_count = 0, pb_count = 0;
     const uint32_t* lb_flags = nullptr;
     const uint64_t* lb_tok = nullptr;

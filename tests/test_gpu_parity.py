@@ -871,3 +871,24 @@ def test_autotune_parser():
         assert be.run_case("both", small) == pyoracle.record("json", "both", small)
         with pytest.raises(hip.HipError):
             be.set_parse_shape(4096, 4)
+
+
+def test_graph_staging_overflow_falls_back():
+    """The graph path stages seg_resolve's outputs for the largest tile of
+    the batch it captured (plus a quarter).  A later batch within the same
+    capacity whose tiles hold many more tree nodes (nested arrays: several
+    nodes per token, against flat arrays of numbers) overflows that staging:
+    its tiles are skipped, the batch runs again eagerly, and the next batch
+    captures again.  Every record stays the oracle's."""
+    from alpacc_amd import hip
+    flat = [b"[" + b",".join(b"1" for _ in range(1500 + i)) + b"]" for i in range(64)]
+    deep = [b"[" * (1200 + i) + b"1" + b"]" * (1200 + i) for i in range(64)]
+    with hip.Backend("json") as be:
+        be.set_graphs(True)
+        for k, batch in enumerate([flat, flat, deep, deep, flat]):
+            print(f"graph staging batch #{k}", flush=True)
+            got = be.run_batch("both", batch)
+            exp = b"".join(pyoracle.record("json", "both", x) for x in batch)
+            assert first_diff(got, exp) is None, k
+        captures, replays = be.graph_stats()
+        assert captures >= 2, (captures, replays)
