@@ -182,20 +182,33 @@ __device__ __forceinline__ void seg_comp_scan(const uint16_t* __restrict__ compo
 
 // ------------------------------------------------------------------- L1
 
-template <int IPT>
+// SC: STEP holds, for endomorphism e and byte class c, the LDS byte offset
+// of the next endomorphism's row (id * nclass * 2) and the class table holds
+// 2c, so a byte costs one add and two LDS loads (no multiply); the row
+// offsets must fit in 16 bits and 2c in 8.  Without SC the tables hold ids
+// and classes.  CH independent chains walk consecutive parts of a full lane
+// interleaved, since every byte is an LDS load that depends on the last, and
+// their endomorphisms are then composed through the table in L2.
+template <int IPT, int CH, bool SC>
 __global__ __launch_bounds__(WG) void lex_fold(LexParams p) {
   extern __shared__ __align__(16) uint8_t lds[];
   uint8_t* s_class = lds;
   uint16_t* s_step = reinterpret_cast<uint16_t*>(lds + 256);
+  const uint8_t* s_stepb = lds + 256;
   __shared__ uint32_t s_wave[NW];
   __shared__ uint16_t s_ichunk[WG];
   const uint32_t tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
-  for (int i = tid; i < 256; i += WG) s_class[i] = p.byte_class[i];
-  for (int i = tid; i < p.nendo * p.nclass; i += WG) s_step[i] = p.step[i];
+  const uint32_t nendo = p.nendo, nclass = p.nclass, row = SC ? 2 * nclass : 1;
+  for (int i = tid; i < 256; i += WG) s_class[i] = (uint8_t)(SC ? 2 * p.byte_class[i] : p.byte_class[i]);
+  for (int i = tid; i < p.nendo * p.nclass; i += WG) s_step[i] = (uint16_t)(p.step[i] * row);
   __syncthreads();
   const uint16_t* __restrict__ compose = p.compose;
-  const uint32_t nendo = p.nendo, nclass = p.nclass;
   auto comp = [&](uint32_t a, uint32_t b) -> uint32_t { return compose[a * nendo + b]; };
+  // one byte: the walk's state x (SC: a row offset, else an id) and byte b
+  auto step = [&](uint32_t x, uint32_t b) -> uint32_t {
+    if (SC) return *reinterpret_cast<const uint16_t*>(s_stepb + x + s_class[b]);
+    return s_step[__umul24(x, nclass) + s_class[b]];
+  };
 
   const uint32_t nt = *p.nt_dev;
   for (uint32_t tile = blockIdx.x; tile < nt; tile += gridDim.x) {
@@ -213,12 +226,27 @@ __global__ __launch_bounds__(WG) void lex_fold(LexParams p) {
     if (valid == IPT) {
       uint32_t a[IPT / 4 + 1];
       load_lane<IPT>(p, ls.base, ls.shift, a);
+      constexpr int KC = IPT / 4 / CH;  // words per chain
+      uint32_t ec[CH];
 #pragma unroll
-      for (int k = 0; k < IPT / 4; ++k) {
-        const uint32_t wk = lane_word(a, k, ls.shift, e);
+      for (int c = 0; c < CH; ++c) ec[c] = 0;
 #pragma unroll
-        for (int b = 0; b < 4; ++b) e = s_step[e * nclass + s_class[(wk >> (8 * b)) & 0xFFu]];
+      for (int k = 0; k < KC; ++k) {
+        uint32_t wk[CH];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) wk[c] = __builtin_amdgcn_alignbyte(a[c * KC + k + 1], a[c * KC + k], ls.shift);
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+          for (int c = 0; c < CH; ++c) ec[c] = step(ec[c], (wk[c] >> (8 * b)) & 0xFFu);
       }
+#pragma unroll
+      for (int c = 0; c < CH; ++c) ec[c] /= row;
+#pragma unroll
+      for (int w = 1; w < CH; w <<= 1)
+#pragma unroll
+        for (int c = 0; c + w < CH; c += 2 * w) ec[c] = comp(ec[c], ec[c + w]);
+      e = ec[0];
     } else if (valid > 0) {
       uint32_t a[IPT / 4 + 1];
       load_lane_part<IPT>(p, ls.base, ls.shift, valid, a);
@@ -227,10 +255,11 @@ __global__ __launch_bounds__(WG) void lex_fold(LexParams p) {
         const uint32_t wk = lane_word(a, k, ls.shift, e);
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-          const uint32_t n = s_step[e * nclass + s_class[(wk >> (8 * b)) & 0xFFu]];
+          const uint32_t n = step(e, (wk >> (8 * b)) & 0xFFu);
           e = (uint32_t)(4 * k + b) < valid ? n : e;
         }
       }
+      e /= row;
     }
     // segmented scan of (restart, endomorphism), restarting at the first
     // chunk of every input: (f1, a) . (f2, b) = (f1 | f2, f2 ? b : a then b);
@@ -788,6 +817,10 @@ uint32_t emit_rcap(const LexParams& p) {
   return r < 16 ? 16 : (r > ipt ? ipt : r);
 }
 
+// The fold's tables hold LDS row offsets when those fit in 16 bits and
+// doubled classes in 8.
+bool fold_scaled(const LexParams& p) { return (uint64_t)p.nendo * p.nclass * 2 <= 0x10000u && p.nclass <= 128; }
+
 // The kernels of an items-per-thread shape.
 struct LexKernels {
   void (*fold)(LexParams);
@@ -795,8 +828,9 @@ struct LexKernels {
 };
 LexKernels lex_kernels(const LexParams& p) {
   const bool sc = emit_scaled(p);
-  if (p.ipt == 64) return {lex_fold<64>, sc ? lex_emit<true, 64> : lex_emit<false, 64>};
-  return {lex_fold<128>, sc ? lex_emit<true, 128> : lex_emit<false, 128>};
+  const bool fs = fold_scaled(p);
+  if (p.ipt == 64) return {fs ? lex_fold<64, 2, true> : lex_fold<64, 1, false>, sc ? lex_emit<true, 64> : lex_emit<false, 64>};
+  return {fs ? lex_fold<128, 2, true> : lex_fold<128, 1, false>, sc ? lex_emit<true, 128> : lex_emit<false, 128>};
 }
 }  // namespace
 
