@@ -77,7 +77,11 @@ struct Seg {
   const uint32_t* nt_dev;
   uint32_t count, ntiles;
   int q, k, base, empty_t, start_t, end_t;
-  const uint16_t* window;
+  // per window code: the pair id (bits 0..15, NO_PAIR for a window without
+  // one) and the head of its record: bracket pops and pushes, productions,
+  // terminal productions, slot pops and pushes (a byte each, bits 16..63;
+  // zero without a pair)
+  const uint64_t* window;
   const uint4* rec;  // 64-byte pair records, 4 x uint4 each; record npairs is empty
   int npairs;
   uint32_t* flags;   // [count] BAD_* bits
@@ -100,9 +104,9 @@ __device__ __forceinline__ TileOf tile_of(const Seg& s, uint32_t tile) {
   return t;
 }
 
-// Pair id of framed position p, or NO_PAIR (32-bit positions: an input has
-// fewer than 2^31 tokens).
-__device__ __forceinline__ uint32_t pair_at(const Seg& s, const TileOf& t, int64_t p64) {
+// Window entry (pair id and record head) of framed position p, or NO_PAIR
+// (32-bit positions: an input has fewer than 2^31 tokens).
+__device__ __forceinline__ uint64_t pair_at(const Seg& s, const TileOf& t, int64_t p64) {
   const int32_t p = (int32_t)p64, m = (int32_t)t.m;
   uint32_t code = 0;
   const int w = s.q + s.k;
@@ -121,12 +125,12 @@ __device__ __forceinline__ uint32_t pair_at(const Seg& s, const TileOf& t, int64
 
 constexpr int MAXW = 8;  // widest window (q + k) the register path of pass A covers
 
-// Pair ids of framed positions p0 .. p0 + n - 1 (NO_PAIR past the input's
-// end): the framed tokens all of them need are loaded first, one
+// Window entries of framed positions p0 .. p0 + n - 1 (NO_PAIR past the
+// input's end): the framed tokens all of them need are loaded first, one
 // independent load each, then every window code is formed in registers.
 // W bounds the window width (q + k <= W).
 template <int W, int N>
-__device__ __forceinline__ void window_ids(const Seg& s, const TileOf& t, int32_t p0, uint32_t* ids) {
+__device__ __forceinline__ void window_ids(const Seg& s, const TileOf& t, int32_t p0, uint64_t* ids) {
   const int w = s.q + s.k;
   const int32_t m = (int32_t)t.m;
   uint32_t v[N + W - 1];
@@ -153,7 +157,7 @@ __device__ __forceinline__ void window_ids(const Seg& s, const TileOf& t, int32_
         code = code * (uint32_t)s.base + v[r + j];
       }
     }
-    ids[r] = ok ? (uint32_t)s.window[code] : NO_PAIR;
+    ids[r] = ok ? s.window[code] : (uint64_t)NO_PAIR;
   }
 }
 
@@ -226,7 +230,7 @@ constexpr int AWAVES = AWG / WAVE;
 // but the lanes at an input's two ends) load their N + W - 1 tokens
 // directly and roll the window code along; the others take window_ids.
 template <int W, int N>
-__device__ __forceinline__ void window_ids_exact(const Seg& s, const TileOf& t, int32_t p0, uint32_t* ids) {
+__device__ __forceinline__ void window_ids_exact(const Seg& s, const TileOf& t, int32_t p0, uint64_t* ids) {
   const int32_t f0 = p0 - s.q;  // framed index of the first token the windows read
   if (f0 > 0 && f0 + N + W - 2 < (int32_t)t.m - 1) {
     uint32_t v[N + W - 1];
@@ -265,7 +269,7 @@ __device__ __forceinline__ void window_ids_exact(const Seg& s, const TileOf& t, 
     for (int r = 0; r < N; ++r) {
       code = code * base + v[r + W - 1];
       bad += v[r + W - 1] >= base;
-      ids[r] = bad ? NO_PAIR : (uint32_t)s.window[code];
+      ids[r] = bad ? (uint64_t)NO_PAIR : s.window[code];
       code -= v[r] * top;
       bad -= v[r] >= base;
     }
@@ -281,27 +285,26 @@ __device__ __forceinline__ void window_ids_exact(const Seg& s, const TileOf& t, 
 template <int W, int PPL>
 __device__ __forceinline__ void profile_round(const Seg& s, const TileOf& t, int32_t p0, uint16_t* pid_at, int32_t& np,
                                               int32_t& nt, bool& bad, SM& wb, SM& ws) {
-  uint32_t ids[PPL];
+  // one load per position: the window entry holds the record's head
+  uint64_t wr[PPL];
   if (W > 0) {
-    window_ids_exact<(W > 0 ? W : 1), PPL>(s, t, p0, ids);
+    window_ids_exact<(W > 0 ? W : 1), PPL>(s, t, p0, wr);
   } else if (s.q + s.k <= MAXW) {
-    window_ids<MAXW, PPL>(s, t, p0, ids);
+    window_ids<MAXW, PPL>(s, t, p0, wr);
   } else {
 #pragma unroll
-    for (int r = 0; r < PPL; ++r) ids[r] = p0 + r < t.m ? pair_at(s, t, p0 + r) : NO_PAIR;
+    for (int r = 0; r < PPL; ++r) wr[r] = p0 + r < t.m ? pair_at(s, t, p0 + r) : (uint64_t)NO_PAIR;
   }
-  uint2 h[PPL];
-#pragma unroll
-  for (int r = 0; r < PPL; ++r)  // positions past the input's end read the empty record
-    h[r] = *reinterpret_cast<const uint2*>(s.rec + 4 * (ids[r] == NO_PAIR ? (uint32_t)s.npairs : ids[r]));
+  uint32_t ids[PPL];
   SM b{0, 0}, sl{0, 0};
 #pragma unroll
   for (int r = 0; r < PPL; ++r) {
+    ids[r] = (uint32_t)wr[r] & 0xFFFFu;
     bad |= p0 + r < t.m && ids[r] == NO_PAIR;
-    const uint2 x = h[r];
-    const int32_t bpop = x.x & 0xFF, bpush = (x.x >> 8) & 0xFF, spop = x.y & 0xFF, spush = (x.y >> 8) & 0xFF;
-    np += (x.x >> 16) & 0xFF;
-    nt += x.x >> 24;
+    const uint32_t xx = (uint32_t)(wr[r] >> 16), xy = (uint32_t)(wr[r] >> 48);
+    const int32_t bpop = xx & 0xFF, bpush = (xx >> 8) & 0xFF, spop = xy & 0xFF, spush = (xy >> 8) & 0xFF;
+    np += (xx >> 16) & 0xFF;
+    nt += xx >> 24;
     b = sm_cat(b, SM{bpush - bpop, -bpop});
     sl = sm_cat(sl, SM{spush - spop, -spop});
   }
@@ -625,6 +628,7 @@ template <int MBPOP, int MBPUSH, int MNP, int MSPUSH, typename BT, int PT, int A
 __device__ __forceinline__ void resolve_tile(const Resolve& r, uint32_t tile, uint32_t tid) {
   constexpr int RWG = PT / APL;    // lanes per tile
   constexpr int RNW = RWG / WAVE;  // waves per tile
+  constexpr bool NARROW = MBPUSH <= 4 && MSPUSH <= 4;
   // dynamic LDS: bracket stacks BT [bcap][RWG], slot stacks u8 [scap][RWG],
   // staged parents of nodes u16 [ncap] and of terminal productions u16
   // [tcap] (tile-local compact indices), staged productions u8 [ncap]
@@ -731,9 +735,13 @@ __device__ __forceinline__ void resolve_tile(const Resolve& r, uint32_t tile, ui
     }
     bdep -= nbpop;
     bmin = bdep < bmin ? bdep : bmin;
+    // pushes of the narrow instantiations are written up to the maximum
+    // with no test: an entry past the window's own pushes lies above the
+    // stack top, where the next push writes before anything reads it (the
+    // room then counts the instantiation's pushes for every position)
 #pragma unroll
     for (int e = 0; e < MBPUSH; ++e)
-      if (e < nbpush) bst[(bsp + e) * RWG] = (BT)rec_br(r0, r1, 4 + e);
+      if (NARROW || e < nbpush) bst[(bsp + e) * RWG] = (BT)rec_br(r0, r1, 4 + e);
     bsp += nbpush;
     bdep += nbpush;
     const int32_t cbk = slot - tinc;  // tile-local compact index of the position's first node
@@ -774,7 +782,7 @@ __device__ __forceinline__ void resolve_tile(const Resolve& r, uint32_t tile, ui
     }
 #pragma unroll
     for (int e = 0; e < MSPUSH; ++e)
-      if (e < nspush) sst[(ssp + e) * RWG] = (uint8_t)(cbk - cb0 + rec_owner(r2, r3, e));
+      if (NARROW || e < nspush) sst[(ssp + e) * RWG] = (uint8_t)(cbk - cb0 + rec_owner(r2, r3, e));
     ssp += nspush;
     sdep += nspush;
     slot += npk;
@@ -1225,7 +1233,19 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
   const int PT = c.parse_pt, APL = c.parse_apl, RWG = PT / APL;
   if (!parse_shape_ok(PT, APL)) return fail(ALPACC_ERR_ARG, "parser tile shape %d / %d", PT, APL);
   const int ncap = PT * g->max_prods, tcap = PT * g->max_nt;
-  const int bcap = APL * (g->max_bpop + g->max_bpush), scap = APL * (g->max_spop + g->max_spush);
+  // the resolve kernel's instantiation: window maxima it is compiled for
+  int ib[4];
+  if (g->max_bpop <= 2 && g->max_bpush <= 3 && g->max_prods <= 4 && g->max_spush <= 3)
+    ib[0] = 2, ib[1] = 3, ib[2] = 4, ib[3] = 3;
+  else if (g->max_bpop <= 3 && g->max_bpush <= 4 && g->max_prods <= 4 && g->max_spush <= 4)
+    ib[0] = 3, ib[1] = 4, ib[2] = 4, ib[3] = 4;
+  else
+    ib[0] = 4, ib[1] = 4, ib[2] = 8, ib[3] = 24;
+  // lane stack room: a position's pops and pushes, the pushes counted to
+  // the narrow instantiations' maxima (those kernels write that many)
+  const bool narrow = ib[3] <= 4;
+  const int bcap = APL * (g->max_bpop + (narrow ? ib[1] : g->max_bpush));
+  const int scap = APL * (g->max_spop + (narrow ? ib[3] : g->max_spush));
   if (c.resolve_br8 < 0) c.resolve_br8 = bracket_symbol_max(g) < 256 ? 1 : 0;
   const bool br8 = c.resolve_bw ? c.resolve_bw == 1 : c.resolve_br8 == 1;
   // seg_resolve's dynamic LDS with node / terminal staging for n / t entries
@@ -1315,7 +1335,7 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
   s.empty_t = g->empty_t;
   s.start_t = g->start_t;
   s.end_t = g->end_t;
-  s.window = c.d_window;
+  s.window = c.d_wrec;
   s.rec = reinterpret_cast<const uint4*>(c.d_pair_rec);
   s.npairs = g->npairs;
   s.flags = reinterpret_cast<uint32_t*>(base + off_fl);
@@ -1417,9 +1437,9 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
                c.tr_parent.as<int64_t>(), c.tr_tokpar.as<int64_t>(), tree ? nullptr : c.ps_d.as<uint8_t>(),
                tree ? 1 : 0, &sout->overflow};
     void (*kern[2])(Resolve);  // one tile per workgroup, striding
-    if (g->max_bpop <= 2 && g->max_bpush <= 3 && g->max_prods <= 4 && g->max_spush <= 3)
+    if (ib[1] == 3)
       resolve_kernels<2, 3, 4, 3>(br8, PT, APL, kern);
-    else if (g->max_bpop <= 3 && g->max_bpush <= 4 && g->max_prods <= 4 && g->max_spush <= 4)
+    else if (narrow)
       resolve_kernels<3, 4, 4, 4>(br8, PT, APL, kern);
     else
       resolve_kernels<4, 4, 8, 24>(br8, PT, APL, kern);

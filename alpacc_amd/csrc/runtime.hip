@@ -162,7 +162,6 @@ int alpacc_hip_open(const char* grammar, int device, alpacc_hip_ctx** out) {
       (r = upload(&c.d_delta_both, g->delta_both, (size_t)g->nstate * 256)) ||
       (r = upload(&c.d_token_lex, g->token_lex, (size_t)g->nstate)) ||
       (r = upload(&c.d_token_both, g->token_both, (size_t)g->nstate)) ||
-      (r = upload(&c.d_window, g->window, (size_t)g->nwindow)) ||
       (r = upload(&c.d_spans, g->spans, (size_t)g->npairs * 4)) ||
       (r = upload(&c.d_profile, g->profile, (size_t)g->npairs * 8)) ||
       (r = upload(&c.d_pair_rec, g->pair_rec, (size_t)(g->npairs + 1) * 64)) ||
@@ -173,6 +172,20 @@ int alpacc_hip_open(const char* grammar, int device, alpacc_hip_ctx** out) {
       (r = upload(&c.d_isterm, g->prod_is_terminal, (size_t)g->nproductions)) || (r = c.small.reserve(4096))) {
     alpacc_hip_close(h);
     return r;
+  }
+  {  // window entries: pair id and the first 6 bytes of its record
+    std::vector<uint64_t> wr((size_t)g->nwindow);
+    for (size_t i = 0; i < wr.size(); ++i) {
+      const uint32_t id = g->window[i];
+      uint64_t v = id;
+      if (id != 0xFFFFu)
+        for (int b = 0; b < 6; ++b) v |= (uint64_t)g->pair_rec[(size_t)id * 64 + b] << (16 + 8 * b);
+      wr[i] = v;
+    }
+    if ((r = upload(&c.d_wrec, wr.data(), wr.size()))) {
+      alpacc_hip_close(h);
+      return r;
+    }
   }
   e = hipHostMalloc(reinterpret_cast<void**>(&c.host_small), 4096, hipHostMallocDefault);
   if (e != hipSuccess) {
@@ -189,7 +202,7 @@ void alpacc_hip_close(alpacc_hip_ctx* h) {
   Context& c = h->c;
   (void)hipSetDevice(c.device);
   if (c.graph) (void)hipGraphExecDestroy(c.graph);
-  void* ptrs[] = {c.d_class, c.d_step, c.d_compose, c.d_endo_init, c.d_delta_lex, c.d_delta_both, c.d_token_lex, c.d_token_both, c.d_window,
+  void* ptrs[] = {c.d_class, c.d_step, c.d_compose, c.d_endo_init, c.d_delta_lex, c.d_delta_both, c.d_token_lex, c.d_token_both, c.d_wrec,
                   c.d_spans, c.d_profile, c.d_pair_rec, c.d_prod_info, c.d_stacks, c.d_productions, c.d_arity, c.d_isterm};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);

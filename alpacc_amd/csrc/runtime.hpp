@@ -98,7 +98,9 @@ struct Context {
   uint16_t* d_delta_both = nullptr;
   uint8_t* d_token_lex = nullptr;
   uint8_t* d_token_both = nullptr;
-  uint16_t* d_window = nullptr;
+  // per window code: pair id | record head << 16 (the parser's pass A reads
+  // one entry per position)
+  uint64_t* d_wrec = nullptr;
   int32_t* d_spans = nullptr;
   uint8_t* d_profile = nullptr;
   uint8_t* d_pair_rec = nullptr;
