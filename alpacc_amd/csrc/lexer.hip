@@ -435,14 +435,16 @@ struct EmitWords {
   uint32_t wave_cnt[NW], wave_last[NW];
 };
 
-// LDS layout of lex_emit: DFA rows, the staged slots, the lanes' kept-token
-// terminals (rcap rows of WG bytes and four sink rows), the workgroup words.
+// LDS layout of lex_emit: DFA rows, the staged slots (start and end,
+// tile-relative; input; terminal), the lanes' kept-token terminals (rcap
+// rows of WG bytes and four sink rows), the workgroup words.
 struct EmitLayout {
-  uint32_t start, end, term, lterm, words, total;
+  uint32_t start, end, seg, term, lterm, words, total;
   __host__ __device__ EmitLayout(uint32_t nstate, uint32_t rcap) {
     start = (nstate * ROWB + 15u) & ~15u;
-    end = start + 4 * STAGE;
-    term = end + 2 * STAGE;
+    end = start + 2 * STAGE;
+    seg = end + 2 * STAGE;
+    term = seg + STAGE;
     lterm = term + STAGE;
     words = (lterm + (rcap + 4) * WG + 15u) & ~15u;
     total = words + (uint32_t)sizeof(EmitWords);
@@ -456,8 +458,9 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
   const uint32_t rcap = p.rcap;
   const EmitLayout L((uint32_t)p.nstate, rcap);
   uint16_t* s_rows = reinterpret_cast<uint16_t*>(lds);              // [nstate][ROW]
-  uint32_t* s_start = reinterpret_cast<uint32_t*>(lds + L.start);   // [STAGE] token start
+  uint16_t* s_start = reinterpret_cast<uint16_t*>(lds + L.start);  // [STAGE] start, tile-relative
   uint16_t* s_end = reinterpret_cast<uint16_t*>(lds + L.end);       // [STAGE] end, tile-relative
+  uint8_t* s_seg = lds + L.seg;                                     // [STAGE] input, from the tile's first
   uint8_t* s_term = lds + L.term;                                   // [STAGE] terminal
   uint16_t* s_ichunk = reinterpret_cast<uint16_t*>(s_term);  // [WG] before the staging
   // [WG] after walk 1, in its sink rows
@@ -643,9 +646,14 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
     // lane, else (NO_START) the last boundary before the tile, known after
     // the lookback; never before the token's input's first byte (applied at
     // the write-out, where a boundary of an earlier input gives way to it).
-    // Boundaries travel as position + 1, 0 meaning none.
-    constexpr uint32_t NO_START = 0xFFFFFFFFu;
-    const uint32_t start_in = elast ? elast - 1 : NO_START;
+    // Boundaries travel as position + 1, 0 meaning none; staged starts are
+    // tile-relative.  A token belongs to the input of the lane that closes
+    // it: a lane closing a token at its first byte is not its input's first
+    // chunk (no token closes at an input's first byte), so the token's last
+    // byte lies in the same input.
+    constexpr uint32_t NO_START = 0xFFFFu;
+    const uint32_t start_in = elast ? elast - 1 - (uint32_t)tbase : NO_START;
+    const uint32_t dl = ls.seg - ts.s_lo;  // the lane's input, from the tile's first
     uint64_t slot0 = 0;
     uint32_t carried = 0;
     bool overflow = false;
@@ -666,7 +674,7 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
 #pragma unroll
           for (int q = 0; q < IPT / 32; ++q) {
             uint32_t m = km[q];
-            const uint32_t pm = bm[q], qb = (uint32_t)base + 32 * q;
+            const uint32_t pm = bm[q], qb = lane_off + 32 * q;
             while (m) {
               const uint32_t jb = __builtin_ctz(m);
               m &= m - 1;
@@ -674,8 +682,9 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
               const uint32_t i = d + k;
               if (i < STAGE) {
                 s_term[i] = lds[L.lterm + k * WG + swz];
-                s_start[i] = below ? qb + 31 - __builtin_clz(below) : start;
+                s_start[i] = (uint16_t)(below ? qb + 31 - __builtin_clz(below) : start);
                 s_end[i] = (uint16_t)(lane_off + 32 * q + jb);
+                s_seg[i] = (uint8_t)dl;
               }
               ++k;
             }
@@ -697,12 +706,13 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
                 const uint32_t i = d + k;
                 if (i < STAGE) {
                   s_term[i] = (uint8_t)lds_load16(r + 512);
-                  s_start[i] = start;
+                  s_start[i] = (uint16_t)start;
                   s_end[i] = (uint16_t)(lane_off + j);
+                  s_seg[i] = (uint8_t)dl;
                 }
                 ++k;
               }
-              start = ((x >> RC::PROD) & 1u) ? (uint32_t)base + j : start;
+              start = ((x >> RC::PROD) & 1u) ? lane_off + j : start;
               r = RC::next_row(x, lds0);
             }
           }
@@ -711,8 +721,9 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
           const uint32_t i = d + k;
           if (i < STAGE) {
             s_term[i] = (uint8_t)fin_t;
-            s_start[i] = start;
+            s_start[i] = (uint16_t)start;
             s_end[i] = (uint16_t)(lane_off + valid);
+            s_seg[i] = (uint8_t)dl;
           }
         }
       }
@@ -738,7 +749,7 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
         for (uint32_t i = tid; i < m; i += WG) {
           const uint64_t slot = slot0 + lo + i;
           const uint32_t b0 = s_start[i];
-          const int64_t b = b0 == NO_START ? cs : (int64_t)b0;
+          const int64_t b = b0 == NO_START ? cs : (int64_t)tbase + b0;
           const uint64_t len = (uint64_t)((int64_t)tbase + s_end[i] - b);
           overflow |= len > max_len;
           p.terminals[slot] = s_term[i];
@@ -749,17 +760,12 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
         for (uint32_t i = tid; i < m; i += WG) {
           const uint64_t slot = slot0 + lo + i;
           const uint32_t e = s_end[i];
-          // the token's input: the one holding its last byte (before the
-          // tile when it closes at the tile's first byte: the first input)
-          uint32_t seg = ts.s_lo;
-          uint64_t seg_lo = ts.seg_lo;
-          const uint32_t d = e ? count_le(s_istart, e - 1) : 0u;
-          if (d) {
-            seg = ts.s_lo + d;
-            seg_lo = tbase + s_istart[d];
-          }
+          // the token's input, staged with it
+          const uint32_t d = s_seg[i];
+          const uint32_t seg = ts.s_lo + d;
+          const uint64_t seg_lo = d ? tbase + s_istart[d] : ts.seg_lo;
           const uint32_t b0 = s_start[i];
-          uint64_t b = b0 == NO_START ? (carried ? carried - 1 : 0u) : b0;
+          uint64_t b = b0 == NO_START ? (carried ? carried - 1 : 0u) : tbase + b0;
           b = b > seg_lo ? b : seg_lo;
           const uint64_t len = tbase + e - b;
           if (len > max_len) atomicOr(p.seg_flags + seg, LEX_REJECT_OVERFLOW);
