@@ -57,6 +57,13 @@ struct GrammarDesc {
   const uint8_t* terminal_names_lex;
   const uint8_t* terminal_names_both;
   const uint8_t* production_names;
+  // 0: window is a dense index by window code (nwindow = window_base^(q+k)
+  // entries); else log2 of the slots of an open-addressing table of the
+  // windows with a pair: window_keys[slot] the window code (all ones when
+  // empty) and window[slot] its pair id, nwindow slots, first slot = the
+  // code times 0x9E3779B97F4A7C15, top window_hash bits, then linear probing
+  int window_hash;
+  const uint64_t* window_keys;   // [nwindow] when hashed, else one unused entry
 };
 
 // A grammar by name: one loaded at run time (alpacc_hip_load_grammar), else

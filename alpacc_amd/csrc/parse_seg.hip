@@ -82,6 +82,11 @@ struct Seg {
   // terminal productions, slot pops and pushes (a byte each, bits 16..63;
   // zero without a pair)
   const uint64_t* window;
+  // hashed window tables (the grammar's dense index would be too large):
+  // the code of each of the 2^hbits slots, all ones when empty; null when
+  // the window entries are a dense index
+  const uint64_t* wkeys;
+  int hbits;
   const uint4* rec;  // 64-byte pair records, 4 x uint4 each; record npairs is empty
   int npairs;
   uint32_t* flags;   // [count] BAD_* bits
@@ -121,6 +126,34 @@ __device__ __forceinline__ uint64_t pair_at(const Seg& s, const TileOf& t, int64
     code = code * s.base + v;
   }
   return s.window[code];
+}
+
+// Window entry of framed position p in a hashed window table: the window
+// code (64 bits), then linear probing from slot code * 0x9E3779B97F4A7C15
+// (top hbits bits) to the code's slot (the table is at most half full, so
+// an empty slot ends every search).
+__device__ __forceinline__ uint64_t pair_hashed(const Seg& s, const TileOf& t, int64_t p64) {
+  const int32_t p = (int32_t)p64, m = (int32_t)t.m;
+  uint64_t code = 0;
+  const int w = s.q + s.k;
+  for (int j = 0; j < w; ++j) {
+    const int32_t f = p - s.q + j;
+    uint32_t v;
+    if (f <= 0 || f >= m - 1)
+      v = f == 0 ? (uint32_t)s.start_t : (f == m - 1 ? (uint32_t)s.end_t : (uint32_t)s.empty_t);
+    else
+      v = t.tk[f - 1];
+    if (v >= (uint32_t)s.base) return NO_PAIR;
+    code = code * (uint64_t)s.base + v;
+  }
+  const uint32_t mask = (1u << s.hbits) - 1u;
+  uint32_t slot = (uint32_t)((code * 0x9E3779B97F4A7C15ull) >> (64 - s.hbits));
+  for (;;) {
+    const uint64_t k = s.wkeys[slot];
+    if (k == code) return s.window[slot];
+    if (k == ~0ull) return NO_PAIR;
+    slot = (slot + 1) & mask;
+  }
 }
 
 constexpr int MAXW = 8;  // widest window (q + k) the register path of pass A covers
@@ -289,6 +322,9 @@ __device__ __forceinline__ void profile_round(const Seg& s, const TileOf& t, int
   uint64_t wr[PPL];
   if (W > 0) {
     window_ids_exact<(W > 0 ? W : 1), PPL>(s, t, p0, wr);
+  } else if (s.wkeys) {
+#pragma unroll
+    for (int r = 0; r < PPL; ++r) wr[r] = p0 + r < t.m ? pair_hashed(s, t, p0 + r) : (uint64_t)NO_PAIR;
   } else if (s.q + s.k <= MAXW) {
     window_ids<MAXW, PPL>(s, t, p0, wr);
   } else {
@@ -1336,6 +1372,8 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
   s.start_t = g->start_t;
   s.end_t = g->end_t;
   s.window = c.d_wrec;
+  s.wkeys = g->window_hash ? c.d_wkeys : nullptr;
+  s.hbits = g->window_hash;
   s.rec = reinterpret_cast<const uint4*>(c.d_pair_rec);
   s.npairs = g->npairs;
   s.flags = reinterpret_cast<uint32_t*>(base + off_fl);
@@ -1376,7 +1414,10 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
     hipLaunchKernelGGL(pseg_map, dim3(g_map), dim3(256), 0, st, d_tf, s.count, s.nt_dev,
                        const_cast<uint32_t*>(s.tile_seg));
     if (!caps) HIP_TRY(tm.begin(st, "seg_profile"));
-    hipLaunchKernelGGL(stride ? profile_kernel<true>(g->q + g->k, PT) : profile_kernel<false>(g->q + g->k, PT),
+    // a hashed window table takes the kernel for any width (its lookups
+    // probe the table)
+    const int pw = g->window_hash ? 0 : g->q + g->k;
+    hipLaunchKernelGGL(stride ? profile_kernel<true>(pw, PT) : profile_kernel<false>(pw, PT),
                        dim3(g_prof), dim3(AWG), 0, st, s, rec, reinterpret_cast<uint16_t*>(base + off_pid));
     if (!caps) HIP_TRY(tm.end(st));
     if (!caps) HIP_TRY(tm.begin(st, "tile_scan"));

@@ -182,7 +182,8 @@ int alpacc_hip_open(const char* grammar, int device, alpacc_hip_ctx** out) {
         for (int b = 0; b < 6; ++b) v |= (uint64_t)g->pair_rec[(size_t)id * 64 + b] << (16 + 8 * b);
       wr[i] = v;
     }
-    if ((r = upload(&c.d_wrec, wr.data(), wr.size()))) {
+    if ((r = upload(&c.d_wrec, wr.data(), wr.size())) ||
+        (g->window_hash && (r = upload(&c.d_wkeys, g->window_keys, (size_t)g->nwindow)))) {
       alpacc_hip_close(h);
       return r;
     }
@@ -202,7 +203,7 @@ void alpacc_hip_close(alpacc_hip_ctx* h) {
   Context& c = h->c;
   (void)hipSetDevice(c.device);
   if (c.graph) (void)hipGraphExecDestroy(c.graph);
-  void* ptrs[] = {c.d_class, c.d_step, c.d_compose, c.d_endo_init, c.d_delta_lex, c.d_delta_both, c.d_token_lex, c.d_token_both, c.d_wrec,
+  void* ptrs[] = {c.d_class, c.d_step, c.d_compose, c.d_endo_init, c.d_delta_lex, c.d_delta_both, c.d_token_lex, c.d_token_both, c.d_wrec, c.d_wkeys,
                   c.d_spans, c.d_profile, c.d_pair_rec, c.d_prod_info, c.d_stacks, c.d_productions, c.d_arity, c.d_isterm};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);

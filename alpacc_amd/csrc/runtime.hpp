@@ -101,6 +101,7 @@ struct Context {
   // per window code: pair id | record head << 16 (the parser's pass A reads
   // one entry per position)
   uint64_t* d_wrec = nullptr;
+  uint64_t* d_wkeys = nullptr;  // hashed window tables: the code of each slot
   int32_t* d_spans = nullptr;
   uint8_t* d_profile = nullptr;
   uint8_t* d_pair_rec = nullptr;

@@ -70,6 +70,7 @@ const Field FIELDS[] = {
     FI(max_brackets), FI(max_prods), FI(max_slot_items), FA(pair_rec, 1), FA(prod_info, 1),
     FI(max_bpop), FI(max_bpush), FI(max_spop), FI(max_spush), FI(max_nt), FI(length_param), FI(has_parser),
     FA(terminal_names_lex, 1), FA(terminal_names_both, 1), FA(production_names, 1),
+    FI(window_hash), FA(window_keys, 8),
 };
 #undef FI
 #undef FA
@@ -92,6 +93,7 @@ uint64_t need_entries(const GrammarDesc& g, size_t off) {
       off == offsetof(GrammarDesc, prod_info))
     return n(g.nproductions);
   if (off == offsetof(GrammarDesc, pair_rec)) return (uint64_t)(g.npairs + 1) * 64;
+  if (off == offsetof(GrammarDesc, window_keys)) return g.window_hash ? n(g.nwindow) : 1;
   return 0;
 }
 
@@ -161,6 +163,20 @@ int parse_grammar_tables(const uint8_t* data, size_t len, GrammarDesc* out, std:
     if (cnt < need_entries(g, off)) return fail(ALPACC_ERR_ARG, "tables file: an array is shorter than its sizes");
   if (g.nstate < 2 || g.nstate > 256 || g.nendo < 2 || g.nclass < 1 || g.q < 0 || g.k < 0 || g.q + g.k > 16)
     return fail(ALPACC_ERR_ARG, "tables file: sizes out of range");
+  // the window table covers every code the parser forms: a dense index of
+  // window_base^(q+k) entries, or a hashed table of 2^window_hash slots
+  if (g.has_parser) {
+    if (g.window_hash < 0 || g.window_hash > 30 || g.window_base < 1)
+      return fail(ALPACC_ERR_ARG, "tables file: bad window table");
+    if (g.window_hash) {
+      if ((uint64_t)g.nwindow != (1ull << g.window_hash))
+        return fail(ALPACC_ERR_ARG, "tables file: hashed window table of %d slots", g.nwindow);
+    } else {
+      uint64_t codes = 1;
+      for (int i = 0; i < g.q + g.k && codes <= (uint64_t)g.nwindow; ++i) codes *= (uint64_t)g.window_base;
+      if (codes > (uint64_t)g.nwindow) return fail(ALPACC_ERR_ARG, "tables file: window index shorter than its codes");
+    }
+  }
   *out = g;
   return 0;
 }

@@ -13,9 +13,13 @@ The GPU path does not use the oracle's packed-endomorphism format.  It needs:
   terminal numberings (lexer-only, combined), staged in LDS for the lexer's
   last pass;
 * the LLP pair table as a dense window index (``WINDOW[key] -> pair id``,
-  0xffff = no pair) over the padded q+k window, pair spans into the bracket
-  and production strings (brackets as u16: symbol | left << 15), arities and
-  the terminal-production flags.
+  0xffff = no pair) over the padded q+k window, or, when that index would
+  exceed MAX_WINDOW_ENTRIES, an open-addressing table of the windows that
+  have a pair (``WINDOW_KEYS[slot]`` the window code, all ones when empty,
+  ``WINDOW[slot]`` its pair id; slot = the code times WINDOW_HASH_MUL, top
+  bits; linear probing), pair spans into the bracket and production strings
+  (brackets as u16: symbol | left << 15), arities and the
+  terminal-production flags.
 """
 
 from __future__ import annotations
@@ -25,6 +29,13 @@ from typing import List
 from .tables import Grammar
 
 MAX_WINDOW_ENTRIES = 1 << 24
+WINDOW_HASH_MUL = 0x9E3779B97F4A7C15  # Fibonacci hashing of window codes
+EMPTY_KEY = (1 << 64) - 1
+
+
+def window_slot(code: int, bits: int) -> int:
+    """First slot of a window code in a hashed window table of 2^bits slots."""
+    return ((code * WINDOW_HASH_MUL) & EMPTY_KEY) >> (64 - bits)
 
 
 class EmitError(ValueError):
@@ -32,13 +43,17 @@ class EmitError(ValueError):
 
 
 def _arr(ctype: str, name: str, vals: List[int], per_line: int = 24) -> str:
-    vals = [str(int(v)) for v in vals] or ["0"]
+    sfx = "ull" if ctype == "uint64_t" else ""
+    vals = [str(int(v)) + sfx for v in vals] or ["0"]
     body = ",\n".join("  " + ", ".join(vals[i:i + per_line]) for i in range(0, len(vals), per_line))
     return f"static const {ctype} {name}[{len(vals)}] = {{\n{body}\n}};\n"
 
 
-def device_tables(g: Grammar) -> dict:
-    """Plain-int device tables (shared by the emitter and the tests)."""
+def device_tables(g: Grammar, hashed=None) -> dict:
+    """Plain-int device tables (shared by the emitter and the tests).
+    hashed: the window table as an open-addressing table (True), a dense
+    index (False), or (None) hashed only when the dense index would exceed
+    MAX_WINDOW_ENTRIES."""
     lb, lo = g.lex_both, g.lex_only
     endos = lb.endos
     dfa = lb.dfa
@@ -92,7 +107,7 @@ def device_tables(g: Grammar) -> dict:
         ignore_both=-1 if lb.ignore_id is None else lb.ignore_id,
         dead_lex=lo.dead_id, dead_both=lb.dead_id,
     )
-    d.update(_parser_tables(g) if g.has_parser else _no_parser(g))
+    d.update(_parser_tables(g, hashed) if g.has_parser else _no_parser(g))
     return d
 
 
@@ -104,24 +119,31 @@ def _no_parser(g: Grammar) -> dict:
         window=[], base=g.lex_both.dead_id + 3, npairs=0, spans=[], stacks=[], productions=[], n_productions=0,
         production_bytes=1, arity=[], prod_is_term=[], profile=[], max_slot_items=1, pair_rec=[0] * 64,
         prod_info=[], max_brackets=0, max_prods=0, max_bpop=0, max_bpush=0, max_spop=0, max_spush=0, max_nt=0,
+        window_hash=0, window_keys=[],
     )
 
 
-def _parser_tables(g: Grammar) -> dict:
+def _parser_tables(g: Grammar, hashed=None) -> dict:
     p = g.parser
     if p.production_bytes != 1:
         raise EmitError(f"{g.name}: {p.n_productions} productions exceed the u8 production field")
     base = p.end_t + 1
     w = p.q + p.k
     nwin = base ** w
-    if nwin > MAX_WINDOW_ENTRIES:
-        raise EmitError(f"{g.name}: window index of {nwin} entries is too large")
-    window = [0xFFFF] * nwin
-    for i, key in enumerate(p.keys):
+    if hashed is None:
+        hashed = nwin > MAX_WINDOW_ENTRIES
+    if nwin >= EMPTY_KEY:
+        raise EmitError(f"{g.name}: window codes of {w} digits base {base} exceed 64 bits")
+    codes = []
+    for key in p.keys:
         code = 0
         for t in key:
             code = code * base + t
-        window[code] = i
+        codes.append(code)
+    window = [] if hashed else [0xFFFF] * nwin
+    for i, code in enumerate(codes):
+        if not hashed:
+            window[code] = i
     top = 1 << (8 * p.bracket_bytes - 1)
     stacks16 = [(v & (top - 1)) | (0x8000 if v & top else 0) for v in p.stacks]
     if p.n_symbols >= 0x8000:
@@ -167,6 +189,7 @@ def _parser_tables(g: Grammar) -> dict:
     # Window-local compact index: production index minus the terminal
     # productions before it (nodes of the compact tree).
     rec = []
+    bad_pairs = set()
     eff = dict(bpop=0, bpush=0, spop=0, spush=0, nt=0)
     for i, (s0, s1, p0, p1) in enumerate(p.spans):
         st, ext = [], []
@@ -202,8 +225,9 @@ def _parser_tables(g: Grammar) -> dict:
         if len(slots) > 24:
             raise EmitError(f"{g.name}: a window keeps more than 24 child slots")
         if not ok:
-            for code in [c for c, v in enumerate(window) if v == i]:
-                window[code] = 0xFFFF
+            bad_pairs.add(i)
+            if not hashed:
+                window[codes[i]] = 0xFFFF
         eff["bpop"] = max(eff["bpop"], len(ext))
         eff["bpush"] = max(eff["bpush"], len(st))
         eff["spop"] = max(eff["spop"], spop)
@@ -216,6 +240,23 @@ def _parser_tables(g: Grammar) -> dict:
         rec += parent + [0] * (8 - len(parent))
         rec += slots + [0] * (24 - len(slots))
     rec += [0] * 64  # record npairs: no effect (positions past an input's end)
+    window_hash, window_keys = 0, []
+    if hashed:
+        # the windows with a pair id, at most half the slots filled
+        live = [(c, i) for i, c in enumerate(codes) if i not in bad_pairs]
+        bits = max(4, (2 * len(live) - 1).bit_length())
+        if bits > 30:
+            raise EmitError(f"{g.name}: {len(live)} windows exceed the hashed window table")
+        mask = (1 << bits) - 1
+        window_keys = [EMPTY_KEY] * (1 << bits)
+        window = [0xFFFF] * (1 << bits)
+        for c, i in live:
+            slot = window_slot(c, bits)
+            while window_keys[slot] != EMPTY_KEY:
+                slot = (slot + 1) & mask
+            window_keys[slot] = c
+            window[slot] = i
+        window_hash = bits
     prod_info = []
     for prod in range(p.n_productions):
         if p.arity[prod] > 127:
@@ -233,7 +274,7 @@ def _parser_tables(g: Grammar) -> dict:
         profile=profile, max_slot_items=max_slot_items, pair_rec=rec, prod_info=prod_info,
         max_brackets=p.max_brackets, max_prods=p.max_prods,
         max_bpop=eff["bpop"], max_bpush=eff["bpush"], max_spop=eff["spop"], max_spush=eff["spush"],
-        max_nt=eff["nt"],
+        max_nt=eff["nt"], window_hash=window_hash, window_keys=window_keys,
     )
 
 
@@ -274,6 +315,7 @@ def _fields(g: Grammar, d: dict) -> list:
         ("terminal_names_lex", "u8", _names(g.lex_only.names())),
         ("terminal_names_both", "u8", _names(g.lex_both.names())),
         ("production_names", "u8", _names(g.parser.prod_names if g.has_parser else [])),
+        ("window_hash", "int", d["window_hash"]), ("window_keys", "u64", one(d["window_keys"])),
     ]
 
 
@@ -282,7 +324,7 @@ def _names(names) -> list:
     return list(b"".join(n.encode() + b"\0" for n in names)) or [0]
 
 
-_CTYPE = {"u8": "uint8_t", "u16": "uint16_t", "i32": "int32_t"}
+_CTYPE = {"u8": "uint8_t", "u16": "uint16_t", "i32": "int32_t", "u64": "uint64_t"}
 
 
 def emit(g: Grammar) -> str:
@@ -311,15 +353,18 @@ def emit(g: Grammar) -> str:
 TABLES_MAGIC = b"ALPHIPT1"
 
 
-def emit_tables(g: Grammar) -> bytes:
+def emit_tables(g: Grammar, hashed=None, name=None) -> bytes:
     """The same descriptor as a binary tables file that the library loads at
     run time (alpacc_hip_load_grammar): the magic, a u32 field count, then
     every field in order -- a string as u32 length + bytes, a scalar as
     tag 'I' + i64, an array as tag 'A' + u32 entry size + u64 count + the
-    entries; each field padded to 8 bytes."""
+    entries; each field padded to 8 bytes.  hashed: as device_tables; name:
+    the grammar's name in the file (default its own)."""
     import struct
-    d = device_tables(g)
+    d = device_tables(g, hashed)
     fields = _fields(g, d)
+    if name is not None:
+        fields[0] = ("name", "str", name)
     out = bytearray(TABLES_MAGIC)
     out += struct.pack("<I", len(fields))
 
@@ -334,7 +379,7 @@ def emit_tables(g: Grammar) -> bytes:
         elif kind == "int":
             out += b"I" + b"\0" * 7 + struct.pack("<q", int(v))
         else:
-            fmt, size = {"u8": ("B", 1), "u16": ("H", 2), "i32": ("i", 4)}[kind]
+            fmt, size = {"u8": ("B", 1), "u16": ("H", 2), "i32": ("i", 4), "u64": ("Q", 8)}[kind]
             out += b"A" + b"\0" * 3 + struct.pack("<IQ", size, len(v)) + struct.pack(f"<{len(v)}{fmt}", *v)
         pad()
     return bytes(out)

@@ -159,6 +159,33 @@ def test_shaped_grammar_bracket_widths(name):
             be.set_graphs(False)
 
 
+@pytest.mark.parametrize("name", ["json", "arithmetic", "sexp"])
+def test_hashed_window_table(name, tmp_path):
+    """A grammar whose windows are looked up in the open-addressing table
+    (the form grammars take when their dense window index would be too
+    large), loaded under another name: every record equals the oracle's for
+    the grammar itself, in the parser and combined modes, eager and through
+    the graph, on the test-vector batches and on generated inputs."""
+    from alpacc_amd import hip
+    g = grammar_of(name)
+    alias = name + "_hashed"
+    f = tmp_path / (alias + ".alphip")
+    f.write_bytes(emit_hip.emit_tables(g, hashed=True, name=alias))
+    assert hip.load_grammar(str(f)) == alias
+    gen = [datagen.generate(name, n, seed=40 + n % 7, conforming=True).tobytes() for n in (1, 100, 5000, 70000)]
+    with hip.Backend(alias) as be:
+        for mode in ("parser", "both"):
+            for kind, items in batches(g, mode) + ([("generated", gen)] if mode == "both" else []):
+                print(f"case {alias}/{mode}/{kind}: {len(items)} inputs", flush=True)
+                exp = b"".join(pyoracle.record(name, mode, x) for x in items)
+                assert be.run_batch(mode, items) == exp, (alias, mode, kind)
+                if mode == "both":
+                    be.set_graphs(True)
+                    for rep in range(2):
+                        assert be.run_batch(mode, items) == exp, (alias, kind, "graph", rep)
+                    be.set_graphs(False)
+
+
 def test_lexer_only_grammar_refuses_parser_modes():
     from alpacc_amd import hip
     with hip.Backend("lexer") as be:

@@ -236,3 +236,90 @@ def test_cli_bogus_frame_length():
     r = subprocess.run([CLI, "-g", "arithmetic"], input=data, capture_output=True)
     assert r.returncode == 1 and r.stderr.decode().strip() == "error: truncated frame"
     assert r.stdout == struct.pack("<Q", 1)
+
+
+# ------------------------------------------------------- hashed window tables
+
+def probe(d, code):
+    """The pair id a hashed window table gives a window code (the parser's
+    probe: first slot from the code's Fibonacci hash, then linear)."""
+    bits = d["window_hash"]
+    slot, mask = emit_hip.window_slot(code, bits), (1 << bits) - 1
+    while True:
+        k = d["window_keys"][slot]
+        if k == code:
+            return d["window"][slot]
+        if k == emit_hip.EMPTY_KEY:
+            return 0xFFFF
+        slot = (slot + 1) & mask
+
+
+@pytest.mark.parametrize("name", ["json", "arithmetic", "sexp", "paper_grammar", "llp01", "list"])
+def test_hashed_window_table_matches_dense_index(name):
+    """The open-addressing window table (grammars whose dense index would be
+    too large) gives every window code the pair id the dense index gives it,
+    including the windows without one; the pair records are the same, the
+    table is at most half full and at least one slot is empty."""
+    g = load(os.path.join(ROOT, "grammars", name + ".alp"))
+    dense, hashed = emit_hip.device_tables(g, hashed=False), emit_hip.device_tables(g, hashed=True)
+    assert dense["window_hash"] == 0 and hashed["window_hash"] >= 4
+    assert len(hashed["window"]) == len(hashed["window_keys"]) == 1 << hashed["window_hash"]
+    live = sum(k != emit_hip.EMPTY_KEY for k in hashed["window_keys"])
+    assert 0 < live <= len(hashed["window"]) // 2
+    assert live == sum(v != 0xFFFF for v in dense["window"])
+    assert hashed["pair_rec"] == dense["pair_rec"]
+    assert all(probe(hashed, c) == v for c, v in enumerate(dense["window"]))
+    # a code no window has (one past the last) finds nothing
+    assert probe(hashed, len(dense["window"])) == 0xFFFF
+
+
+def test_hashed_window_table_is_automatic_past_the_dense_limit(monkeypatch):
+    """Past MAX_WINDOW_ENTRIES the generator emits the hashed table instead
+    of refusing the grammar."""
+    g = load(os.path.join(ROOT, "grammars", "json.alp"))
+    monkeypatch.setattr(emit_hip, "MAX_WINDOW_ENTRIES", 15 ** 4 - 1)
+    assert emit_hip.device_tables(g)["window_hash"] > 0
+    monkeypatch.setattr(emit_hip, "MAX_WINDOW_ENTRIES", 15 ** 4)
+    assert emit_hip.device_tables(g)["window_hash"] == 0
+
+
+@needs_lib
+def test_hashed_tables_file_loads_and_checks(tmp_path):
+    """A hashed tables file loads under its own name; the loader refuses a
+    dense index shorter than the window codes and a hashed table whose slot
+    count is not 2^window_hash."""
+    from alpacc_amd import hip
+    g = load(os.path.join(ROOT, "grammars", "json.alp"))
+    f = tmp_path / "json_hashed_cpu.alphip"
+    f.write_bytes(emit_hip.emit_tables(g, hashed=True, name="json_hashed_cpu"))
+    assert hip.load_grammar(str(f)) == "json_hashed_cpu"
+    assert hip.names("json_hashed_cpu", "terminal") == g.lex_both.names()
+
+    def with_fields(d, name, **over):
+        fields = emit_hip._fields(g, d)
+        fields[0] = ("name", "str", name)
+        fields = [(n, k, over.get(n, v)) for n, k, v in fields]
+        blob = bytearray(emit_hip.TABLES_MAGIC) + struct.pack("<I", len(fields))
+        blob += b"\0" * (-len(blob) % 8)
+        for n, kind, v in fields:
+            if kind == "str":
+                blob += struct.pack("<I", len(v)) + v.encode()
+            elif kind == "int":
+                blob += b"I" + b"\0" * 7 + struct.pack("<q", int(v))
+            else:
+                fmt, size = {"u8": ("B", 1), "u16": ("H", 2), "i32": ("i", 4), "u64": ("Q", 8)}[kind]
+                blob += b"A" + b"\0" * 3 + struct.pack("<IQ", size, len(v)) + struct.pack(f"<{len(v)}{fmt}", *v)
+            blob += b"\0" * (-len(blob) % 8)
+        return bytes(blob)
+
+    dense = emit_hip.device_tables(g, hashed=False)
+    short = dense["window"][:-1]
+    bad = tmp_path / "short.alphip"
+    bad.write_bytes(with_fields(dense, "json_short", nwindow=len(short), window=short))
+    with pytest.raises(hip.HipError):
+        hip.load_grammar(str(bad))
+    hashed = emit_hip.device_tables(g, hashed=True)
+    bad2 = tmp_path / "slots.alphip"
+    bad2.write_bytes(with_fields(hashed, "json_slots", window_hash=hashed["window_hash"] + 1))
+    with pytest.raises(hip.HipError):
+        hip.load_grammar(str(bad2))
