@@ -592,6 +592,8 @@ struct Resolve {
   int ncap, tcap;      // staged nodes / terminal productions per tile
   uint16_t* ob_list;   // open pushes (bracket symbols)
   uint16_t* ub_list;   // unmatched pops (bracket symbols)
+  uint32_t* ub_tile;   // the tile of each unmatched pop (and of each unmatched slot below)
+  uint32_t* us_tile;
   int64_t* os_list;    // open child slots (owner's compact index, all inputs)
   int64_t* us_list;    // unmatched slots (output target)
   uint8_t* tree_prod;
@@ -892,7 +894,7 @@ __device__ __forceinline__ void resolve_tile(const Resolve& r, uint32_t tile, ui
     if (l >= 0)
       bad |= ball[(D - (int32_t)(int16_t)(w >> 16) - 1) * RWG + l] != sym;
     else
-      r.ub_list[ti.ub_off + (-D)] = sym;
+      r.ub_list[ti.ub_off + (-D)] = sym, r.ub_tile[ti.ub_off + (-D)] = tile;
   }
   // entry e sits at level lbm + 1 + e and stays open iff no later lane
   // falls below it: the first min(bsp, bsuf - lbm) entries
@@ -921,6 +923,7 @@ __device__ __forceinline__ void resolve_tile(const Resolve& r, uint32_t tile, ui
           s_par[idx] = pc;
       } else {
         r.us_list[ti.us_off + (-D)] = term ? (TOKEN_TARGET | (ti.t_off + idx)) : (c_off + idx);
+        r.us_tile[ti.us_off + (-D)] = tile;
       }
     }
     const int64_t sroom = (int64_t)ssuf - lsm;
@@ -1022,6 +1025,7 @@ __device__ int64_t tile_prev_less(const TileTree& t, uint64_t i, int32_t v) {
 struct Far {
   Seg s;
   const uint32_t* overflow;  // set: seg_resolve skipped tiles, the lists are not this launch's
+  const ScanOut* so;         // list lengths
   const TileInfo* info;
   const SegBase* sb;
   TileTree tb, ts;
@@ -1029,58 +1033,59 @@ struct Far {
   const uint16_t* ub_list;
   const int64_t* os_list;
   const int64_t* us_list;
+  const uint32_t* ub_tile;
+  const uint32_t* us_tile;
   int64_t* tree_par;
   int64_t* tok_par;
   int tree;
 };
 
-// One wave per tile: a tile has only a few pops left over.
-__device__ __forceinline__ void far_tile(const Far& f, uint32_t tile, uint32_t lane) {
-  const TileInfo ti = f.info[tile];
-  const uint32_t seg = f.s.tile_seg[tile];
-  const int64_t first = f.s.tile_first[seg];
-  const SegBase in = f.sb[seg];
-  const int64_t c_in = in.p_off - in.t_off;
-  bool bad = false, bad_root = false;
-  for (int u = lane; u < -ti.bmin; u += WAVE) {
-    const int32_t D = (int32_t)ti.bstart - u;  // depth before the pop
-    const uint16_t b = f.ub_list[ti.ub_off + u];
-    const int64_t t2 = tile_prev_less(f.tb, tile, D);
-    if (t2 < first) {
-      bad = true;
-      continue;
-    }
-    const TileInfo& o = f.info[t2];
-    const uint16_t push = f.ob_list[o.ob_off + (D - (int32_t)o.bstart - o.bmin - 1)];
-    bad |= push != b;
-  }
-  for (int u = lane; f.tree && u < -ti.smin; u += WAVE) {
-    const int32_t D = (int32_t)ti.sstart - u;  // open child slots before the pop
-    const int64_t target = f.us_list[ti.us_off + u];
-    const int64_t t2 = tile_prev_less(f.ts, tile, D);
-    int64_t pc;
-    if (t2 < first) {
-      if (D != (int32_t)in.sstart) bad_root = true;
-      pc = 0;  // the root is its own parent
-    } else {
-      const TileInfo& o = f.info[t2];
-      pc = f.os_list[o.os_off + (D - (int32_t)o.sstart - o.smin - 1)] - c_in;
-    }
-    if (target & TOKEN_TARGET)
-      f.tok_par[target & ~TOKEN_TARGET] = pc;
-    else
-      f.tree_par[target] = pc;
-  }
-  if (bad) atomicOr(f.s.flags + seg, BAD_MATCH);
-  if (bad_root) atomicOr(f.s.flags + seg, BAD_ROOT);
-}
-
-// One wave per tile, the grid striding over the launch's tiles.
+// One lane per pop left over at tile level, bracket pops then slot pops
+// (the lists hold each pop's tile): the pop at depth D meets the open push
+// of the last earlier tile of its input whose minimum is below D (32-ary
+// min tree), which must carry its bracket symbol, or whose open child slot
+// owns the production (the root slot, parent 0, when there is none).  A
+// tile has few pops on JSON and thousands under deep nesting; every pop is
+// its own lane either way.
 __global__ __launch_bounds__(256) void far_resolve(Far f) {
   if (*f.overflow) return;  // the lists were not written: the batch runs again
-  const uint32_t nt = *f.s.nt_dev, lane = threadIdx.x & (WAVE - 1);
-  for (uint32_t tile = blockIdx.x * (256 / WAVE) + threadIdx.x / WAVE; tile < nt; tile += gridDim.x * (256 / WAVE))
-    far_tile(f, tile, lane);
+  const int64_t nub = f.so->ub, nus = f.tree ? f.so->us : 0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < nub + nus; i += (int64_t)gridDim.x * 256) {
+    if (i < nub) {
+      const uint32_t tile = f.ub_tile[i];
+      const TileInfo& ti = f.info[tile];
+      const int32_t D = (int32_t)(ti.bstart - (i - ti.ub_off));  // depth before the pop
+      const uint32_t seg = f.s.tile_seg[tile];
+      const int64_t t2 = tile_prev_less(f.tb, tile, D);
+      bool bad = t2 < (int64_t)f.s.tile_first[seg];
+      if (!bad) {
+        const TileInfo& o = f.info[t2];
+        bad = f.ob_list[o.ob_off + (D - (int32_t)o.bstart - o.bmin - 1)] != f.ub_list[i];
+      }
+      if (bad) atomicOr(f.s.flags + seg, BAD_MATCH);
+    } else {
+      const int64_t j = i - nub;
+      const uint32_t tile = f.us_tile[j];
+      const TileInfo& ti = f.info[tile];
+      const int32_t D = (int32_t)(ti.sstart - (j - ti.us_off));  // open child slots before the pop
+      const uint32_t seg = f.s.tile_seg[tile];
+      const int64_t target = f.us_list[j];
+      const int64_t t2 = tile_prev_less(f.ts, tile, D);
+      const SegBase& in = f.sb[seg];
+      int64_t pc;
+      if (t2 < (int64_t)f.s.tile_first[seg]) {
+        if (D != (int32_t)in.sstart) atomicOr(f.s.flags + seg, BAD_ROOT);
+        pc = 0;  // the root is its own parent
+      } else {
+        const TileInfo& o = f.info[t2];
+        pc = f.os_list[o.os_off + (D - (int32_t)o.sstart - o.smin - 1)] - (in.p_off - in.t_off);
+      }
+      if (target & TOKEN_TARGET)
+        f.tok_par[target & ~TOKEN_TARGET] = pc;
+      else
+        f.tree_par[target] = pc;
+    }
+  }
 }
 
 __global__ __launch_bounds__(256) void tiles_min_level(const int32_t* in, uint64_t n_in, int32_t* out, uint64_t n_out) {
@@ -1407,9 +1412,8 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
   const bool stride = caps && ntiles > (1u << 20);
   const uint32_t g_prof = (uint32_t)(stride ? 4096 : (ntiles + AWAVES - 1) / AWAVES);
   const uint32_t g_res = (uint32_t)(stride ? 2048 : ntiles);
-  // far_resolve strides: 2048 workgroups of 4 tile-waves measured faster
-  // than a wave per tile (0.136 against 0.143 ms on 1 GiB of JSON)
-  const uint32_t g_far = (uint32_t)((ntiles + 3) / 4 > 2048 ? 2048 : (ntiles + 3) / 4);
+  // far_resolve: one lane per leftover pop, at most 4096 workgroups striding (the
+  // grid is set from the lists' sizes below)
   if (ntiles) {
     hipLaunchKernelGGL(pseg_map, dim3(g_map), dim3(256), 0, st, d_tf, s.count, s.nt_dev,
                        const_cast<uint32_t*>(s.tile_seg));
@@ -1445,7 +1449,11 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
                 so.us > 0x7FFFFFFF))
     return fail(ALPACC_ERR_LIMIT, "derivation or bracket lists beyond 2^31 entries");
   const uint64_t nodes = caps ? np : np - nterm;
-  if ((r = c.ps_b.reserve((size_t)(so.ob + so.ub) * 2 + 64)) || (r = c.ps_c.reserve((size_t)(so.os + so.us) * 8 + 64)) ||
+  // ps_b: bracket lists (u16), then the tile of every unmatched pop and slot
+  // (u32, 4-byte aligned)
+  const size_t off_ptile = ((size_t)(so.ob + so.ub) * 2 + 63) & ~size_t(63);
+  if ((r = c.ps_b.reserve(off_ptile + (size_t)(so.ub + so.us) * 4 + 64)) ||
+      (r = c.ps_c.reserve((size_t)(so.os + so.us) * 8 + 64)) ||
       (tree && ((r = c.tr_prod.reserve(nodes + 16)) || (r = c.tr_parent.reserve(nodes * 8 + 16)) ||
                 (r = c.tr_tokpar.reserve(nterm * 8 + 16)))) ||
       (!tree && (r = c.ps_d.reserve(np + 16))))
@@ -1454,6 +1462,8 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
   uint16_t* ub_list = ob_list + so.ob;
   int64_t* os_list = c.ps_c.as<int64_t>();
   int64_t* us_list = os_list + so.os;
+  uint32_t* ub_tile = reinterpret_cast<uint32_t*>(c.ps_b.as<uint8_t>() + off_ptile);
+  uint32_t* us_tile = ub_tile + so.ub;
   if (ntiles) {
     TileTree tb{}, ts{};
     if (!caps) HIP_TRY(tm.begin(st, "tile_tree"));
@@ -1474,7 +1484,7 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
     const size_t lds_r = resolve_lds(n_st, t_st, br8 ? 1 : 2);
     Resolve rs{s,         reinterpret_cast<const uint16_t*>(base + off_pid),
                info,      sb,       bcap,  scap, n_st, t_st,
-               ob_list,   ub_list,   os_list,  us_list,       c.tr_prod.as<uint8_t>(),
+               ob_list,   ub_list,   ub_tile,  us_tile,  os_list,  us_list,       c.tr_prod.as<uint8_t>(),
                c.tr_parent.as<int64_t>(), c.tr_tokpar.as<int64_t>(), tree ? nullptr : c.ps_d.as<uint8_t>(),
                tree ? 1 : 0, &sout->overflow};
     void (*kern[2])(Resolve);  // one tile per workgroup, striding
@@ -1488,9 +1498,12 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
     if (!caps) HIP_TRY(tm.begin(st, "seg_resolve"));
     hipLaunchKernelGGL(kern[stride ? 1 : 0], dim3(g_res), dim3(RWG), lds_r, st, rs);
     if (!caps) HIP_TRY(tm.end(st));
-    Far fr{s, &sout->overflow, info, sb, tb, ts, ob_list, ub_list, os_list, us_list, c.tr_parent.as<int64_t>(),
+    Far fr{s, &sout->overflow, sout, info, sb, tb, ts, ob_list, ub_list, os_list, us_list, ub_tile, us_tile,
+           c.tr_parent.as<int64_t>(),
            c.tr_tokpar.as<int64_t>(), tree ? 1 : 0};
     if (!caps) HIP_TRY(tm.begin(st, "far_resolve"));
+    const uint64_t pops = (uint64_t)(so.ub + (tree ? so.us : 0));
+    const uint32_t g_far = (uint32_t)std::min<uint64_t>(4096, std::max<uint64_t>(1, (pops + 255) / 256));
     hipLaunchKernelGGL(far_resolve, dim3(g_far), dim3(256), 0, st, fr);
     if (!caps) HIP_TRY(tm.end(st));
   }
