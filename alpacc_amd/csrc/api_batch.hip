@@ -62,7 +62,7 @@ int lex_parse_batch_graph(Context& c, const uint8_t* d_in, const uint64_t* off, 
   for (uint64_t i = 0; i < count; ++i)
     if (off[i + 1] < off[i]) return fail(ALPACC_ERR_ARG, "offsets decrease at input %llu", (unsigned long long)i);
   const uint64_t bytes = off[count];
-  BatchCaps caps{pow2_at_least(count, 16), pow2_at_least(bytes, 1 << 16), count};
+  BatchCaps caps{pow2_at_least(count, 16), pow2_at_least(bytes, 1 << 16), count, c.cap_tokens};
   if (c.graph) {  // a larger capacity already captured serves this batch too
     caps.count = caps.count > c.graph_caps.count ? caps.count : c.graph_caps.count;
     caps.bytes = caps.bytes > c.graph_caps.bytes ? caps.bytes : c.graph_caps.bytes;
@@ -82,6 +82,7 @@ int lex_parse_batch_graph(Context& c, const uint8_t* d_in, const uint64_t* off, 
   c.timer.on = false;  // no event pairs inside a graph
   int r = 0;
   const bool reuse = c.graph && caps.count == c.graph_caps.count && caps.bytes == c.graph_caps.bytes &&
+                     caps.tokens == c.graph_caps.tokens &&
                      c.graph_in == d_in && c.graph_stream == st && c.graph_gen == c.graph_buffers_gen();
   auto copy = [&](const Context::Copy& x) {
     return x.bytes ? hipMemcpyAsync(x.dst, x.src, x.bytes, x.kind, st) : hipSuccess;
@@ -115,6 +116,7 @@ int lex_parse_batch_graph(Context& c, const uint8_t* d_in, const uint64_t* off, 
         (void)hipGraphExecDestroy(c.graph);
         c.graph = nullptr;
         c.cap_stage_n = c.cap_stage_t = 0;
+        c.cap_tokens = 0;
         c.timer.on = timing;
         r = lex_batch_on_device(c, ALPACC_MODE_BOTH, d_in, off, count, st);
         if (r >= 0) r = parse_batch_on_device(c, c.lx_term.as<uint8_t>(), c.d_lb_tok, c.lb_tok, c.lb_flags, count, true, st);
@@ -137,6 +139,10 @@ int lex_parse_batch_graph(Context& c, const uint8_t* d_in, const uint64_t* off, 
       }
       return rr;
     };
+    // the eager run sizes the parser for the bytes (as many tokens as
+    // bytes), so every buffer the captured sequence uses, with the token
+    // capacity learned below, is already there when the capture starts
+    caps.tokens = 0;
     r = eager();
     if (r >= 0 && parse_scan_overflow(c, caps.count)) {
       c.cap_stage_n = c.cap_stage_t = 0;
@@ -149,6 +155,13 @@ int lex_parse_batch_graph(Context& c, const uint8_t* d_in, const uint64_t* off, 
       parse_scan_maxima(c, caps.count, true, &n, &t);
       c.cap_stage_n = (n + n / 4 + 64) & ~63;
       c.cap_stage_t = (t + t / 4 + 64) & ~63;
+      // and the parser is sized for half again this batch's tokens, to a
+      // power of two (the grids and workspaces of a bound by bytes are
+      // several times what a batch of JSON needs)
+      const uint64_t toks = c.lb_tok[count] - c.lb_tok[0];
+      c.cap_tokens = pow2_at_least(toks + toks / 2, 1 << 16);
+      caps.tokens = c.cap_tokens < caps.bytes ? c.cap_tokens : 0;
+      c.cap_tokens = caps.tokens;
     }
     if (r >= 0) {
       const uint64_t gen = c.graph_buffers_gen();

@@ -228,6 +228,7 @@ int alpacc_hip_set_parse_shape(alpacc_hip_ctx* h, const alpacc_hip_parse_shape* 
   c.parse_pt = shape->positions_per_tile;
   c.parse_apl = shape->positions_per_lane;
   c.cap_stage_n = c.cap_stage_t = 0;  // staging learned for other tiles
+  c.cap_tokens = 0;
   if (c.graph) {  // captured with the old kernels
     (void)hipGraphExecDestroy(c.graph);
     c.graph = nullptr;

@@ -1190,9 +1190,15 @@ __global__ void pseg_map(const uint32_t* tile_first, uint32_t count, const uint3
 // *nt_dev the total.  One workgroup of PLAN_WG lanes, PLAN_IPT inputs each
 // per round.
 constexpr int PLAN_WG = 1024, PLAN_IPT = 4;
+// A batch with more tokens than the sequence is sized for (tokens_cap, a
+// capacity learned from earlier batches; 0: none beyond the bytes) or more
+// tiles than ntiles_cap sets *overflow and leaves no tile: the batch runs
+// again eagerly.  (The token bound is the one that matters: positions size
+// the lists and the tree outputs.)
 __global__ __launch_bounds__(PLAN_WG) void parse_plan(const uint64_t* tok_off, const uint32_t* skip,
                                                       const uint32_t* live, uint32_t count, uint32_t PT,
-                                                      uint32_t* tile_first, uint32_t* nt_dev) {
+                                                      uint32_t* tile_first, uint32_t* nt_dev, uint32_t ntiles_cap,
+                                                      uint64_t tokens_cap, uint32_t* overflow) {
   __shared__ uint32_t s_w[PLAN_WG / WAVE];
   const uint32_t tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE, n_live = *live;
   uint32_t carry = 0;
@@ -1223,9 +1229,16 @@ __global__ __launch_bounds__(PLAN_WG) void parse_plan(const uint64_t* tok_off, c
     carry += tot;
     __syncthreads();
   }
+  // (carry is the same in every lane)
+  const bool fits = carry <= ntiles_cap && (tokens_cap == 0 || tok_off[count] - tok_off[0] <= tokens_cap);
+  if (!fits) {  // no input parsed
+    __syncthreads();
+    for (uint32_t i = tid; i < count; i += PLAN_WG) tile_first[i] = 0;
+  }
   if (tid == 0) {
-    tile_first[count] = carry;
-    *nt_dev = carry;
+    tile_first[count] = fits ? carry : 0u;
+    *nt_dev = fits ? carry : 0u;
+    if (!fits) *overflow = 1u;
   }
 }
 
@@ -1305,8 +1318,9 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
   uint64_t ntiles = 0, positions = 0;
   uint32_t* h_tf = nullptr;
   if (caps) {
-    positions = caps->bytes + 2 * count;
-    ntiles = (caps->bytes + count * (PT + 1)) / PT + 1;
+    const uint64_t tok = caps->tokens && caps->tokens < caps->bytes ? caps->tokens : caps->bytes;
+    positions = tok + 2 * count;
+    ntiles = (tok + count * (PT + 1)) / PT + 1;
     if ((r = c.h_pmeta.reserve(64))) return r;
   } else {
     for (uint64_t i = 0; i < count; ++i) {
@@ -1391,13 +1405,14 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
   ScanOut* sout = reinterpret_cast<ScanOut*>(base + off_out);
   ParseResult* res = reinterpret_cast<ParseResult*>(base + off_res);
 
+  if ((r = zero_words(s.flags, count * 4, st)) || (r = zero_words(sout, sizeof(ScanOut), st))) return r;
   if (caps) {
     hipLaunchKernelGGL(parse_plan, dim3(1), dim3(PLAN_WG), 0, st, d_tok_off, d_skip, d_live, s.count, (uint32_t)PT, d_tf,
-                       d_tf + count + 1);
+                       d_tf + count + 1, (uint32_t)ntiles,
+                       caps->tokens && caps->tokens < caps->bytes ? caps->tokens : 0ull, &sout->overflow);
   } else {
     HIP_TRY(hipMemcpyAsync(d_tf, h_tf, (count + 2) * 4, hipMemcpyHostToDevice, st));
   }
-  if ((r = zero_words(s.flags, count * 4, st)) || (r = zero_words(sout, sizeof(ScanOut), st))) return r;
   KernelTimer& tm = c.timer;
   const bool timed = c.timing && !caps;
   if (timed) HIP_TRY(hipEventRecord(c.ev[2], st));

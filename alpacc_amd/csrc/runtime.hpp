@@ -69,6 +69,9 @@ struct BatchLayout {
 // sequence ends.
 struct BatchCaps {
   uint64_t count, bytes, live;
+  // tokens the parser is sized for (0: at most bytes); a batch with more
+  // flags the launch's overflow word and runs again eagerly
+  uint64_t tokens = 0;
 };
 
 struct Context {
@@ -87,6 +90,9 @@ struct Context {
   // sequence sized for a capacity, learned from its eager run (0: the
   // grammar's maxima)
   int cap_stage_n = 0, cap_stage_t = 0;
+  // tokens such a sequence's parser is sized for, learned the same way (0:
+  // as many as bytes)
+  uint64_t cap_tokens = 0;
   int resolve_br8 = -1;              // 1: every bracket symbol below 256 (-1: not checked yet)
   int resolve_bw = 0;                // bracket stack bytes an entry (0: 1 when resolve_br8, else 2)
   // resident tables

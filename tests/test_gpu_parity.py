@@ -892,3 +892,25 @@ def test_graph_staging_overflow_falls_back():
             assert first_diff(got, exp) is None, k
         captures, replays = be.graph_stats()
         assert captures >= 2, (captures, replays)
+
+
+def test_graph_token_capacity_overflow_falls_back():
+    """The graph path sizes its parser for half again the tokens of the
+    batch it captured (to a power of two).  A later batch within the same
+    input and byte capacity with far more tokens (flat arrays of one-digit
+    numbers against one long string per input) overflows that capacity: the
+    device plan leaves no tile, the batch runs again eagerly, and the next
+    batch captures again.  Every record stays the oracle's."""
+    from alpacc_amd import hip
+    sparse = [b'"' + b"a" * (3000 + i) + b'"' for i in range(64)]
+    dense = [b"[" + b",".join(b"1" for _ in range(1500 + i // 2)) + b"]" for i in range(64)]
+    assert sum(map(len, sparse)) // 2 < sum(map(len, dense)) <= 2 * sum(map(len, sparse))
+    with hip.Backend("json") as be:
+        be.set_graphs(True)
+        for k, batch in enumerate([sparse, sparse, dense, dense, sparse]):
+            print(f"graph token capacity batch #{k}", flush=True)
+            got = be.run_batch("both", batch)
+            exp = b"".join(pyoracle.record("json", "both", x) for x in batch)
+            assert first_diff(got, exp) is None, k
+        captures, replays = be.graph_stats()
+        assert captures >= 2 and replays >= 1, (captures, replays)
