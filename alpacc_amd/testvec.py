@@ -138,14 +138,38 @@ def terminal_texts(g: tables.Grammar) -> Dict[int, bytes]:
 
 # ---------------------------------------------------------------- the CFG
 
+def _heights(g: tables.Grammar) -> Dict[int, int]:
+    """The least height of a derivation tree from each nonterminal to a
+    terminal string (a fixpoint over the productions)."""
+    nterm = g.parser.end_t + 1
+    h: Dict[int, int] = {}
+    changed = True
+    while changed:
+        changed = False
+        for lhs, rhs in g.productions:
+            if all(x < nterm or x in h for x in rhs):
+                v = 1 + max([h[x] for x in rhs if x >= nterm], default=0)
+                if v < h.get(lhs, 1 << 30):
+                    h[lhs] = v
+                    changed = True
+    return h
+
+
 def derivation(g: tables.Grammar, rng: random.Random, budget: int = 400) -> Optional[List[int]]:
     """Terminal ids of a random leftmost derivation from the start symbol
-    (None when it outgrows its budget)."""
+    (None when it outgrows its budget).  Past the budget every nonterminal
+    takes an alternative of least height (then fewest symbols), so each
+    expansion lowers the height left and the derivation ends."""
     prods = g.productions
     nterm = g.parser.end_t + 1
+    height = _heights(g)
     by_lhs: Dict[int, list] = {}
     for lhs, rhs in prods:
         by_lhs.setdefault(lhs, []).append(rhs)
+
+    def rank(rhs):
+        return (max([height.get(x, 1 << 30) for x in rhs if x >= nterm], default=0), len(rhs))
+
     form, out = [prods[-1][1][1]], []
     while form:
         s = form.pop(0)
@@ -155,7 +179,7 @@ def derivation(g: tables.Grammar, rng: random.Random, budget: int = 400) -> Opti
         alts = by_lhs[s]
         budget -= 1
         if budget < 0:
-            alts = sorted(alts, key=len)[:1]
+            alts = [min(alts, key=rank)]
         form = list(rng.choice(alts)) + form
         if len(form) > 4 * 400:
             return None
@@ -182,9 +206,13 @@ def generate(g: tables.Grammar, mode: str, exhaustive: int = 0, long: int = 0, d
     texts = terminal_texts(g)
     ign = g.lex_both.ignore_id
     sep = texts.get(ign, b"") if ign is not None else b""
+    misses = 0
     while len(out) < derivations:
         toks = derivation(g, rng)
         if toks is None:
+            misses += 1
+            if misses > 100 * derivations + 1000:
+                raise ValueError(f"{g.name}: random derivations keep outgrowing their budget")
             continue
         if mode == "parser":
             item = bytes(toks)

@@ -58,6 +58,46 @@ def test_derivations_parse(grammar):
         assert all(r.valid for r in recs[0::2])
 
 
+# A random grammar (alpacc_amd.gen.randgrammar) whose fewest-symbols
+# alternative of C ("d h D B") recurses through B: a derivation that takes
+# the shortest alternative past its budget grows without end.
+RECURSIVE_SHORTEST = """params {
+  lookback = 2.
+  lookahead = 1.
+}
+
+a = /c/.
+b = /c/.
+c = /[beeacebcd]/.
+d = /([ceeeb]|(([ae] e) [ebc]))/.
+e = /c/.
+f = /c/.
+g = /([dbdbb]|([bdb] ()))/.
+h = /a/.
+
+B -> f b A.
+A -> c C.
+C -> e b g d a.
+D -> b e.
+C -> d h D B.
+C -> g B C D B.
+"""
+
+
+def test_derivations_end_when_the_shortest_alternative_recurses():
+    """Past its budget a derivation takes alternatives of least height, so it
+    ends (C -> e b g d a here) instead of growing until it is dropped, and
+    the batch generator finishes."""
+    import random
+    from alpacc_amd.gen import tables
+    g = tables.analyse("recursive_shortest", RECURSIVE_SHORTEST)
+    for seed in range(20):
+        toks = testvec.derivation(g, random.Random(seed), budget=5)
+        assert toks is not None and len(toks) < 1000
+    items = testvec.generate(g, "parser", derivations=60, seed=1)
+    assert len(items) == 60 and all(items[0::2])
+
+
 def test_cli_round_trip(tmp_path):
     req, out = tmp_path / "req.bin", tmp_path / "out.bin"
     assert testvec.main(["generate", "json", "--mode", "both", "--derivations", "10", "-o", str(req)]) == 0
