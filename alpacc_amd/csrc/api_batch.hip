@@ -301,14 +301,15 @@ int batch_records(Context& c, int mode, const uint8_t* packed, const uint64_t* o
     HIP_TRY(hipMemcpyAsync(c.pmeta.ptr, h_off, (count + 1) * 8, hipMemcpyHostToDevice, st));
     if ((r = parse_batch_on_device(c, d_in, c.pmeta.as<uint64_t>(), h_off, nullptr, count, false, st)) < 0)
       return r;
+    const uint64_t pb = (uint64_t)c.g->production_bytes;
     uint64_t np = 0;
     size_t total = 0;
     for (uint64_t i = 0; i < count; ++i) {
       const ParseResult& o = c.pb_res[i];
       if (!o.flags && o.slot_off + o.slots > np) np = o.slot_off + o.slots;
-      total += o.flags ? 1 : 9 + o.slots;
+      total += o.flags ? 1 : 9 + o.slots * pb;
     }
-    Pull v[1] = {{c.ps_d.ptr, np}};
+    Pull v[1] = {{c.ps_d.ptr, np * pb}};
     if ((r = pull_all(c, v, 1, st))) return r;
     uint8_t* w = alloc(total);
     if (!w) return fail(ALPACC_ERR_NOMEM, "host allocation failed");
@@ -322,8 +323,8 @@ int batch_records(Context& c, int mode, const uint8_t* packed, const uint64_t* o
       *w++ = 1;
       memcpy(w, &o.slots, 8);
       w += 8;
-      memcpy(w, v[0].h + o.slot_off, o.slots);
-      w += o.slots;
+      memcpy(w, v[0].h + o.slot_off * pb, o.slots * pb);
+      w += o.slots * pb;
     }
     return 0;
   }

@@ -69,10 +69,11 @@ int parser_record(Context& c, uint64_t n, const uint8_t* payload, std::vector<ui
     rec.assign(1, 0);
     return r;
   }
-  rec.resize(9 + np);
+  const uint64_t pb = (uint64_t)c.g->production_bytes;
+  rec.resize(9 + np * pb);
   rec[0] = 1;
   memcpy(&rec[1], &np, 8);
-  if (np) HIP_TRY(hipMemcpyAsync(rec.data() + 9, c.ps_d.ptr, np, hipMemcpyDeviceToHost, st));
+  if (np) HIP_TRY(hipMemcpyAsync(rec.data() + 9, c.ps_d.ptr, np * pb, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   return ALPACC_VALID;
 }
@@ -167,7 +168,7 @@ int alpacc_hip_parse_test(alpacc_hip_ctx* h, const void* tokens, uint64_t n, voi
   r = parse_on_device(c, d_tok, n, false, c.stream, &np);
   if (r != ALPACC_VALID) return r;
   int err = 0;
-  void* hp = host_copy(c.ps_d.ptr, np, c.stream, &err);
+  void* hp = host_copy(c.ps_d.ptr, np * c.g->production_bytes, c.stream, &err);
   if (!hp) return err;
   HIP_TRY(hipStreamSynchronize(c.stream));
   *prods = hp;

@@ -37,7 +37,7 @@ struct GrammarDesc {
   int nstacks;
   const uint16_t* stacks;        // bracket: symbol | left << 15
   int nprods_flat;
-  const uint8_t* productions;    // production strings
+  const uint16_t* productions;   // production strings
   int nproductions;
   int production_bytes;
   const uint8_t* arity;          // [nproductions]
@@ -64,6 +64,10 @@ struct GrammarDesc {
   // code times 0x9E3779B97F4A7C15, top window_hash bits, then linear probing
   int window_hash;
   const uint64_t* window_keys;   // [nwindow] when hashed, else one unused entry
+  // production_bytes 2 (more than 255 productions): the productions of
+  // every pair record as u16 ids, 8 a pair (the records hold their low
+  // bytes); else one unused entry
+  const uint16_t* pair_prod16;
 };
 
 // A grammar by name: one loaded at run time (alpacc_hip_load_grammar), else

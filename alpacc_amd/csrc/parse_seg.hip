@@ -602,6 +602,10 @@ struct Resolve {
   uint8_t* deriv;  // flat derivation (parser-only mode) or null
   int tree;
   uint32_t* overflow;  // ScanOut::overflow of the launch
+  // two-byte production ids (more than 255 productions): 8 u16 a pair
+  // record, 16 bytes; productions are then staged and written as u16.
+  // Null: the records' own bytes, written as u8.
+  const uint4* prod16;
 };
 
 // Last lane l <= from whose minimum (tb[0][l]) is below L, or -1.  tb[k][l]
@@ -662,7 +666,7 @@ __device__ __forceinline__ void block_scan4(int32_t* v, int32_t* tot, int32_t (*
 // consecutive levels just above its minimum, so a pop left over by a later
 // lane at tile-relative level D reads entry D - min - 1 of the last earlier
 // lane whose minimum is below D (a sparse table over the lanes' minima).
-template <int MBPOP, int MBPUSH, int MNP, int MSPUSH, typename BT, int PT, int APL>
+template <int MBPOP, int MBPUSH, int MNP, int MSPUSH, typename BT, int PT, int APL, bool P16>
 __device__ __forceinline__ void resolve_tile(const Resolve& r, uint32_t tile, uint32_t tid) {
   constexpr int RWG = PT / APL;    // lanes per tile
   constexpr int RNW = RWG / WAVE;  // waves per tile
@@ -686,6 +690,7 @@ __device__ __forceinline__ void resolve_tile(const Resolve& r, uint32_t tile, ui
   uint16_t* const s_par = reinterpret_cast<uint16_t*>(sall + (scap * RWG + 1) / 2 * 2);
   uint16_t* const s_tpar = s_par + r.ncap;
   uint8_t* const s_prod = reinterpret_cast<uint8_t*>(s_tpar + r.tcap);
+  uint16_t* const s_prod16 = s_tpar + r.tcap;  // (two-byte ids: the same room, twice as long)
   const uint32_t seg = s.tile_seg[tile];
   const TileInfo ti = r.info[tile];
   const SegBase in = r.sb[seg];
@@ -754,6 +759,8 @@ __device__ __forceinline__ void resolve_tile(const Resolve& r, uint32_t tile, ui
     const uint4 r1 = q[1], r2 = q[2], r3 = q[3];
     const uint4 r0 = h[k];
     const uint32_t hx = r0.x, hy = r0.y;
+    uint4 p16{0, 0, 0, 0};
+    if (P16) p16 = r.prod16[ids[k] == NO_PAIR ? (uint32_t)s.npairs : ids[k]];
     const int nbpop = hx & 0xFF, nbpush = (hx >> 8) & 0xFF, npk = (hx >> 16) & 0xFF, ntk = hx >> 24;
     const int nspush = (hy >> 8) & 0xFF;
     const uint32_t tmask = (hy >> 16) & 0xFF;
@@ -787,12 +794,18 @@ __device__ __forceinline__ void resolve_tile(const Resolve& r, uint32_t tile, ui
 #pragma unroll
     for (int j = 0; j < MNP; ++j) {
       if (j < npk) {
-        const uint32_t pr = rec_prod(r1, j);
+        const uint32_t w16 = j < 2 ? p16.x : j < 4 ? p16.y : j < 6 ? p16.z : p16.w;
+        const uint32_t pr = P16 ? (w16 >> (16 * (j & 1))) & 0xFFFFu : rec_prod(r1, j);
         const bool term = (tmask >> j) & 1u;
         const int32_t c = cbk + j - tseen;  // node (tile-local compact index)
         const int32_t tl = tinc + tseen;    // terminal production (tile-local)
         tseen += term;
-        if (!r.tree) {
+        if (P16) {
+          if (!r.tree)
+            s_prod16[slot + j] = (uint16_t)pr;
+          else if (!term)
+            s_prod16[c] = (uint16_t)pr;
+        } else if (!r.tree) {
           s_prod[slot + j] = (uint8_t)pr;
         } else if (!term) {
           s_prod[c] = (uint8_t)pr;
@@ -942,47 +955,56 @@ __device__ __forceinline__ void resolve_tile(const Resolve& r, uint32_t tile, ui
     const int32_t nn = tile_np - tile_nt;
 #pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
     for (int32_t i = tid; i < nn; i += RWG) {
-      r.tree_prod[c_off + i] = s_prod[i];
+      if (P16)
+        reinterpret_cast<uint16_t*>(r.tree_prod)[c_off + i] = s_prod16[i];
+      else
+        r.tree_prod[c_off + i] = s_prod[i];
       r.tree_par[c_off + i] = c_off + s_par[i] - c_in;
     }
 #pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
     for (int32_t i = tid; i < tile_nt; i += RWG) r.tok_par[ti.t_off + i] = c_off + s_tpar[i] - c_in;
   } else {
 #pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
-    for (int32_t i = tid; i < tile_np; i += RWG) r.deriv[ti.p_off + i] = s_prod[i];
+    for (int32_t i = tid; i < tile_np; i += RWG)
+      if (P16)
+        reinterpret_cast<uint16_t*>(r.deriv)[ti.p_off + i] = s_prod16[i];
+      else
+        r.deriv[ti.p_off + i] = s_prod[i];
   }
 }
 
 // One tile per workgroup, as seg_profile.
-template <int MBPOP, int MBPUSH, int MNP, int MSPUSH, bool STRIDE, typename BT, int PT, int APL>
+// P16: two-byte production ids (their own kernels, so that the one-byte
+// grammars keep the registers of the one-byte path).
+template <int MBPOP, int MBPUSH, int MNP, int MSPUSH, bool STRIDE, typename BT, int PT, int APL, bool P16>
 __global__ __launch_bounds__(PT / APL) void seg_resolve(Resolve r) {
   const uint32_t nt = *r.s.nt_dev;
   if (!STRIDE) {
-    if (blockIdx.x < nt) resolve_tile<MBPOP, MBPUSH, MNP, MSPUSH, BT, PT, APL>(r, blockIdx.x, threadIdx.x);
+    if (blockIdx.x < nt) resolve_tile<MBPOP, MBPUSH, MNP, MSPUSH, BT, PT, APL, P16>(r, blockIdx.x, threadIdx.x);
     return;
   }
   for (uint32_t tile = blockIdx.x; tile < nt; tile += gridDim.x) {
-    resolve_tile<MBPOP, MBPUSH, MNP, MSPUSH, BT, PT, APL>(r, tile, tid_per_tile());
+    resolve_tile<MBPOP, MBPUSH, MNP, MSPUSH, BT, PT, APL, P16>(r, tile, tid_per_tile());
     __syncthreads();
   }
 }
 
 // The single-tile and striding kernels of one instantiation and tile shape,
 // bracket stacks u8 (br8) or u16.
-template <int MBPOP, int MBPUSH, int MNP, int MSPUSH, int PT, int APL>
+template <int MBPOP, int MBPUSH, int MNP, int MSPUSH, int PT, int APL, bool P16>
 void resolve_kernels_shape(bool br8, void (**kern)(Resolve)) {
-  kern[0] = br8 ? seg_resolve<MBPOP, MBPUSH, MNP, MSPUSH, false, uint8_t, PT, APL>
-                : seg_resolve<MBPOP, MBPUSH, MNP, MSPUSH, false, uint16_t, PT, APL>;
-  kern[1] = br8 ? seg_resolve<MBPOP, MBPUSH, MNP, MSPUSH, true, uint8_t, PT, APL>
-                : seg_resolve<MBPOP, MBPUSH, MNP, MSPUSH, true, uint16_t, PT, APL>;
+  kern[0] = br8 ? seg_resolve<MBPOP, MBPUSH, MNP, MSPUSH, false, uint8_t, PT, APL, P16>
+                : seg_resolve<MBPOP, MBPUSH, MNP, MSPUSH, false, uint16_t, PT, APL, P16>;
+  kern[1] = br8 ? seg_resolve<MBPOP, MBPUSH, MNP, MSPUSH, true, uint8_t, PT, APL, P16>
+                : seg_resolve<MBPOP, MBPUSH, MNP, MSPUSH, true, uint16_t, PT, APL, P16>;
 }
-template <int MBPOP, int MBPUSH, int MNP, int MSPUSH>
+template <int MBPOP, int MBPUSH, int MNP, int MSPUSH, bool P16 = false>
 void resolve_kernels(bool br8, int pt, int apl, void (**kern)(Resolve)) {
-  if (pt == 512) resolve_kernels_shape<MBPOP, MBPUSH, MNP, MSPUSH, 512, 4>(br8, kern);
-  else if (pt == 2048) resolve_kernels_shape<MBPOP, MBPUSH, MNP, MSPUSH, 2048, 4>(br8, kern);
-  else if (apl == 2) resolve_kernels_shape<MBPOP, MBPUSH, MNP, MSPUSH, 1024, 2>(br8, kern);
-  else if (apl == 8) resolve_kernels_shape<MBPOP, MBPUSH, MNP, MSPUSH, 1024, 8>(br8, kern);
-  else resolve_kernels_shape<MBPOP, MBPUSH, MNP, MSPUSH, 1024, 4>(br8, kern);
+  if (pt == 512) resolve_kernels_shape<MBPOP, MBPUSH, MNP, MSPUSH, 512, 4, P16>(br8, kern);
+  else if (pt == 2048) resolve_kernels_shape<MBPOP, MBPUSH, MNP, MSPUSH, 2048, 4, P16>(br8, kern);
+  else if (apl == 2) resolve_kernels_shape<MBPOP, MBPUSH, MNP, MSPUSH, 1024, 2, P16>(br8, kern);
+  else if (apl == 8) resolve_kernels_shape<MBPOP, MBPUSH, MNP, MSPUSH, 1024, 8, P16>(br8, kern);
+  else resolve_kernels_shape<MBPOP, MBPUSH, MNP, MSPUSH, 1024, 4, P16>(br8, kern);
 }
 
 // ---------------------------------------------------------------- pass C
@@ -1288,8 +1310,11 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
   if (!parse_shape_ok(PT, APL)) return fail(ALPACC_ERR_ARG, "parser tile shape %d / %d", PT, APL);
   const int ncap = PT * g->max_prods, tcap = PT * g->max_nt;
   // the resolve kernel's instantiation: window maxima it is compiled for
+  // (two-byte production ids: the widest only)
   int ib[4];
-  if (g->max_bpop <= 2 && g->max_bpush <= 3 && g->max_prods <= 4 && g->max_spush <= 3)
+  if (g->production_bytes == 2)
+    ib[0] = 4, ib[1] = 4, ib[2] = 8, ib[3] = 24;
+  else if (g->max_bpop <= 2 && g->max_bpush <= 3 && g->max_prods <= 4 && g->max_spush <= 3)
     ib[0] = 2, ib[1] = 3, ib[2] = 4, ib[3] = 3;
   else if (g->max_bpop <= 3 && g->max_bpush <= 4 && g->max_prods <= 4 && g->max_spush <= 4)
     ib[0] = 3, ib[1] = 4, ib[2] = 4, ib[3] = 4;
@@ -1305,7 +1330,8 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
   // seg_resolve's dynamic LDS with node / terminal staging for n / t entries
   // (bracket stacks of bw bytes an entry)
   auto resolve_lds = [&](int n, int t, int bw) {
-    return (size_t)bcap * RWG * bw + ((size_t)scap * RWG + 1) / 2 * 2 + (size_t)(n + t) * 2 + (size_t)n + 16;
+    return (size_t)bcap * RWG * bw + ((size_t)scap * RWG + 1) / 2 * 2 + (size_t)(n + t) * 2 +
+           (size_t)n * g->production_bytes + 16;
   };
   const size_t lds_b = resolve_lds(ncap, tcap, br8 ? 1 : 2);
   // tile-local depths are int16, compact indices u16, lane-relative slot
@@ -1469,9 +1495,9 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
   const size_t off_ptile = ((size_t)(so.ob + so.ub) * 2 + 63) & ~size_t(63);
   if ((r = c.ps_b.reserve(off_ptile + (size_t)(so.ub + so.us) * 4 + 64)) ||
       (r = c.ps_c.reserve((size_t)(so.os + so.us) * 8 + 64)) ||
-      (tree && ((r = c.tr_prod.reserve(nodes + 16)) || (r = c.tr_parent.reserve(nodes * 8 + 16)) ||
+      (tree && ((r = c.tr_prod.reserve(nodes * g->production_bytes + 16)) || (r = c.tr_parent.reserve(nodes * 8 + 16)) ||
                 (r = c.tr_tokpar.reserve(nterm * 8 + 16)))) ||
-      (!tree && (r = c.ps_d.reserve(np + 16))))
+      (!tree && (r = c.ps_d.reserve(np * g->production_bytes + 16))))
     return r;
   uint16_t* ob_list = c.ps_b.as<uint16_t>();
   uint16_t* ub_list = ob_list + so.ob;
@@ -1501,9 +1527,12 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
                info,      sb,       bcap,  scap, n_st, t_st,
                ob_list,   ub_list,   ub_tile,  us_tile,  os_list,  us_list,       c.tr_prod.as<uint8_t>(),
                c.tr_parent.as<int64_t>(), c.tr_tokpar.as<int64_t>(), tree ? nullptr : c.ps_d.as<uint8_t>(),
-               tree ? 1 : 0, &sout->overflow};
+               tree ? 1 : 0, &sout->overflow,
+               g->production_bytes == 2 ? reinterpret_cast<const uint4*>(c.d_prod16) : nullptr};
     void (*kern[2])(Resolve);  // one tile per workgroup, striding
-    if (ib[1] == 3)
+    if (g->production_bytes == 2)
+      resolve_kernels<4, 4, 8, 24, true>(br8, PT, APL, kern);
+    else if (ib[1] == 3)
       resolve_kernels<2, 3, 4, 3>(br8, PT, APL, kern);
     else if (narrow)
       resolve_kernels<3, 4, 4, 4>(br8, PT, APL, kern);

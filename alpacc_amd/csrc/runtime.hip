@@ -162,14 +162,9 @@ int alpacc_hip_open(const char* grammar, int device, alpacc_hip_ctx** out) {
       (r = upload(&c.d_delta_both, g->delta_both, (size_t)g->nstate * 256)) ||
       (r = upload(&c.d_token_lex, g->token_lex, (size_t)g->nstate)) ||
       (r = upload(&c.d_token_both, g->token_both, (size_t)g->nstate)) ||
-      (r = upload(&c.d_spans, g->spans, (size_t)g->npairs * 4)) ||
-      (r = upload(&c.d_profile, g->profile, (size_t)g->npairs * 8)) ||
       (r = upload(&c.d_pair_rec, g->pair_rec, (size_t)(g->npairs + 1) * 64)) ||
-      (r = upload(&c.d_prod_info, g->prod_info, (size_t)g->nproductions)) ||
-      (r = upload(&c.d_stacks, g->stacks, (size_t)g->nstacks)) ||
-      (r = upload(&c.d_productions, g->productions, (size_t)g->nprods_flat)) ||
-      (r = upload(&c.d_arity, g->arity, (size_t)g->nproductions)) ||
-      (r = upload(&c.d_isterm, g->prod_is_terminal, (size_t)g->nproductions)) || (r = c.small.reserve(4096))) {
+      (g->production_bytes == 2 && (r = upload(&c.d_prod16, g->pair_prod16, (size_t)(g->npairs + 1) * 8))) ||
+      (r = c.small.reserve(4096))) {
     alpacc_hip_close(h);
     return r;
   }
@@ -204,7 +199,7 @@ void alpacc_hip_close(alpacc_hip_ctx* h) {
   (void)hipSetDevice(c.device);
   if (c.graph) (void)hipGraphExecDestroy(c.graph);
   void* ptrs[] = {c.d_class, c.d_step, c.d_compose, c.d_endo_init, c.d_delta_lex, c.d_delta_both, c.d_token_lex, c.d_token_both, c.d_wrec, c.d_wkeys,
-                  c.d_spans, c.d_profile, c.d_pair_rec, c.d_prod_info, c.d_stacks, c.d_productions, c.d_arity, c.d_isterm};
+                  c.d_pair_rec, c.d_prod16};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (DevBuf* b : c.all_buffers()) b->release();

@@ -108,14 +108,8 @@ struct Context {
   // one entry per position)
   uint64_t* d_wrec = nullptr;
   uint64_t* d_wkeys = nullptr;  // hashed window tables: the code of each slot
-  int32_t* d_spans = nullptr;
-  uint8_t* d_profile = nullptr;
   uint8_t* d_pair_rec = nullptr;
-  uint8_t* d_prod_info = nullptr;
-  uint16_t* d_stacks = nullptr;
-  uint8_t* d_productions = nullptr;
-  uint8_t* d_arity = nullptr;
-  uint8_t* d_isterm = nullptr;
+  uint16_t* d_prod16 = nullptr;  // two-byte production ids of the pair records (else null)
   // workspaces
   DevBuf small;                       // counters, flags, tickets
   DevBuf lx_term, lx_start, lx_len;   // lexer outputs

@@ -65,12 +65,12 @@ const Field FIELDS[] = {
     FI(terminal_bytes_lex), FI(terminal_bytes_both), FI(index_bytes), FI(length_bytes),
     FI(empty_t), FI(start_t), FI(end_t), FI(window_base),
     FI(nwindow), FA(window, 2), FI(npairs), FA(spans, 4), FA(profile, 1),
-    FI(nstacks), FA(stacks, 2), FI(nprods_flat), FA(productions, 1),
+    FI(nstacks), FA(stacks, 2), FI(nprods_flat), FA(productions, 2),
     FI(nproductions), FI(production_bytes), FA(arity, 1), FA(prod_is_terminal, 1),
     FI(max_brackets), FI(max_prods), FI(max_slot_items), FA(pair_rec, 1), FA(prod_info, 1),
     FI(max_bpop), FI(max_bpush), FI(max_spop), FI(max_spush), FI(max_nt), FI(length_param), FI(has_parser),
     FA(terminal_names_lex, 1), FA(terminal_names_both, 1), FA(production_names, 1),
-    FI(window_hash), FA(window_keys, 8),
+    FI(window_hash), FA(window_keys, 8), FA(pair_prod16, 2),
 };
 #undef FI
 #undef FA
@@ -94,6 +94,7 @@ uint64_t need_entries(const GrammarDesc& g, size_t off) {
     return n(g.nproductions);
   if (off == offsetof(GrammarDesc, pair_rec)) return (uint64_t)(g.npairs + 1) * 64;
   if (off == offsetof(GrammarDesc, window_keys)) return g.window_hash ? n(g.nwindow) : 1;
+  if (off == offsetof(GrammarDesc, pair_prod16)) return g.production_bytes == 2 ? (uint64_t)(g.npairs + 1) * 8 : 1;
   return 0;
 }
 
@@ -165,6 +166,8 @@ int parse_grammar_tables(const uint8_t* data, size_t len, GrammarDesc* out, std:
     return fail(ALPACC_ERR_ARG, "tables file: sizes out of range");
   // the window table covers every code the parser forms: a dense index of
   // window_base^(q+k) entries, or a hashed table of 2^window_hash slots
+  if (g.production_bytes < 1 || g.production_bytes > (g.has_parser ? 2 : 1))
+    return fail(ALPACC_ERR_ARG, "tables file: %d-byte production ids", g.production_bytes);
   if (g.has_parser) {
     if (g.window_hash < 0 || g.window_hash > 30 || g.window_base < 1)
       return fail(ALPACC_ERR_ARG, "tables file: bad window table");

@@ -119,14 +119,14 @@ def _no_parser(g: Grammar) -> dict:
         window=[], base=g.lex_both.dead_id + 3, npairs=0, spans=[], stacks=[], productions=[], n_productions=0,
         production_bytes=1, arity=[], prod_is_term=[], profile=[], max_slot_items=1, pair_rec=[0] * 64,
         prod_info=[], max_brackets=0, max_prods=0, max_bpop=0, max_bpush=0, max_spop=0, max_spush=0, max_nt=0,
-        window_hash=0, window_keys=[],
+        window_hash=0, window_keys=[], pair_prod16=[],
     )
 
 
 def _parser_tables(g: Grammar, hashed=None) -> dict:
     p = g.parser
-    if p.production_bytes != 1:
-        raise EmitError(f"{g.name}: {p.n_productions} productions exceed the u8 production field")
+    if p.production_bytes > 2:
+        raise EmitError(f"{g.name}: {p.n_productions} productions exceed the u16 production field")
     base = p.end_t + 1
     w = p.q + p.k
     nwin = base ** w
@@ -190,6 +190,9 @@ def _parser_tables(g: Grammar, hashed=None) -> dict:
     # productions before it (nodes of the compact tree).
     rec = []
     bad_pairs = set()
+    # two-byte production ids (more than 255 productions): the records keep
+    # the low bytes, and a side table the whole ids, 8 a pair
+    prod16 = []
     eff = dict(bpop=0, bpush=0, spop=0, spush=0, nt=0)
     for i, (s0, s1, p0, p1) in enumerate(p.spans):
         st, ext = [], []
@@ -236,10 +239,14 @@ def _parser_tables(g: Grammar, hashed=None) -> dict:
         br = ext + [0] * (4 - len(ext)) + st + [0] * (4 - len(st))
         rec += [len(ext), len(st), len(prods), terms, spop, len(slots), tmask, 0]
         rec += [b for v in br for b in (v & 0xFF, v >> 8)]
-        rec += prods + [0] * (8 - len(prods))
+        rec += [x & 0xFF for x in prods] + [0] * (8 - len(prods))
+        prod16 += prods + [0] * (8 - len(prods))
         rec += parent + [0] * (8 - len(parent))
         rec += slots + [0] * (24 - len(slots))
     rec += [0] * 64  # record npairs: no effect (positions past an input's end)
+    prod16 += [0] * 8
+    if p.production_bytes == 1:
+        prod16 = []
     window_hash, window_keys = 0, []
     if hashed:
         # the windows with a pair id, at most half the slots filled
@@ -274,7 +281,7 @@ def _parser_tables(g: Grammar, hashed=None) -> dict:
         profile=profile, max_slot_items=max_slot_items, pair_rec=rec, prod_info=prod_info,
         max_brackets=p.max_brackets, max_prods=p.max_prods,
         max_bpop=eff["bpop"], max_bpush=eff["bpush"], max_spop=eff["spop"], max_spush=eff["spush"],
-        max_nt=eff["nt"], window_hash=window_hash, window_keys=window_keys,
+        max_nt=eff["nt"], window_hash=window_hash, window_keys=window_keys, pair_prod16=prod16,
     )
 
 
@@ -301,7 +308,7 @@ def _fields(g: Grammar, d: dict) -> list:
         ("nwindow", "int", len(d["window"])), ("window", "u16", one(d["window"])),
         ("npairs", "int", d["npairs"]), ("spans", "i32", one(d["spans"])), ("profile", "u8", one(d["profile"])),
         ("nstacks", "int", max(1, len(d["stacks"]))), ("stacks", "u16", one(d["stacks"])),
-        ("nprods_flat", "int", max(1, len(d["productions"]))), ("productions", "u8", one(d["productions"])),
+        ("nprods_flat", "int", max(1, len(d["productions"]))), ("productions", "u16", one(d["productions"])),
         ("nproductions", "int", d["n_productions"]), ("production_bytes", "int", d["production_bytes"]),
         ("arity", "u8", one(d["arity"])), ("prod_is_terminal", "u8", one(d["prod_is_term"])),
         ("max_brackets", "int", max(1, d["max_brackets"])), ("max_prods", "int", max(1, d["max_prods"])),
@@ -316,6 +323,7 @@ def _fields(g: Grammar, d: dict) -> list:
         ("terminal_names_both", "u8", _names(g.lex_both.names())),
         ("production_names", "u8", _names(g.parser.prod_names if g.has_parser else [])),
         ("window_hash", "int", d["window_hash"]), ("window_keys", "u64", one(d["window_keys"])),
+        ("pair_prod16", "u16", one(d["pair_prod16"])),
     ]
 
 

@@ -327,7 +327,8 @@ class Backend:
         ok = _check(self.lib.alpacc_hip_parse_test(self.h, tok.ctypes.data, len(tok), C.byref(prods), C.byref(npr)))
         if not ok:
             return False, None
-        return True, _take(prods.value, npr.value, np.uint8)
+        pb = self.layouts["both"].production
+        return True, _take(prods.value, npr.value * pb, f"<u{pb}")
 
     def lex_parse(self, data: bytes):
         """lex_string + parse_test + compute_parents + compact_tree."""
@@ -341,7 +342,7 @@ class Backend:
             terminals=_take(out.tokens.terminals, T * lay.terminal, np.uint8),
             starts=_take(out.tokens.starts, T * 8, "<i8"),
             lengths=_take(out.tokens.lengths, T * lay.length, f"<u{lay.length}"),
-            productions=_take(out.productions, N * lay.production, np.uint8),
+            productions=_take(out.productions, N * lay.production, f"<u{lay.production}"),
             parents=_take(out.parents, N * 8, "<i8"),
             token_parents=_take(out.token_parents, T * 8, "<i8"),
         )

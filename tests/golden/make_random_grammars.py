@@ -38,8 +38,8 @@ def accepted(name, text):
 
 # Grammars of a chosen shape, for the widest cases of the kernels: windows
 # with more than four productions (the resolve kernel's widest
-# instantiation), bracket symbols beyond 255 (two-byte bracket stacks), and
-# a lexer of 26 keywords beside identifiers whose byte-step table (1300
+# instantiation), bracket symbols beyond 255 (two-byte bracket stacks), 268
+# productions (two-byte production ids), and a lexer of 26 keywords beside identifiers whose byte-step table (1300
 # endomorphisms x 30 byte classes, 78,000 bytes) is too large for the fold's
 # 16-bit row offsets, so it takes the fold kernel with plain ids.
 SHAPED = {
@@ -66,6 +66,10 @@ W -> y S S z.
         ["params {", "  lookback = 1.", "  lookahead = 1.", "}", "", "c = /c/.", "ignore = /\\s+/.", "",
          "S -> N0."]
         + [f'N{i} -> "a{i}" {f"N{i + 1}" if i + 1 < 60 else "c"} "b{i}" | c.' for i in range(60)]) + "\n",
+    "many_productions": "\n".join(
+        ["params {", "  lookback = 1.", "  lookahead = 1.", "}", "", "c = /c/.", "ignore = /\\s+/.", "",
+         "S -> N0."]
+        + [f'N{i} -> "a{i}" {f"N{i + 1}" if i + 1 < 66 else "c"} "b{i}" | c.' for i in range(66)]) + "\n",
     "keywords": "".join(
         f"k{w} = /{w}/.\n"
         for w in ["if", "then", "else", "while", "do", "end", "return", "for", "function", "switch", "case",

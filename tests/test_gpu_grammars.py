@@ -12,7 +12,8 @@ known answers on the GPU.
   (2, 2), (3, 3), as the reference's differential suite runs them,
   Makefile:80-88; a grammar whose windows hold up to 7 productions, which
   takes the resolve kernel's widest instantiation; one with bracket symbols
-  up to 307, which needs two-byte bracket stacks): the test-vector tool's
+  up to 307, which needs two-byte bracket stacks; one of 268 productions,
+  whose ids take two bytes): the test-vector tool's
   exhaustive, DFA-walk and derivation batches in every mode the grammar has,
   one batched call each, every record byte-identical to the oracle's; the
   combined batches also through the captured graph, and every bracket width
@@ -134,7 +135,7 @@ def test_tables_file_grammar(name):
     check_grammar(name)
 
 
-@pytest.mark.parametrize("name", ["wide_windows", "many_symbols"])
+@pytest.mark.parametrize("name", ["wide_windows", "many_symbols", "many_productions"])
 def test_shaped_grammar_bracket_widths(name):
     """Every bracket stack width the grammar allows gives the oracle's
     records, eager and through the graph; one-byte stacks are refused when a
@@ -146,7 +147,7 @@ def test_shaped_grammar_bracket_widths(name):
     exp = b"".join(pyoracle.record(name, "both", x) for x in items)
     with hip.Backend(name) as be:
         for width in (0, 1, 2):
-            if name == "many_symbols" and width == 1:
+            if name in ("many_symbols", "many_productions") and width == 1:
                 with pytest.raises(hip.HipError):
                     be.set_bracket_width(1)
                 continue

@@ -845,7 +845,7 @@ def test_parser_shapes(shape):
     finally:
         for b in bes.values():
             b.close()
-    for name in ("wide_windows", "many_symbols"):
+    for name in ("wide_windows", "many_symbols", "many_productions"):
         TG.load_tables(name)
         g = TG.grammar_of(name)
         items = TG.testvec.generate(g, "both", derivations=200, seed=5)
