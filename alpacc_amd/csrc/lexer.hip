@@ -435,17 +435,17 @@ struct EmitWords {
   uint32_t wave_cnt[NW], wave_last[NW];
 };
 
-// LDS layout of lex_emit: DFA rows, the staged slots (start and end,
-// tile-relative; input; terminal), the lanes' kept-token terminals (rcap
-// rows of WG bytes and four sink rows), the workgroup words.
+// LDS layout of lex_emit: DFA rows, the staged slots (a word of start |
+// end << 16, both tile-relative, and a half-word of terminal | input << 8,
+// the input counted from the tile's first), the lanes' kept-token
+// terminals (rcap rows of WG bytes and four sink rows), the workgroup
+// words.
 struct EmitLayout {
-  uint32_t start, end, seg, term, lterm, words, total;
+  uint32_t se, ts, lterm, words, total;
   __host__ __device__ EmitLayout(uint32_t nstate, uint32_t rcap) {
-    start = (nstate * ROWB + 15u) & ~15u;
-    end = start + 2 * STAGE;
-    seg = end + 2 * STAGE;
-    term = seg + STAGE;
-    lterm = term + STAGE;
+    se = (nstate * ROWB + 15u) & ~15u;
+    ts = se + 4 * STAGE;
+    lterm = ts + 2 * STAGE;
     words = (lterm + (rcap + 4) * WG + 15u) & ~15u;
     total = words + (uint32_t)sizeof(EmitWords);
   }
@@ -458,11 +458,9 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
   const uint32_t rcap = p.rcap;
   const EmitLayout L((uint32_t)p.nstate, rcap);
   uint16_t* s_rows = reinterpret_cast<uint16_t*>(lds);              // [nstate][ROW]
-  uint16_t* s_start = reinterpret_cast<uint16_t*>(lds + L.start);  // [STAGE] start, tile-relative
-  uint16_t* s_end = reinterpret_cast<uint16_t*>(lds + L.end);       // [STAGE] end, tile-relative
-  uint8_t* s_seg = lds + L.seg;                                     // [STAGE] input, from the tile's first
-  uint8_t* s_term = lds + L.term;                                   // [STAGE] terminal
-  uint16_t* s_ichunk = reinterpret_cast<uint16_t*>(s_term);  // [WG] before the staging
+  uint32_t* s_se = reinterpret_cast<uint32_t*>(lds + L.se);  // [STAGE] start | end << 16
+  uint16_t* s_ts = reinterpret_cast<uint16_t*>(lds + L.ts);  // [STAGE] terminal | input << 8
+  uint16_t* s_ichunk = s_ts;                                  // [WG] before the staging
   // [WG] after walk 1, in its sink rows
   uint16_t* s_istart = reinterpret_cast<uint16_t*>(lds + L.lterm + rcap * WG);
   EmitWords& w = *reinterpret_cast<EmitWords*>(lds + L.words);
@@ -653,7 +651,7 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
     // byte lies in the same input.
     constexpr uint32_t NO_START = 0xFFFFu;
     const uint32_t start_in = elast ? elast - 1 - (uint32_t)tbase : NO_START;
-    const uint32_t dl = ls.seg - ts.s_lo;  // the lane's input, from the tile's first
+    const uint32_t dl8 = (ls.seg - ts.s_lo) << 8;  // the lane's input, from the tile's first
     uint64_t slot0 = 0;
     uint32_t carried = 0;
     bool overflow = false;
@@ -681,10 +679,8 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
               const uint32_t below = pm & ((1u << jb) - 1u);
               const uint32_t i = d + k;
               if (i < STAGE) {
-                s_term[i] = lds[L.lterm + k * WG + swz];
-                s_start[i] = (uint16_t)(below ? qb + 31 - __builtin_clz(below) : start);
-                s_end[i] = (uint16_t)(lane_off + 32 * q + jb);
-                s_seg[i] = (uint8_t)dl;
+                s_ts[i] = (uint16_t)(lds[L.lterm + k * WG + swz] | dl8);
+                s_se[i] = (below ? qb + 31 - __builtin_clz(below) : start) | (qb + jb) << 16;
               }
               ++k;
             }
@@ -705,10 +701,8 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
               if ((x >> RC::KEPT) & 1u) {
                 const uint32_t i = d + k;
                 if (i < STAGE) {
-                  s_term[i] = (uint8_t)lds_load16(r + 512);
-                  s_start[i] = (uint16_t)start;
-                  s_end[i] = (uint16_t)(lane_off + j);
-                  s_seg[i] = (uint8_t)dl;
+                  s_ts[i] = (uint16_t)((lds_load16(r + 512) & 0xFFu) | dl8);
+                  s_se[i] = start | (lane_off + j) << 16;
                 }
                 ++k;
               }
@@ -720,10 +714,8 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
         if (fin_kept) {
           const uint32_t i = d + k;
           if (i < STAGE) {
-            s_term[i] = (uint8_t)fin_t;
-            s_start[i] = (uint16_t)start;
-            s_end[i] = (uint16_t)(lane_off + valid);
-            s_seg[i] = (uint8_t)dl;
+            s_ts[i] = (uint16_t)((fin_t & 0xFFu) | dl8);
+            s_se[i] = start | (lane_off + valid) << 16;
           }
         }
       }
@@ -748,28 +740,28 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
                                    : lo_in + p.carry_start;
         for (uint32_t i = tid; i < m; i += WG) {
           const uint64_t slot = slot0 + lo + i;
-          const uint32_t b0 = s_start[i];
+          const uint32_t se = s_se[i], b0 = se & 0xFFFFu;
           const int64_t b = b0 == NO_START ? cs : (int64_t)tbase + b0;
-          const uint64_t len = (uint64_t)((int64_t)tbase + s_end[i] - b);
+          const uint64_t len = (uint64_t)((int64_t)tbase + (se >> 16) - b);
           overflow |= len > max_len;
-          p.terminals[slot] = s_term[i];
+          p.terminals[slot] = (uint8_t)s_ts[i];
           p.starts[slot] = b - lo_in + p.start_bias;
           store_length(p, slot, len);
         }
       } else {
         for (uint32_t i = tid; i < m; i += WG) {
           const uint64_t slot = slot0 + lo + i;
-          const uint32_t e = s_end[i];
+          const uint32_t se = s_se[i], e = se >> 16, tw = s_ts[i];
           // the token's input, staged with it
-          const uint32_t d = s_seg[i];
+          const uint32_t d = tw >> 8;
           const uint32_t seg = ts.s_lo + d;
           const uint64_t seg_lo = d ? tbase + s_istart[d] : ts.seg_lo;
-          const uint32_t b0 = s_start[i];
+          const uint32_t b0 = se & 0xFFFFu;
           uint64_t b = b0 == NO_START ? (carried ? carried - 1 : 0u) : tbase + b0;
           b = b > seg_lo ? b : seg_lo;
           const uint64_t len = tbase + e - b;
           if (len > max_len) atomicOr(p.seg_flags + seg, LEX_REJECT_OVERFLOW);
-          p.terminals[slot] = s_term[i];
+          p.terminals[slot] = (uint8_t)tw;
           p.starts[slot] = (int64_t)(b - seg_lo);
           store_length(p, slot, len);
         }
