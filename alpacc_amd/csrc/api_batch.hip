@@ -95,8 +95,10 @@ int lex_parse_batch_graph(Context& c, const uint8_t* d_in, const uint64_t* off, 
       hipError_t e = copy(gs.up);
       if (e == hipSuccess) e = hipGraphLaunch(c.graph, st);
       if (e == hipSuccess) e = copy(gs.lex_down);
-      if (e == hipSuccess) e = copy(gs.parse_down);
-      if (e == hipSuccess) e = copy(gs.scan_down);
+      // the scan's totals and the results of the live inputs, in one copy
+      Context::Copy pd = gs.parse_down;
+      pd.bytes -= (caps.count - count) * sizeof(ParseResult);
+      if (e == hipSuccess) e = copy(pd);
       if (e == hipSuccess) e = hipStreamSynchronize(st);
       if (e != hipSuccess) r = fail(ALPACC_ERR_HIP, "graph replay: %s", hipGetErrorString(e));
       c.graph_replays++;
@@ -109,7 +111,7 @@ int lex_parse_batch_graph(Context& c, const uint8_t* d_in, const uint64_t* off, 
       c.d_lb_live = gs.d_lb_live;
       c.pb_count = gs.pb_count;
       c.pb_res = gs.pb_res;
-      if (r >= 0 && parse_scan_overflow(c, caps.count)) {
+      if (r >= 0 && parse_scan_overflow(c)) {
         // a tile beyond the staging learned from earlier batches: this
         // batch runs again eagerly, and the next one captures again with
         // staging for the grammar's maxima
@@ -133,8 +135,7 @@ int lex_parse_batch_graph(Context& c, const uint8_t* d_in, const uint64_t* off, 
     auto eager = [&]() {
       int rr = lex_parse_enqueue_caps(c, d_in, padded.data(), caps, st);
       if (rr >= 0) {
-        hipError_t e = copy(c.cp_scan_down);
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        const hipError_t e = hipStreamSynchronize(st);
         if (e != hipSuccess) rr = fail(ALPACC_ERR_HIP, "batch: %s", hipGetErrorString(e));
       }
       return rr;
@@ -144,7 +145,7 @@ int lex_parse_batch_graph(Context& c, const uint8_t* d_in, const uint64_t* off, 
     // capacity learned below, is already there when the capture starts
     caps.tokens = 0;
     r = eager();
-    if (r >= 0 && parse_scan_overflow(c, caps.count)) {
+    if (r >= 0 && parse_scan_overflow(c)) {
       c.cap_stage_n = c.cap_stage_t = 0;
       r = eager();
     }
@@ -152,7 +153,7 @@ int lex_parse_batch_graph(Context& c, const uint8_t* d_in, const uint64_t* off, 
       // the captured sequence stages a quarter more than this batch's
       // largest tile needs (a later batch beyond it runs again eagerly)
       int n = 0, t = 0;
-      parse_scan_maxima(c, caps.count, true, &n, &t);
+      parse_scan_maxima(c, true, &n, &t);
       c.cap_stage_n = (n + n / 4 + 64) & ~63;
       c.cap_stage_t = (t + t / 4 + 64) & ~63;
       // and the parser is sized for half again this batch's tokens, to a
@@ -186,7 +187,6 @@ int lex_parse_batch_graph(Context& c, const uint8_t* d_in, const uint64_t* off, 
         gs.up = c.cp_meta_up;
         gs.lex_down = c.cp_lex_down;
         gs.parse_down = c.cp_parse_down;
-        gs.scan_down = c.cp_scan_down;
         gs.lb_count = c.lb_count;
         gs.lb_flags = c.lb_flags;
         gs.lb_tok = c.lb_tok;

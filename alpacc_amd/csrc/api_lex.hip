@@ -34,11 +34,12 @@ int lex_batch_on_device(Context& c, int mode, const uint8_t* d_in, const uint64_
   if (g->nendo > 0x8000) return fail(ALPACC_ERR_LIMIT, "%d lexer endomorphisms exceed 32768", g->nendo);
   // every input owns whole chunks of ipt bytes, at least one
   const uint64_t ipt = (uint64_t)c.lex_ipt;
+  const int ish = __builtin_ctzll(ipt);  // 64 or 128 bytes a chunk
   uint64_t nchunks = 0;
   for (uint64_t i = 0; i < count; ++i) {
     if (h_off[i + 1] < h_off[i]) return fail(ALPACC_ERR_ARG, "offsets decrease at input %llu", (unsigned long long)i);
     const uint64_t n = h_off[i + 1] - h_off[i];
-    nchunks += n ? (n + ipt - 1) / ipt : 1;
+    nchunks += n ? (n + ipt - 1) >> ish : 1;
   }
   if (nchunks > 0x7FFFFFFFull) return fail(ALPACC_ERR_LIMIT, "too many lexer chunks");
   const uint64_t nt_actual = (nchunks + LEX_WG - 1) / LEX_WG;
@@ -57,7 +58,7 @@ int lex_batch_on_device(Context& c, int mode, const uint8_t* d_in, const uint64_
     h_seg[i] = h_off[i];
     h_cf[i] = k;
     const uint64_t n = h_off[i + 1] - h_off[i];
-    k += n ? (uint32_t)((n + ipt - 1) / ipt) : 1u;
+    k += n ? (uint32_t)((n + ipt - 1) >> ish) : 1u;
   }
   h_seg[count] = h_off[count];
   h_cf[count] = k;
@@ -124,9 +125,7 @@ int lex_batch_on_device(Context& c, int mode, const uint8_t* d_in, const uint64_
   if (!c.lex_grid) HIP_TRY(lex_grid_size(p, c.lex_wpc, &c.lex_grid, &c.lex_grid2));
   c.cp_meta_up = Context::Copy{dm, hm, L.up_bytes, hipMemcpyHostToDevice};
   if (!c.capturing) HIP_TRY(hipMemcpyAsync(dm, hm, L.up_bytes, hipMemcpyHostToDevice, st));
-  if ((r = zero_words(c.small.ptr, 16, st)) || (r = zero_words(c.lx_status.ptr, st_bytes, st)) ||
-      (r = zero_words(dm + L.off_flags, count * 4, st)))
-    return r;
+  if ((r = zero_ranges({{c.small.ptr, 16}, {c.lx_status.ptr, st_bytes}, {dm + L.off_flags, count * 4}}, st))) return r;
   const bool timed = c.timing && !caps;
   if (timed) HIP_TRY(hipEventRecord(c.ev[0], st));
   HIP_TRY(lex_launch(p, c.lex_grid, c.lex_grid2, st, c.timer));

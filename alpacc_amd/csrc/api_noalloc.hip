@@ -156,8 +156,7 @@ extern "C" int alpacc_hip_lex_device_noalloc(alpacc_hip_ctx* h, int mode, const 
   hipLaunchKernelGGL(noalloc_meta, dim3(1), dim3(64), 0, st, dm, L, n, (uint32_t)nchunks, (uint32_t)ntiles);
   HIP_TRY(hipGetLastError());
   int r;
-  if ((r = zero_words(sc, 16, st)) || (r = zero_words(p.count_status, ntiles * 8, st)) ||
-      (r = zero_words(p.seg_flags, 4, st)))
+  if ((r = zero_ranges({{sc, 16}, {p.count_status, ntiles * 8}, {p.seg_flags, 4}}, st)))
     return r;
   KernelTimer off;  // no events: they would be created on first use
   HIP_TRY(lex_launch(p, c.lex_grid, c.lex_grid2, st, off));

@@ -145,15 +145,15 @@ __device__ __forceinline__ void load_lane(const LexParams& p, uint64_t base, uin
   a[IPT / 4] = shift ? reinterpret_cast<const uint32_t*>(src)[IPT / 4] : 0u;
 }
 
-// The dwords holding the lane's first `valid` bytes (fewer than IPT: the
-// input ends inside the lane), zero beyond them: only dwords holding one of
-// the input's bytes are read.
+// The dwords holding the lane's first `valid` bytes (any count up to IPT;
+// none for a lane without bytes, whose base may lie past the inputs), zero
+// beyond them: only dwords holding one of the input's bytes are read.
 template <int IPT>
 __device__ __forceinline__ void load_lane_part(const LexParams& p, uint64_t base, uint32_t shift, uint32_t valid,
                                                uint32_t* a) {
   const uint32_t* src = reinterpret_cast<const uint32_t*>(p.in + base - shift);
 #pragma unroll
-  for (int d = 0; d <= IPT / 4; ++d) a[d] = (uint32_t)(4 * d) < valid + shift ? src[d] : 0u;
+  for (int d = 0; d <= IPT / 4; ++d) a[d] = valid && (uint32_t)(4 * d) < valid + shift ? src[d] : 0u;
 }
 
 // Bytes 4k..4k+3 of the lane.  The empty asm makes the realignment wait
@@ -515,7 +515,10 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
       asm volatile("" : "+v"(waddr));  // a running address, not rebuilt from a count
       row = RC::next_row(x, lds0);
     };
-    if (valid == IPT) {
+    // One walk a wave: the plain one when every lane of the wave is full,
+    // else the bounded one in every lane (a wave taking both would walk its
+    // bytes twice)
+    if (__ballot(valid != IPT) == 0) {
       uint32_t a[IPT / 4 + 1];
       load_lane<IPT>(p, base, ls.shift, a);
 #pragma unroll
@@ -530,7 +533,7 @@ __global__ __launch_bounds__(WG) void lex_emit(LexParams p) {
         step1(e02 >> 16, bm[k >> 3], km[k >> 3], (4 * k + 2) & 31);
         step1(e13 >> 16, bm[k >> 3], km[k >> 3], (4 * k + 3) & 31);
       }
-    } else if (valid > 0) {
+    } else {
       uint32_t a[IPT / 4 + 1];
       load_lane_part<IPT>(p, base, ls.shift, valid, a);
 #pragma unroll

@@ -1110,32 +1110,86 @@ __global__ __launch_bounds__(256) void far_resolve(Far f) {
   }
 }
 
-__global__ __launch_bounds__(256) void tiles_min_level(const int32_t* in, uint64_t n_in, int32_t* out, uint64_t n_out) {
+// Levels 1 and 2 of the two tile trees (bracket and slot depths) in one
+// launch, blockIdx.y the tree: a workgroup takes 256 level-1 entries (8192
+// tiles) and, when the trees have a level 2, the 8 level-2 entries over
+// them (from LDS).
+struct TreeLevels {
+  const int32_t* in[2];
+  int32_t* l1[2];
+  int32_t* l2[2];  // null: no level 2
+  uint64_t n0, n1, n2;
+};
+__global__ __launch_bounds__(256) void tiles_min_12(TreeLevels t) {
+  __shared__ int32_t s_m[256];
+  const uint32_t y = blockIdx.y;
+  const uint64_t c = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  int32_t m = INT32_MAX;  // past the last entry: no effect on a minimum
+  if (c < t.n1) {
+    const int32_t* in = t.in[y];
+    const uint64_t lo = c * 32, hi = lo + 32 < t.n0 ? lo + 32 : t.n0;
+    m = in[lo];
+    for (uint64_t i = lo + 1; i < hi; ++i) m = in[i] < m ? in[i] : m;
+    t.l1[y][c] = m;
+  }
+  if (!t.l2[y]) return;  // the same in the whole workgroup
+  s_m[threadIdx.x] = m;
+  __syncthreads();
+  const uint64_t c2 = (uint64_t)blockIdx.x * 8 + threadIdx.x;
+  if (threadIdx.x < 8 && c2 < t.n2) {
+    int32_t v = s_m[32 * threadIdx.x];
+    for (uint32_t i = 1; i < 32; ++i) v = s_m[32 * threadIdx.x + i] < v ? s_m[32 * threadIdx.x + i] : v;
+    t.l2[y][c2] = v;
+  }
+}
+
+// One further level of both trees (blockIdx.y the tree).
+__global__ __launch_bounds__(256) void tiles_min_level(const int32_t* in0, const int32_t* in1, uint64_t n_in,
+                                                       int32_t* out0, int32_t* out1, uint64_t n_out) {
   const uint64_t c = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (c >= n_out) return;
+  const int32_t* in = blockIdx.y ? in1 : in0;
   const uint64_t lo = c * 32, hi = lo + 32 < n_in ? lo + 32 : n_in;
   int32_t m = in[lo];
   for (uint64_t i = lo + 1; i < hi; ++i) m = in[i] < m ? in[i] : m;
-  out[c] = m;
+  (blockIdx.y ? out1 : out0)[c] = m;
 }
 
-int build_tile_tree(const int32_t* lv0, uint64_t n, int32_t* scratch, hipStream_t st, TileTree* t) {
-  t->levels = 1;
-  t->lv[0] = lv0;
-  t->n[0] = n;
-  int32_t* p = scratch;
-  uint64_t m = n;
-  while (m > 32) {
-    if (t->levels == MT_LEVELS) return fail(ALPACC_ERR_LIMIT, "tile tree too deep");
+// The min trees over the n tile minima a (bracket depths) and b (slot
+// depths), their upper levels in scratch_a / scratch_b: level L + 1 holds
+// the minimum of every 32 entries of level L, up to a level of at most 32.
+int build_tile_trees(const int32_t* a, const int32_t* b, uint64_t n, int32_t* scratch_a, int32_t* scratch_b,
+                     hipStream_t st, TileTree* ta, TileTree* tb) {
+  ta->levels = tb->levels = 1;
+  ta->lv[0] = a;
+  tb->lv[0] = b;
+  ta->n[0] = tb->n[0] = n;
+  int32_t *pa = scratch_a, *pb = scratch_b;
+  for (uint64_t m = n; m > 32;) {
+    if (ta->levels == MT_LEVELS) return fail(ALPACC_ERR_LIMIT, "tile tree too deep");
     const uint64_t nn = (m + 31) / 32;
-    hipLaunchKernelGGL(tiles_min_level, dim3((uint32_t)((nn + 255) / 256)), dim3(256), 0, st, t->lv[t->levels - 1], m,
-                       p, nn);
-    t->lv[t->levels] = p;
-    t->n[t->levels] = nn;
-    t->levels++;
-    p += nn;
+    ta->lv[ta->levels] = pa;
+    tb->lv[tb->levels] = pb;
+    ta->n[ta->levels] = tb->n[tb->levels] = nn;
+    ta->levels++;
+    tb->levels++;
+    pa += nn;
+    pb += nn;
     m = nn;
   }
+  const int levels = ta->levels;
+  if (levels >= 2) {
+    const bool two = levels >= 3;
+    TreeLevels t{{a, b},
+                 {const_cast<int32_t*>(ta->lv[1]), const_cast<int32_t*>(tb->lv[1])},
+                 {two ? const_cast<int32_t*>(ta->lv[2]) : nullptr, two ? const_cast<int32_t*>(tb->lv[2]) : nullptr},
+                 n, ta->n[1], two ? ta->n[2] : 0};
+    hipLaunchKernelGGL(tiles_min_12, dim3((uint32_t)((ta->n[1] + 255) / 256), 2), dim3(256), 0, st, t);
+  }
+  for (int L = 3; L < levels; ++L)
+    hipLaunchKernelGGL(tiles_min_level, dim3((uint32_t)((ta->n[L] + 255) / 256), 2), dim3(256), 0, st, ta->lv[L - 1],
+                       tb->lv[L - 1], ta->n[L - 1], const_cast<int32_t*>(ta->lv[L]), const_cast<int32_t*>(tb->lv[L]),
+                       ta->n[L]);
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -1209,8 +1263,8 @@ __global__ void pseg_map(const uint32_t* tile_first, uint32_t count, const uint3
 // The tiles of a batch whose token counts are known on the device only:
 // input i gets ceil((tokens + 2) / PT) tiles unless skip[i] is set or i is
 // at or past *live (padding); tile_first is their exclusive scan and
-// *nt_dev the total.  One workgroup of PLAN_WG lanes, PLAN_IPT inputs each
-// per round.
+// *nt_dev the total.  One workgroup of PLAN_WG lanes, in rounds of
+// PLAN_WG * PLAN_IPT inputs.
 constexpr int PLAN_WG = 1024, PLAN_IPT = 4;
 // A batch with more tokens than the sequence is sized for (tokens_cap, a
 // capacity learned from earlier batches; 0: none beyond the bytes) or more
@@ -1223,14 +1277,20 @@ __global__ __launch_bounds__(PLAN_WG) void parse_plan(const uint64_t* tok_off, c
                                                       uint64_t tokens_cap, uint32_t* overflow) {
   __shared__ uint32_t s_w[PLAN_WG / WAVE];
   const uint32_t tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE, n_live = *live;
+  const uint32_t sh = __builtin_ctz(PT);  // a power of two
   uint32_t carry = 0;
   for (uint32_t r0 = 0; r0 < count; r0 += PLAN_WG * PLAN_IPT) {
+    // framed positions ([|-] tokens [-|]) in whole tiles (every load
+    // unconditional, from an index clamped to the last input, so that none
+    // waits on another); an input's tokens fit 32 bits (inputs are below
+    // 2^31 bytes)
     uint32_t t[PLAN_IPT], sum = 0;
 #pragma unroll
     for (int k = 0; k < PLAN_IPT; ++k) {
-      const uint32_t i = r0 + tid * PLAN_IPT + k;
-      const bool on = i < count && i < n_live && !skip[i];
-      t[k] = on ? (uint32_t)((tok_off[i + 1] - tok_off[i] + 2 + PT - 1) / PT) : 0u;
+      const uint32_t i = r0 + tid * PLAN_IPT + k, ic = i < count ? i : count - 1;
+      const uint64_t a = tok_off[ic], b = tok_off[ic + 1];
+      const uint32_t sk = skip[ic];
+      t[k] = i < count && i < n_live && !sk ? ((uint32_t)(b - a) + 2 + PT - 1) >> sh : 0u;
       sum += t[k];
     }
     const uint32_t inc = wave_scan_add32(sum);
@@ -1266,13 +1326,10 @@ __global__ __launch_bounds__(PLAN_WG) void parse_plan(const uint64_t* tok_off, c
 
 }  // namespace
 
-bool parse_scan_overflow(const Context& c, uint64_t count) {
-  const ScanOut* so = reinterpret_cast<const ScanOut*>(c.h_pres.as<uint8_t>() + count * sizeof(ParseResult));
-  return so->overflow != 0;
-}
+bool parse_scan_overflow(const Context& c) { return c.h_pres.as<ScanOut>()->overflow != 0; }
 
-void parse_scan_maxima(const Context& c, uint64_t count, bool tree, int* nodes, int* terms) {
-  const ScanOut* so = reinterpret_cast<const ScanOut*>(c.h_pres.as<uint8_t>() + count * sizeof(ParseResult));
+void parse_scan_maxima(const Context& c, bool tree, int* nodes, int* terms) {
+  const ScanOut* so = c.h_pres.as<ScanOut>();
   *nodes = (int)(tree ? so->tmax_node : so->tmax_slot);
   *terms = tree ? (int)so->tmax_term : 0;
 }
@@ -1392,11 +1449,13 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
   const size_t off_tsx = al(off_tb + tts * 4);
   const size_t off_bs = al(off_tsx + tts * 4);
   const size_t off_out = al(off_bs + (size_t)(nsb ? nsb : 1) * NV * 8);
-  const size_t off_res = al(off_out + sizeof(ScanOut));
+  // the scan's totals, then the per-input results: one copy brings both to
+  // the host mirror (h_pres, the same layout)
+  const size_t off_res = al(off_out + sizeof(ScanOut)), res_head = off_res - off_out;
   const size_t off_fl = al(off_res + count * sizeof(ParseResult));
   const size_t off_pid = al(off_fl + count * 4);
   const size_t total = off_pid + ntiles * PT * 2 + 64;
-  if ((r = c.ps_a.reserve(total)) || (r = c.h_pres.reserve(count * sizeof(ParseResult) + sizeof(ScanOut) + 64)))
+  if ((r = c.ps_a.reserve(total)) || (r = c.h_pres.reserve(res_head + count * sizeof(ParseResult) + 64)))
     return r;
   uint8_t* base = c.ps_a.as<uint8_t>();
   uint32_t* d_tf = reinterpret_cast<uint32_t*>(base + off_tf);
@@ -1431,7 +1490,7 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
   ScanOut* sout = reinterpret_cast<ScanOut*>(base + off_out);
   ParseResult* res = reinterpret_cast<ParseResult*>(base + off_res);
 
-  if ((r = zero_words(s.flags, count * 4, st)) || (r = zero_words(sout, sizeof(ScanOut), st))) return r;
+  if ((r = zero_ranges({{s.flags, count * 4}, {sout, sizeof(ScanOut)}}, st))) return r;
   if (caps) {
     hipLaunchKernelGGL(parse_plan, dim3(1), dim3(PLAN_WG), 0, st, d_tok_off, d_skip, d_live, s.count, (uint32_t)PT, d_tf,
                        d_tf + count + 1, (uint32_t)ntiles,
@@ -1508,8 +1567,8 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
   if (ntiles) {
     TileTree tb{}, ts{};
     if (!caps) HIP_TRY(tm.begin(st, "tile_tree"));
-    if ((r = build_tile_tree(mb, ntiles, reinterpret_cast<int32_t*>(base + off_tb), st, &tb)) ||
-        (r = build_tile_tree(ms, ntiles, reinterpret_cast<int32_t*>(base + off_tsx), st, &ts)))
+    if ((r = build_tile_trees(mb, ms, ntiles, reinterpret_cast<int32_t*>(base + off_tb),
+                              reinterpret_cast<int32_t*>(base + off_tsx), st, &tb, &ts)))
       return r;
     if (!caps) HIP_TRY(tm.end(st));
     // staging sized to the largest tile when the scan's maxima are on the
@@ -1555,11 +1614,9 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
                      tree ? 1 : 0, res);
   HIP_TRY(hipGetLastError());
   if (timed) HIP_TRY(hipEventRecord(c.ev[3], st));
-  c.cp_parse_down = Context::Copy{c.h_pres.ptr, res, count * sizeof(ParseResult), hipMemcpyDeviceToHost};
-  // the scan's totals (maxima, overflow) land after the per-input results
-  c.cp_scan_down = Context::Copy{c.h_pres.as<uint8_t>() + count * sizeof(ParseResult), sout, sizeof(ScanOut),
-                                 hipMemcpyDeviceToHost};
-  if (!c.capturing) HIP_TRY(hipMemcpyAsync(c.h_pres.ptr, res, count * sizeof(ParseResult), hipMemcpyDeviceToHost, st));
+  // the scan's totals (maxima, overflow) and the per-input results
+  c.cp_parse_down = Context::Copy{c.h_pres.ptr, sout, res_head + count * sizeof(ParseResult), hipMemcpyDeviceToHost};
+  if (!c.capturing) HIP_TRY(hipMemcpyAsync(c.h_pres.ptr, sout, res_head + count * sizeof(ParseResult), hipMemcpyDeviceToHost, st));
   if (!caps) {
     HIP_TRY(hipStreamSynchronize(st));
     HIP_TRY(tm.collect());
@@ -1571,7 +1628,7 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
     c.parse_launches++;
   }
   c.pb_count = count;
-  c.pb_res = c.h_pres.as<ParseResult>();
+  c.pb_res = reinterpret_cast<const ParseResult*>(c.h_pres.as<uint8_t>() + res_head);
   return 0;
 }
 

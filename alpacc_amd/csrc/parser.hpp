@@ -48,12 +48,12 @@ uint64_t parse_max_positions(const GrammarDesc* g);
 // lane of the resolve pass (4; or 2, 4, 8 at 1024 positions per tile).
 bool parse_shape_ok(int positions_per_tile, int positions_per_lane);
 
-// After the copy c.cp_scan_down of a launch of `count` strings has landed
-// in host memory: whether seg_resolve met a tile beyond its staging (the
-// launch's results are not valid), and the largest per-tile node and
-// terminal-production counts.
-bool parse_scan_overflow(const Context& c, uint64_t count);
-void parse_scan_maxima(const Context& c, uint64_t count, bool tree, int* nodes, int* terms);
+// After the copy c.cp_parse_down of a launch has landed in host memory:
+// whether seg_resolve met a tile beyond its staging (the launch's results
+// are not valid), and the largest per-tile node and terminal-production
+// counts.
+bool parse_scan_overflow(const Context& c);
+void parse_scan_maxima(const Context& c, bool tree, int* nodes, int* terms);
 
 // Largest bracket symbol of the grammar's pair records.
 uint32_t bracket_symbol_max(const GrammarDesc* g);

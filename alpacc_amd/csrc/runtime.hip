@@ -126,6 +126,36 @@ int zero_words(void* p, size_t bytes, hipStream_t st) {
   return 0;
 }
 
+// Several ranges in one launch: workgroup row y clears range y.
+struct ZeroRanges {
+  uint32_t* p[ZERO_RANGES_MAX];
+  size_t n[ZERO_RANGES_MAX];
+};
+__global__ void fill_zero_ranges(ZeroRanges z) {
+  uint32_t* p = z.p[blockIdx.y];
+  const size_t n = z.n[blockIdx.y];
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0;
+}
+
+int zero_ranges(std::initializer_list<std::pair<void*, size_t>> ranges, hipStream_t st) {
+  if (ranges.size() < 1 || ranges.size() > ZERO_RANGES_MAX) return fail(ALPACC_ERR_ARG, "zero_ranges of %zu ranges", ranges.size());
+  ZeroRanges z{};
+  size_t most = 0;
+  int k = 0;
+  for (const auto& r : ranges) {
+    z.p[k] = static_cast<uint32_t*>(r.first);
+    z.n[k] = (r.second + 3) / 4;
+    most = z.n[k] > most ? z.n[k] : most;
+    ++k;
+  }
+  int blocks = (int)((most + 255) / 256);
+  if (blocks > 4096) blocks = 4096;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(fill_zero_ranges, dim3(blocks, k), dim3(256), 0, st, z);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 }  // namespace alpacc
 
 using namespace alpacc;

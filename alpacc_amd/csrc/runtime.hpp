@@ -4,6 +4,8 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <initializer_list>
+#include <utility>
 #include <vector>
 
 #include "grammar.hpp"
@@ -169,10 +171,10 @@ struct Context {
     hipMemcpyKind kind = hipMemcpyHostToDevice;
   };
   bool capturing = false;
-  Copy cp_meta_up, cp_lex_down, cp_parse_down, cp_scan_down;
+  Copy cp_meta_up, cp_lex_down, cp_parse_down;
   // what a replay restores (calls between replays reuse the fields)
   struct GraphState {
-    Copy up, lex_down, parse_down, scan_down;
+    Copy up, lex_down, parse_down;
     uint64_t lb_count = 0, pb_count = 0;
     const uint32_t* lb_flags = nullptr;
     const uint64_t* lb_tok = nullptr;
@@ -208,6 +210,10 @@ namespace alpacc {
 
 int fail(int code, const char* fmt, ...);
 int zero_words(void* p, size_t bytes, hipStream_t st);
+// Zero-fills up to ZERO_RANGES_MAX ranges of (pointer, bytes; 4-byte words)
+// in one launch.
+constexpr int ZERO_RANGES_MAX = 4;
+int zero_ranges(std::initializer_list<std::pair<void*, size_t>> ranges, hipStream_t st);
 // Raises kernel k's dynamic-LDS limit on the current device to at least
 // `bytes`.  The limit belongs to the kernel for the whole process, so the
 // grants of every context are kept in one mutex-guarded table per (device,
