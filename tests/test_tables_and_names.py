@@ -33,6 +33,26 @@ def grammar_files():
     return out + [os.path.join(TEST_GRAMMARS, f) for f in sorted(os.listdir(TEST_GRAMMARS)) if f.endswith(".alp")]
 
 
+def tables_blob(g, d, name, **over):
+    """A tables file of grammar g from its device tables d under another
+    name, with fields replaced by `over` (to plant a malformed field)."""
+    fields = emit_hip._fields(g, d)
+    fields[0] = ("name", "str", name)
+    fields = [(n, k, over.get(n, v)) for n, k, v in fields]
+    blob = bytearray(emit_hip.TABLES_MAGIC) + struct.pack("<I", len(fields))
+    blob += b"\0" * (-len(blob) % 8)
+    for n, kind, v in fields:
+        if kind == "str":
+            blob += struct.pack("<I", len(v)) + v.encode()
+        elif kind == "int":
+            blob += b"I" + b"\0" * 7 + struct.pack("<q", int(v))
+        else:
+            fmt, size = {"u8": ("B", 1), "u16": ("H", 2), "i32": ("i", 4), "u64": ("Q", 8)}[kind]
+            blob += b"A" + b"\0" * 3 + struct.pack("<IQ", size, len(v)) + struct.pack(f"<{len(v)}{fmt}", *v)
+        blob += b"\0" * (-len(blob) % 8)
+    return bytes(blob)
+
+
 # ---------------------------------------------------------------- names
 
 JSON_TERMINALS = ["string", "number"] + [f"literal_{i}" for i in range(2, 11)] + ["ignore", "empty_12"]
@@ -295,31 +315,57 @@ def test_hashed_tables_file_loads_and_checks(tmp_path):
     assert hip.load_grammar(str(f)) == "json_hashed_cpu"
     assert hip.names("json_hashed_cpu", "terminal") == g.lex_both.names()
 
-    def with_fields(d, name, **over):
-        fields = emit_hip._fields(g, d)
-        fields[0] = ("name", "str", name)
-        fields = [(n, k, over.get(n, v)) for n, k, v in fields]
-        blob = bytearray(emit_hip.TABLES_MAGIC) + struct.pack("<I", len(fields))
-        blob += b"\0" * (-len(blob) % 8)
-        for n, kind, v in fields:
-            if kind == "str":
-                blob += struct.pack("<I", len(v)) + v.encode()
-            elif kind == "int":
-                blob += b"I" + b"\0" * 7 + struct.pack("<q", int(v))
-            else:
-                fmt, size = {"u8": ("B", 1), "u16": ("H", 2), "i32": ("i", 4), "u64": ("Q", 8)}[kind]
-                blob += b"A" + b"\0" * 3 + struct.pack("<IQ", size, len(v)) + struct.pack(f"<{len(v)}{fmt}", *v)
-            blob += b"\0" * (-len(blob) % 8)
-        return bytes(blob)
-
     dense = emit_hip.device_tables(g, hashed=False)
     short = dense["window"][:-1]
     bad = tmp_path / "short.alphip"
-    bad.write_bytes(with_fields(dense, "json_short", nwindow=len(short), window=short))
+    bad.write_bytes(tables_blob(g, dense, "json_short", nwindow=len(short), window=short))
     with pytest.raises(hip.HipError):
         hip.load_grammar(str(bad))
     hashed = emit_hip.device_tables(g, hashed=True)
     bad2 = tmp_path / "slots.alphip"
-    bad2.write_bytes(with_fields(hashed, "json_slots", window_hash=hashed["window_hash"] + 1))
+    bad2.write_bytes(tables_blob(g, hashed, "json_slots", window_hash=hashed["window_hash"] + 1))
     with pytest.raises(hip.HipError):
         hip.load_grammar(str(bad2))
+
+
+def test_two_byte_production_ids():
+    """A grammar of more than 255 productions (many_productions, 268) takes
+    two-byte production ids: each pair record keeps the low byte of its
+    productions (bytes 24..31), the side table the whole ids, 8 a pair, in
+    the LLP table's order for that pair, zero past the pair's count and for
+    the empty record after the last pair; some id passes 255."""
+    g = load(os.path.join(TEST_GRAMMARS, "many_productions.alp"))
+    d = emit_hip.device_tables(g)
+    assert d["production_bytes"] == 2 and d["n_productions"] == 268
+    npairs, rec, p16 = d["npairs"], d["pair_rec"], d["pair_prod16"]
+    assert len(p16) == 8 * (npairs + 1) and len(rec) == 64 * (npairs + 1)
+    for i in range(npairs):
+        p0, pn = d["spans"][4 * i + 2], d["spans"][4 * i + 3]
+        want = list(d["productions"][p0:p0 + pn])
+        ids = p16[8 * i:8 * i + 8]
+        assert rec[64 * i + 2] == pn
+        assert ids == want + [0] * (8 - pn)
+        assert rec[64 * i + 24:64 * i + 32] == [x & 0xFF for x in ids]
+    assert p16[8 * npairs:] == [0] * 8
+    assert max(p16) > 255 and max(p16) < d["n_productions"]
+    # one-byte grammars carry no side table
+    assert emit_hip.device_tables(load(os.path.join(ROOT, "grammars", "json.alp")))["pair_prod16"] == []
+
+
+@needs_lib
+def test_two_byte_production_tables_file(tmp_path):
+    """The tables file of a two-byte-id grammar loads; the loader refuses a
+    side table shorter than 8 ids a pair and production ids wider than two
+    bytes."""
+    from alpacc_amd import hip
+    g = load(os.path.join(TEST_GRAMMARS, "many_productions.alp"))
+    f = tmp_path / "many_productions_cpu.alphip"
+    f.write_bytes(emit_hip.emit_tables(g, name="many_productions_cpu"))
+    assert hip.load_grammar(str(f)) == "many_productions_cpu"
+    assert len(hip.names("many_productions_cpu", "production")) == 268
+    d = emit_hip.device_tables(g)
+    for name, over in [("mp_short", dict(pair_prod16=d["pair_prod16"][:-1])), ("mp_wide", dict(production_bytes=3))]:
+        bad = tmp_path / (name + ".alphip")
+        bad.write_bytes(tables_blob(g, d, name, **over))
+        with pytest.raises(hip.HipError):
+            hip.load_grammar(str(bad))
