@@ -414,8 +414,10 @@ struct Records {
 // positions), and in parser mode at most parse_max_positions framed
 // positions.  In combined mode the token count is known only after the
 // lexer, so a launch the parser refuses for its size is split in halves.
-// An input beyond the lexer's limit alone is answered by run_case (the
-// stream path in lexer mode; a size error in the others).
+// An input beyond BATCH_LAUNCH_BYTES alone is answered by run_case: past the
+// lexer's launch limit through the stream path (in combined mode followed
+// by one parser launch over all its tokens); in parser mode by one parser
+// launch, refused beyond that launch's positions.
 constexpr uint64_t BATCH_LAUNCH_BYTES = 1ull << 30;
 
 int records_in_launches(Context& c, int mode, const uint8_t* packed, const uint64_t* off, uint64_t a, uint64_t b,

@@ -442,7 +442,6 @@ namespace alpacc {
 // The lexer-only record of an input of any size: its bytes are staged and
 // lexed in chunks of LEX_STREAM_CHUNK through the stream entry points, the
 // tokens of every chunk appended to the record as they close.
-constexpr uint64_t LEX_STREAM_CHUNK = 1ull << 30;
 
 int lexer_record_stream(Context& c, uint64_t n, const uint8_t* payload, std::vector<uint8_t>& rec) {
   hipStream_t st = c.stream;

@@ -2,6 +2,7 @@
 // lex + parse + tree run, their device forms and wire records).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -20,6 +21,9 @@ namespace alpacc {
 
 int stage_input(Context& c, const void* host, uint64_t bytes, hipStream_t st, const uint8_t** d);
 int lexemes_to_host(Context& c, int mode, uint64_t count, hipStream_t st, alpacc_hip_lexemes* out);
+void lex_stream_begin(Context& c, int mode);
+int lex_stream_push(Context& c, const uint8_t* d_chunk, uint64_t n, int is_last_chunk, alpacc_hip_lexemes* d_out,
+                    hipStream_t st);
 
 static int lex_parse_device(Context& c, const uint8_t* d_in, uint64_t n, hipStream_t st, alpacc_hip_tree* out) {
   uint64_t count = 0;
@@ -78,8 +82,95 @@ int parser_record(Context& c, uint64_t n, const uint8_t* payload, std::vector<ui
   return ALPACC_VALID;
 }
 
+// Combined record of an input beyond one lexer launch: its bytes are lexed
+// in chunks of LEX_STREAM_CHUNK through the stream path (combined
+// numbering; the tokens gathered on the host as they close), then its
+// terminals are parsed in one launch.  An input whose tokens pass one
+// parser launch's positions is refused (a size error).
+static int both_record_stream(Context& c, uint64_t n, const uint8_t* payload, std::vector<uint8_t>& rec) {
+  hipStream_t st = c.stream;
+  const GrammarDesc* g = c.g;
+  const int tb = g->terminal_bytes_both, ib = g->index_bytes, lb = g->length_bytes, pb = g->production_bytes;
+  std::vector<uint8_t> ht, hl;
+  std::vector<int64_t> hs;
+  uint64_t T = 0;
+  int r;
+  lex_stream_begin(c, ALPACC_MODE_BOTH);
+  for (uint64_t at = 0; at < n;) {
+    const uint64_t m = std::min(n - at, LEX_STREAM_CHUNK);
+    const bool last = at + m == n;
+    const uint8_t* d_in = nullptr;
+    alpacc_hip_lexemes d{};
+    if ((r = stage_input(c, payload + at, m, st, &d_in)) || (r = lex_stream_push(c, d_in, m, last ? 1 : 0, &d, st)) < 0) {
+      c.lstream.on = false;
+      return r;
+    }
+    if (r == ALPACC_REJECT) {
+      c.lstream.on = false;
+      rec.assign(1, 0);
+      return ALPACC_REJECT;
+    }
+    if (d.count) {  // the chunk's tokens, starts counted from the input's first byte
+      ht.resize((T + d.count) * tb);
+      hs.resize(T + d.count);
+      hl.resize((T + d.count) * lb);
+      HIP_TRY(hipMemcpyAsync(ht.data() + T * tb, d.terminals, d.count * tb, hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipMemcpyAsync(hs.data() + T, d.starts, d.count * 8, hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipMemcpyAsync(hl.data() + T * lb, d.lengths, d.count * lb, hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+      T += d.count;
+    }
+    at += m;
+  }
+  if (T + 2 > parse_max_positions(g))
+    return fail(ALPACC_ERR_LIMIT, "%llu tokens exceed one parser launch (%llu framed positions)", (unsigned long long)T,
+                (unsigned long long)parse_max_positions(g));
+  const uint8_t* d_tok = nullptr;
+  if ((r = stage_input(c, ht.data(), T * tb, st, &d_tok))) return r;
+  uint64_t np = 0;
+  if ((r = parse_on_device(c, d_tok, T, true, st, &np)) < 0) return r;
+  if (r == ALPACC_REJECT) {
+    rec.assign(1, 0);
+    return r;
+  }
+  const uint64_t N = np - T;
+  rec.resize(1 + 8 + T * (tb + ib + lb) + 8 + N * (pb + ib) + T * ib);
+  uint8_t* o = rec.data();
+  *o++ = 1;
+  memcpy(o, &T, 8);
+  o += 8;
+  // indices are i64 here and on the device; a 32-bit index_t keeps the low
+  // half of each (little-endian)
+  auto put_index = [&](const int64_t* v, uint64_t count) {
+    if (ib == 8) {
+      memcpy(o, v, count * 8);
+    } else {
+      for (uint64_t i = 0; i < count; ++i) memcpy(o + i * ib, &v[i], ib);
+    }
+    o += count * ib;
+  };
+  memcpy(o, ht.data(), T * tb);
+  o += T * tb;
+  put_index(hs.data(), T);
+  memcpy(o, hl.data(), T * lb);
+  o += T * lb;
+  memcpy(o, &N, 8);
+  o += 8;
+  if (N) HIP_TRY(hipMemcpyAsync(o, c.tr_prod.ptr, N * pb, hipMemcpyDeviceToHost, st));
+  o += N * pb;
+  std::vector<int64_t> par(std::max(N, T));
+  if (N) HIP_TRY(hipMemcpyAsync(par.data(), c.tr_parent.ptr, N * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  put_index(par.data(), N);
+  if (T) HIP_TRY(hipMemcpyAsync(par.data(), c.tr_tokpar.ptr, T * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  put_index(par.data(), T);
+  return ALPACC_VALID;
+}
+
 // Combined record (C/Generator.hs:114-152): SoA tokens, then the compact tree.
 int both_record(Context& c, uint64_t n, const uint8_t* payload, std::vector<uint8_t>& rec) {
+  if (n > LEX_MAX_LAUNCH_BYTES) return both_record_stream(c, n, payload, rec);
   hipStream_t st = c.stream;
   const uint8_t* d_in = nullptr;
   int r;

@@ -15,6 +15,8 @@ constexpr int LEX_LB_WIN = 16;  // predecessor tiles per lookback probe by defau
 // Input bytes of one lexer launch: positions and counts travel in 31-bit
 // fields of the lookback words.
 constexpr uint64_t LEX_MAX_LAUNCH_BYTES = (1ull << 31) - 1;
+// Chunk of an input beyond one launch, lexed through the stream path.
+constexpr uint64_t LEX_STREAM_CHUNK = 1ull << 30;
 
 constexpr uint32_t LEX_REJECT_FINAL = 1u;     // last state not accepting
 constexpr uint32_t LEX_REJECT_OVERFLOW = 2u;  // a kept token is longer than length_t

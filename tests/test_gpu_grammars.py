@@ -19,8 +19,9 @@ known answers on the GPU.
   combined batches also through the captured graph, and every bracket width
   the grammar allows.
 * Inputs beyond one launch: a counted batch of two 1.2 GiB JSON frames
-  through the wire program (answered in two launches), and a 2.2 GiB JSON
-  lexer-only input (lexed through the stream path in chunks).
+  through the wire program (answered in two launches), a 2.2 GiB JSON
+  lexer-only input (lexed through the stream path in chunks) and a 2.05 GiB
+  JSON combined input (the same, then one parser launch).
 """
 
 import hashlib
@@ -59,10 +60,18 @@ def load_tables(name):
 
 
 def first_diff(a: bytes, b: bytes):
+    """Offset of the first differing byte (None when equal), found block by
+    block so that records of several GB compare quickly."""
     if a == b:
         return None
     n = min(len(a), len(b))
-    return next((i for i in range(n) if a[i] != b[i]), n)
+    va, vb = memoryview(a), memoryview(b)
+    step = 1 << 26
+    for s in range(0, n, step):
+        e = min(n, s + step)
+        if va[s:e] != vb[s:e]:
+            return s + int(np.argmax(np.frombuffer(va[s:e], np.uint8) != np.frombuffer(vb[s:e], np.uint8)))
+    return n
 
 
 # ------------------------------------------------------------- known answers
@@ -245,4 +254,19 @@ def test_lexer_input_beyond_launch_limit():
     with hip.Backend("json") as be:
         got = be.run_case("lexer", data)
     exp = pyoracle.record("json", "lexer", data)
+    assert got[0] == 1 and first_diff(got, exp) is None
+
+
+def test_combined_input_beyond_launch_limit():
+    """A combined-mode input past one lexer launch (2.05 GiB of JSON): lexed
+    in chunks through the stream path, then parsed in one launch; the whole
+    record (about 8 GB: tokens, then the tree) equals the oracle's."""
+    from alpacc_amd import hip
+    n = int(2.05 * (1 << 30))
+    data = datagen.generate("json", n, seed=9, conforming=True).tobytes()
+    print("case json combined 2.05 GiB", flush=True)
+    with hip.Backend("json") as be:
+        got = be.run_case("both", data)
+    print("case json combined 2.05 GiB: oracle", flush=True)
+    exp = pyoracle.record("json", "both", data)
     assert got[0] == 1 and first_diff(got, exp) is None
