@@ -82,6 +82,7 @@ struct Seg {
   // terminal productions, slot pops and pushes (a byte each, bits 16..63;
   // zero without a pair)
   const uint64_t* window;
+  uint32_t window_bytes;  // its size (the bound of the buffer loads of pass A)
   // hashed window tables (the grammar's dense index would be too large):
   // the code of each of the 2^hbits slots, all ones when empty; null when
   // the window entries are a dense index
@@ -298,11 +299,19 @@ __device__ __forceinline__ void window_ids_exact(const Seg& s, const TileOf& t, 
       code = code * base + v[j];
       bad += v[j] >= base;
     }
+    // the entries through a buffer resource: a 32-bit offset from the
+    // table's base and no branch around the load; a window holding a token
+    // that is no terminal (bad) has a code past the table or a wrong one,
+    // and its load, bounds-checked by the hardware (zero past
+    // window_bytes), is replaced by NO_PAIR
+    const __amdgpu_buffer_rsrc_t wtab =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<uint64_t*>(s.window), 0, (int)s.window_bytes, 0x00020000);
 #pragma unroll
     for (int r = 0; r < N; ++r) {
       code = code * base + v[r + W - 1];
       bad += v[r + W - 1] >= base;
-      ids[r] = bad ? (uint64_t)NO_PAIR : s.window[code];
+      const auto e = __builtin_amdgcn_raw_buffer_load_b64(wtab, code * 8u, 0, 0);
+      ids[r] = bad ? (uint64_t)NO_PAIR : ((uint64_t)e[1] << 32 | e[0]);
       code -= v[r] * top;
       bad -= v[r] >= base;
     }
@@ -714,13 +723,21 @@ __device__ __forceinline__ void resolve_tile(const Resolve& r, uint32_t tile, ui
       }
     }
   }
-  const uint4* rp[APL];
+  // the records through a buffer resource: 32-bit offsets instead of a
+  // 64-bit address per position
+  const __amdgpu_buffer_rsrc_t rtab = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<uint4*>(s.rec), 0, (int)(((uint32_t)s.npairs + 1) * 64), 0x00020000);
+  auto rec16 = [&](uint32_t off) -> uint4 {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(rtab, off, 0, 0);
+    return uint4{v[0], v[1], v[2], v[3]};
+  };
+  uint32_t ro[APL];  // byte offset of each position's record
   uint4 h[APL];
   int32_t np = 0, nt = 0, bd = 0, sd = 0;
 #pragma unroll
   for (int k = 0; k < APL; ++k) {
-    rp[k] = s.rec + 4 * (ids[k] == NO_PAIR ? (uint32_t)s.npairs : ids[k]);
-    h[k] = rp[k][0];
+    ro[k] = (ids[k] == NO_PAIR ? (uint32_t)s.npairs : ids[k]) * 64u;
+    h[k] = rec16(ro[k]);
     np += (h[k].x >> 16) & 0xFF;
     nt += h[k].x >> 24;
     bd += (int32_t)((h[k].x >> 8) & 0xFF) - (int32_t)(h[k].x & 0xFF);
@@ -753,10 +770,9 @@ __device__ __forceinline__ void resolve_tile(const Resolve& r, uint32_t tile, ui
   // walk's state, so the compiler does not hoist the four loads together)
 #pragma unroll
   for (int k = 0; k < APL; ++k) {
-    uintptr_t ra = reinterpret_cast<uintptr_t>(rp[k]);
+    uint32_t ra = ro[k];
     asm volatile("" : "+v"(ra) : "v"(slot));
-    const uint4* q = reinterpret_cast<const uint4*>(ra);
-    const uint4 r1 = q[1], r2 = q[2], r3 = q[3];
+    const uint4 r1 = rec16(ra + 16), r2 = rec16(ra + 32), r3 = rec16(ra + 48);
     const uint4 r0 = h[k];
     const uint32_t hx = r0.x, hy = r0.y;
     uint4 p16{0, 0, 0, 0};
@@ -1476,6 +1492,7 @@ int parse_batch_on_device(Context& c, const uint8_t* d_tok, const uint64_t* d_to
   s.start_t = g->start_t;
   s.end_t = g->end_t;
   s.window = c.d_wrec;
+  s.window_bytes = (uint32_t)std::min<uint64_t>((uint64_t)g->nwindow * 8, 0xFFFFFFF8ull);
   s.wkeys = g->window_hash ? c.d_wkeys : nullptr;
   s.hbits = g->window_hash;
   s.rec = reinterpret_cast<const uint4*>(c.d_pair_rec);
