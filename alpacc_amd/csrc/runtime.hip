@@ -38,18 +38,28 @@ int fail(int code, const char* fmt, ...) {
     if (e_ != hipSuccess) return fail(ALPACC_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
   } while (0)
 
+static void free_retired(std::vector<void*>& v, hipError_t (*f)(void*)) {
+  for (void* p : v) (void)f(p);
+  v.clear();
+}
+
 int DevBuf::reserve(size_t want) {
   if (want <= cap) return 0;
   const size_t old = cap;
-  if (ptr) (void)hipFree(ptr);
+  if (ptr) retired.push_back(ptr);
   ptr = nullptr;
   cap = 0;
   // at least double: a workspace that keeps growing (a server's batches)
-  // is reallocated a logarithmic number of times, and hipFree waits for the device
+  // is reallocated a logarithmic number of times
   size_t sz = want < 2 * old ? 2 * old : want;
   sz = sz < 256 ? 256 : sz;
   sz = (sz + 255) & ~size_t(255);
   hipError_t e = hipMalloc(&ptr, sz);
+  if (e != hipSuccess && !retired.empty()) {  // room from the retired blocks (this waits for the device)
+    (void)hipGetLastError();
+    free_retired(retired, hipFree);
+    e = hipMalloc(&ptr, sz);
+  }
   if (e != hipSuccess && sz > want) {  // no room to double: the exact size
     (void)hipGetLastError();
     sz = (want + 255) & ~size_t(255);
@@ -66,6 +76,7 @@ int DevBuf::reserve(size_t want) {
 
 void DevBuf::release() {
   if (ptr) (void)hipFree(ptr);
+  free_retired(retired, hipFree);
   ptr = nullptr;
   cap = 0;
 }
@@ -73,13 +84,18 @@ void DevBuf::release() {
 int PinnedBuf::reserve(size_t want) {
   if (want <= cap) return 0;
   const size_t old = cap;
-  if (ptr) (void)hipHostFree(ptr);
+  if (ptr) retired.push_back(ptr);
   ptr = nullptr;
   cap = 0;
   size_t sz = want < 2 * old ? 2 * old : want;  // at least double, as DevBuf
   sz = sz < 4096 ? 4096 : sz;
   sz = (sz + 4095) & ~size_t(4095);
   hipError_t e = hipHostMalloc(&ptr, sz, hipHostMallocDefault);
+  if (e != hipSuccess && !retired.empty()) {
+    (void)hipGetLastError();
+    free_retired(retired, hipHostFree);
+    e = hipHostMalloc(&ptr, sz, hipHostMallocDefault);
+  }
   if (e != hipSuccess && sz > want) {
     (void)hipGetLastError();
     sz = (want + 4095) & ~size_t(4095);
@@ -96,6 +112,7 @@ int PinnedBuf::reserve(size_t want) {
 
 void PinnedBuf::release() {
   if (ptr) (void)hipHostFree(ptr);
+  free_retired(retired, hipHostFree);
   ptr = nullptr;
   cap = 0;
 }

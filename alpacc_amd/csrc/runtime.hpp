@@ -18,10 +18,16 @@ struct ParseResult;
 
 // Grow-only device allocation: steady-state calls allocate nothing.  gen
 // counts the allocations (a captured launch sequence holds the pointer).
+// A block outgrown is retired, not freed (hipFree waits for the whole
+// device, stalling every other stream's work, such as a server's other
+// lanes); retired blocks go with release(), or when an allocation would
+// not fit otherwise.  Growth at least doubles, so they add at most the
+// current size.
 struct DevBuf {
   void* ptr = nullptr;
   size_t cap = 0;
   uint64_t gen = 0;
+  std::vector<void*> retired;
   int reserve(size_t bytes);
   void release();
   template <typename T>
@@ -30,11 +36,13 @@ struct DevBuf {
   }
 };
 
-// Grow-only pinned host allocation (hipHostMalloc).
+// Grow-only pinned host allocation (hipHostMalloc), outgrown blocks retired
+// as DevBuf's.
 struct PinnedBuf {
   void* ptr = nullptr;
   size_t cap = 0;
   uint64_t gen = 0;
+  std::vector<void*> retired;
   int reserve(size_t bytes);
   void release();
   template <typename T>
