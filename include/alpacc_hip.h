@@ -94,9 +94,10 @@ int alpacc_hip_load_grammar(const char *path, char *name, size_t name_cap);
 /* Size limits of one launch for a grammar, without a device: the input
  * bytes one lexer launch takes (alpacc_hip_run_batch and run_case split
  * larger batches into several launches; a larger lexer-only input goes
- * through the stream path, a larger input of the other modes is refused
- * with ALPACC_ERR_LIMIT) and the framed parser positions (tokens + 2 per
- * input) one parser launch is sure to take. */
+ * through the stream path, a larger combined one through the stream path
+ * and then one parser launch over all its tokens) and the framed parser
+ * positions (tokens + 2 per input) one parser launch is sure to take (an
+ * input beyond them is refused with ALPACC_ERR_LIMIT). */
 int alpacc_hip_grammar_limits(const char *grammar, uint64_t *launch_bytes, uint64_t *parser_positions);
 
 /* Names of ids, as the reference's Futhark target names them in its
