@@ -385,7 +385,7 @@ def main(argv=None):
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--serve-rates", default="2000,10000,50000", help="request rates of the serving workload ('' = skip)")
     ap.add_argument("--serve-seconds", type=float, default=1.0)
-    ap.add_argument("--serve-capacity", default="100000,150000,200000,300000",
+    ap.add_argument("--serve-capacity", default="100000,125000,150000,200000,300000",
                     help="rising request rates of the capacity sweep, until p99 >= 10 ms ('' = skip)")
     ap.add_argument("--capacity-seconds", type=float, default=0.25)
     ap.add_argument("--no-literal", action="store_true", help="skip the step on the literal (as-written) input")
