@@ -351,8 +351,9 @@ def _oracle_worker(args):
 def cpu_parallel_baseline(items, mode: str, label: str, cores: int = 0):
     """The oracle over `items` ((grammar, payload) pairs) in `cores` worker
     processes on this host; aggregate GB/s over the wall-clock of the
-    parallel run.  The default is min(16, nproc): a one-GPU box gives its
-    process 16 CPUs whatever nproc reports (the line records both)."""
+    parallel run.  The default is min(16, nproc): a process with one GPU
+    gets a share of 16 CPUs on the machines this is measured on, whatever
+    nproc reports (the line records both)."""
     import multiprocessing as mp
     cores = cores or max(1, min(16, os.cpu_count() or 1))
     chunks = [(mode, items[i::cores]) for i in range(cores)]

@@ -1,7 +1,7 @@
 """Known answers that pin the oracle's numbering, its lexer and the library's
 names, independently of the table generator that both share.
 
-* SURVEY-derived numbering rules (src/Alpacc/Encode.hs:118-158,
+* The reference's numbering rules (src/Alpacc/Encode.hs:118-158,
   src/Alpacc/Generator/Analyzer.hs:113-126, src/Alpacc/Lexer/DFA.hs:312-313,
   src/Alpacc/Grammar.hs:191-211, 339-361): lexer-only terminal ids follow
   the sorted terminal set, combined ids the grammar's order, so json.alp
