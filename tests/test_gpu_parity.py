@@ -62,6 +62,22 @@ def check(backends, grammar, mode, payload: bytes, label=""):
     assert d is None, f"{grammar}/{mode} {label}: record mismatch at byte {d} (gpu {len(got)} B, oracle {len(exp)} B)"
 
 
+def test_parser_ids_past_the_terminals(backends):
+    """Parser-mode strings holding ids past the grammar's terminals (the
+    window codes they form lie past the window table, whose entries pass A
+    reads with bounds-checked loads) are rejected exactly as the oracle
+    rejects them, alone and in a batch beside valid strings."""
+    for grammar in ("json", "arithmetic", "sexp"):
+        strings = [bytes([200]), bytes([255] * 10), bytes([4, 1, 200, 1, 5, 0]), bytes(range(15, 60)),
+                   bytes([0, 1, 2, 3] * 8 + [250])]
+        for x in strings:
+            check(backends, grammar, "parser", x, "ids past the terminals")
+        toks = lexed_tokens(grammar, datagen.generate(grammar, 3000, seed=2, conforming=True).tobytes())
+        batch = [toks, strings[2], toks[:40] + bytes([201]) + toks[40:], toks]
+        exp = b"".join(pyoracle.record(grammar, "parser", x) for x in batch)
+        assert first_diff(backends[grammar].run_batch("parser", batch), exp) is None, grammar
+
+
 def test_layout_matches_oracle(backends):
     import subprocess
     for g in GRAMMARS:
