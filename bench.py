@@ -160,10 +160,11 @@ def batch_inputs(count: int, seed: int, median: int = 4096, sigma: float = 1.5, 
     return host, offs
 
 
-def batch_workload(be, count: int, steps: int, seed: int = 3):
+def batch_workload(be, count: int, steps: int, seed: int = 3, plain: bool = True):
     """Batching: all inputs lexed and parsed by one batched call per step,
     against the plain path (one lex+parse call, i.e. one launch sequence,
-    per input) over the same device-resident inputs."""
+    per input) over the same device-resident inputs (skipped without
+    `plain`)."""
     import numpy as np
     import torch
     host, offs = batch_inputs(count, seed)
@@ -214,16 +215,18 @@ def batch_workload(be, count: int, steps: int, seed: int = 3):
     batch_shape = be.parse_shape()
     be.set_parse_shape(keep["positions_per_tile"], keep["positions_per_lane"])
     # plain: one call per input, in input order
-    for i in range(min(count, 200)):  # warmup of the per-input path
-        be.lex_parse_device(base + int(offs[i]), int(offs[i + 1] - offs[i]), stream=stream)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    plain_valid = 0
-    for i in range(count):
-        ok, _ = be.lex_parse_device(base + int(offs[i]), int(offs[i + 1] - offs[i]), stream=stream)
-        plain_valid += bool(ok)
-    torch.cuda.synchronize()
-    plain_s = time.perf_counter() - t0
+    plain_valid, plain_s = None, None
+    if plain:
+        for i in range(min(count, 200)):  # warmup of the per-input path
+            be.lex_parse_device(base + int(offs[i]), int(offs[i + 1] - offs[i]), stream=stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plain_valid = 0
+        for i in range(count):
+            ok, _ = be.lex_parse_device(base + int(offs[i]), int(offs[i + 1] - offs[i]), stream=stream)
+            plain_valid += bool(ok)
+        torch.cuda.synchronize()
+        plain_s = time.perf_counter() - t0
     del dev
     items = [("json", host[int(offs[i]):int(offs[i + 1])].tobytes()) for i in range(count)]
     return items, {
@@ -233,8 +236,9 @@ def batch_workload(be, count: int, steps: int, seed: int = 3):
         "graph_ms": round(graph_s * 1e3, 3), "graph_GBps": round(total / graph_s / 1e9, 3),
         "graph_valid": graph_valid, "graph_captures": captures, "graph_replays": replays,
         "parser_shape": batch_shape,
-        "plain_ms": round(plain_s * 1e3, 3), "plain_GBps": round(total / plain_s / 1e9, 3),
-        "speedup_vs_plain": round(plain_s / batched_s, 2),
+        "plain_ms": round(plain_s * 1e3, 3) if plain else None,
+        "plain_GBps": round(total / plain_s / 1e9, 3) if plain else None,
+        "speedup_vs_plain": round(plain_s / batched_s, 2) if plain else None,
     }
 
 
@@ -384,6 +388,9 @@ def main(argv=None):
                     help="use this parser tile shape (positions per tile, per lane) instead of tuning")
     ap.add_argument("--batch-count", type=int, default=20000, help="inputs of the batching workload (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the batching workload's batched and graph calls (no 1 GiB step, plain path or "
+                         "serving; for profiling the batch path), one JSON line of its results")
     ap.add_argument("--serve-rates", default="2000,10000,50000", help="request rates of the serving workload ('' = skip)")
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--serve-capacity", default="100000,125000,150000,200000,300000",
@@ -408,6 +415,15 @@ def main(argv=None):
         return 0
     if a.copy_calibration:
         copy_calibration(a.copy_calibration, 3)
+        return 0
+    if a.batch_only:
+        from alpacc_amd import hip
+        be = hip.Backend("json", device=0)
+        if a.lexer_shape:
+            be.set_lex_shape(*[int(x) for x in a.lexer_shape.split(",")])
+        _, batched = batch_workload(be, a.batch_count, a.batch_steps, plain=False)
+        be.close()
+        print(json.dumps(batched))
         return 0
 
     import numpy as np
