@@ -501,6 +501,18 @@ def main(argv=None):
     }
     batched = None
     batch_items = serve_items = None
+    # the lexer-only workload on the same resident input (lexer numbering,
+    # tokens only): the whole launch sequence of alpacc_hip_lex_device per step
+    be.lex_device(dev.data_ptr(), n, "lexer", stream=stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        xok, xl = be.lex_device(dev.data_ptr(), n, "lexer", stream=stream)
+    torch.cuda.synchronize()
+    xms = (time.perf_counter() - t0) * 1e3 / a.steps
+    lexer_only = {"workload": f"json lexer-only, the same {n}-byte input", "ms_per_step": round(xms, 3),
+                  "GBps": round(n / (xms / 1e3) / 1e9, 3), "tokens": int(xl.count), "valid": bool(xok)}
+    log(f"lexer-only: {lexer_only}")
     if a.batch_count:
         log(f"batching workload: {a.batch_count} inputs")
         batch_items, batched = batch_workload(be, a.batch_count, a.batch_steps)
@@ -575,6 +587,7 @@ def main(argv=None):
         "kernels_ms": {k: round(v, 4) for k, v in sorted(kernels.items(), key=lambda x: -x[1])},
         "roofline": roofline,
         "literal": literal,
+        "lexer_only": lexer_only,
         "batched": batched,
         "served": served,
         "serve_capacity": capacity,
