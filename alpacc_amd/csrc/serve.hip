@@ -115,7 +115,17 @@ int pick(alpacc_hip_server* s, uint32_t g) {
 void worker(alpacc_hip_server* s, uint32_t g, alpacc_hip_ctx* ctx) {
   (void)hipSetDevice(s->device);
   std::vector<Req*> batch;
-  std::vector<uint8_t> packed;
+  // the batch's payloads, packed in pinned memory (the upload is then a
+  // direct DMA instead of the runtime's staged copy from pageable memory);
+  // reused batch after batch, each upload done before batch_records returns
+  alpacc::PinnedBuf packed;
+  // pinned buffers sized for a full batch before the first request: growing
+  // one (hipHostMalloc pins its pages) stalls for milliseconds, so it
+  // happens only for a request beyond the batch cap, not while serving
+  // ordinary ones (the outputs read back take about four bytes per input
+  // byte on JSON)
+  (void)packed.reserve(kBatchBytes + 16);
+  (void)ctx->c.h_out.reserve(4 * kBatchBytes);
   std::vector<uint64_t> off;
   std::vector<size_t> rec_off;
   for (;;) {
@@ -127,7 +137,10 @@ void worker(alpacc_hip_server* s, uint32_t g, alpacc_hip_ctx* ctx) {
       std::unique_lock<std::mutex> lk(s->mu);
       s->cv_work.wait(lk, [&] { return s->stop || pick(s, g) >= 0; });
       mode = pick(s, g);
-      if (mode < 0) return;  // stopping with nothing left for this grammar
+      if (mode < 0) {  // stopping with nothing left for this grammar
+        packed.release();
+        return;
+      }
       auto& d = s->q[g * 3 + mode];
       batch.clear();
       uint64_t bytes = 0, room = held;
@@ -158,16 +171,20 @@ void worker(alpacc_hip_server* s, uint32_t g, alpacc_hip_ctx* ctx) {
       s->growing += grow;
       if (s->in_use > s->peak_in_use) s->peak_in_use = s->in_use;
     }
-    packed.clear();
     off.assign(1, 0);
+    uint64_t total = 0;
+    for (Req* r : batch) total += r->n;
+    int prc = packed.reserve(total + 16);
     for (Req* r : batch) {
-      packed.insert(packed.end(), r->data, r->data + r->n);
-      off.push_back(packed.size());
+      if (prc == 0 && r->n) memcpy(packed.as<uint8_t>() + off.back(), r->data, r->n);
+      off.push_back(off.back() + r->n);
     }
     uint8_t* out = nullptr;
     size_t out_len = 0;
     ctx->c.use_graphs = s->graphs.load();
-    const int rc = alpacc::batch_records(ctx->c, mode, packed.data(), off.data(), batch.size(), &out, &out_len, &rec_off);
+    const int rc = prc < 0 ? prc
+                           : alpacc::batch_records(ctx->c, mode, packed.as<uint8_t>(), off.data(), batch.size(), &out,
+                                                   &out_len, &rec_off);
     const std::string err = rc < 0 ? alpacc_hip_last_error() : "";
     std::shared_ptr<uint8_t> shared(out, free);
     const Clock::time_point now = Clock::now();
