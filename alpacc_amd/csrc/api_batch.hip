@@ -243,7 +243,7 @@ int pull_all(Context& c, Pull* v, int n, hipStream_t st) {
 // `off` is rebased so that input i is bytes [off[i] - off[0],
 // off[i + 1] - off[0]) of `packed`.
 int batch_records(Context& c, int mode, const uint8_t* packed, const uint64_t* off_in, uint64_t count, uint8_t** out,
-                  size_t* out_len, std::vector<size_t>* rec_off) {
+                  size_t* out_len, std::vector<size_t>* rec_off, uint8_t* (*alloc_fn)(void*, size_t), void* alloc_ctx) {
   hipStream_t st = c.stream;
   const GrammarDesc* g = c.g;
   *out = nullptr;
@@ -256,7 +256,7 @@ int batch_records(Context& c, int mode, const uint8_t* packed, const uint64_t* o
   if (rec_off) rec_off->resize(count);
   const int ib = g->index_bytes, lb = g->length_bytes;
   auto alloc = [&](size_t n) -> uint8_t* {
-    uint8_t* p = static_cast<uint8_t*>(malloc(n ? n : 1));
+    uint8_t* p = alloc_fn ? alloc_fn(alloc_ctx, n ? n : 1) : static_cast<uint8_t*>(malloc(n ? n : 1));
     if (p) {
       *out = p;
       *out_len = n;

@@ -249,10 +249,12 @@ int lex_batch_on_device(Context& c, int mode, const uint8_t* d_in, const uint64_
                         hipStream_t st, const BatchCaps* caps = nullptr, bool enqueue = true,
                         const LexStreamStep* stream = nullptr);
 
-// Wire records of a batch of host payloads, concatenated into one malloc'd
-// buffer; rec_off (if not null) receives the start of every input's record.
+// Wire records of a batch of host payloads, concatenated into one buffer,
+// malloc'd or (alloc_fn) taken from alloc_fn(alloc_ctx, bytes); rec_off (if
+// not null) receives the start of every input's record.
 int batch_records(Context& c, int mode, const uint8_t* packed, const uint64_t* off, uint64_t count, uint8_t** out,
-                  size_t* out_len, std::vector<size_t>* rec_off);
+                  size_t* out_len, std::vector<size_t>* rec_off, uint8_t* (*alloc_fn)(void*, size_t) = nullptr,
+                  void* alloc_ctx = nullptr);
 
 // One input: returns 1/0 (valid / rejected) and fills count.
 int lex_on_device(Context& c, int mode, const uint8_t* d_in, uint64_t n, hipStream_t st, uint64_t* count);

@@ -78,6 +78,56 @@ uint64_t lane_capacity(alpacc_hip_ctx* h) {
 
 }  // namespace
 
+// Record buffers kept for reuse.  A batch's records take tens of MB; a
+// fresh block of that size is mapped from the system and unmapped when
+// freed, and the page faults of its first writes and the unmapping's TLB
+// shootdowns stall every lane's thread (a server's p99 went from ~20 ms to
+// ~2 ms at 150k req/s with large blocks kept).  So blocks go back to this
+// pool (up to kKeep of them) when their batch's last request is collected,
+// and a batch takes the first idle block large enough.  Each block starts
+// with a 16-byte header holding its capacity.
+struct RecPool {
+  static constexpr size_t kKeep = 64;
+  std::mutex mu;
+  std::vector<uint8_t*> idle;  // block bases
+  ~RecPool() {
+    for (uint8_t* b : idle) free(b);
+  }
+  static size_t cap_of(const uint8_t* base) {
+    size_t c;
+    memcpy(&c, base, sizeof c);
+    return c;
+  }
+  uint8_t* get(size_t n) {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      for (size_t i = 0; i < idle.size(); ++i)
+        if (cap_of(idle[i]) >= n) {
+          uint8_t* b = idle[i];
+          idle.erase(idle.begin() + (ptrdiff_t)i);
+          return b + 16;
+        }
+    }
+    const size_t cap = n + n / 4 + 4096;  // room for somewhat larger batches later
+    uint8_t* b = static_cast<uint8_t*>(malloc(cap + 16));
+    if (!b) return nullptr;
+    memcpy(b, &cap, sizeof cap);
+    return b + 16;
+  }
+  void put(uint8_t* p) {
+    if (!p) return;
+    uint8_t* b = p - 16;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      if (idle.size() < kKeep) {
+        idle.push_back(b);
+        return;
+      }
+    }
+    free(b);
+  }
+};
+
 struct alpacc_hip_server {
   std::vector<std::string> names;
   int device = 0;
@@ -95,6 +145,10 @@ struct alpacc_hip_server {
   uint64_t next_ticket = 1;
   uint64_t n_batches = 0, n_reqs = 0, biggest = 0, peak_in_use = 0;
   std::atomic<bool> graphs{false};  // combined batches through each lane's captured graph
+  // record blocks for reuse; requests hold their batch's block (a shared
+  // owner of the pool too, so blocks released after the server is gone
+  // still find it)
+  std::shared_ptr<RecPool> pool = std::make_shared<RecPool>();
 };
 
 namespace {
@@ -183,10 +237,13 @@ void worker(alpacc_hip_server* s, uint32_t g, alpacc_hip_ctx* ctx) {
     size_t out_len = 0;
     ctx->c.use_graphs = s->graphs.load();
     const int rc = prc < 0 ? prc
-                           : alpacc::batch_records(ctx->c, mode, packed.as<uint8_t>(), off.data(), batch.size(), &out,
-                                                   &out_len, &rec_off);
+                           : alpacc::batch_records(
+                                 ctx->c, mode, packed.as<uint8_t>(), off.data(), batch.size(), &out, &out_len, &rec_off,
+                                 [](void* pool, size_t n) { return static_cast<RecPool*>(pool)->get(n); },
+                                 s->pool.get());
     const std::string err = rc < 0 ? alpacc_hip_last_error() : "";
-    std::shared_ptr<uint8_t> shared(out, free);
+    std::shared_ptr<RecPool> pool = s->pool;
+    std::shared_ptr<uint8_t> shared(out, [pool](uint8_t* p) { pool->put(p); });
     const Clock::time_point now = Clock::now();
     {
       std::lock_guard<std::mutex> lk(s->mu);
