@@ -84,10 +84,12 @@ uint64_t lane_capacity(alpacc_hip_ctx* h) {
 // shootdowns stall every lane's thread (a server's p99 went from ~20 ms to
 // ~2 ms at 150k req/s with large blocks kept).  So blocks go back to this
 // pool (up to kKeep of them) when their batch's last request is collected,
-// and a batch takes the first idle block large enough.  Each block starts
-// with a 16-byte header holding its capacity.
+// and a batch takes the first idle block large enough.  Only large blocks
+// (kMin and up; smaller ones come from the heap and are cheap) are pooled.
+// Each block starts with a 16-byte header holding its capacity (0: not
+// pooled).
 struct RecPool {
-  static constexpr size_t kKeep = 64;
+  static constexpr size_t kKeep = 64, kMin = 8ull << 20;
   std::mutex mu;
   std::vector<uint8_t*> idle;  // block bases
   ~RecPool() {
@@ -99,6 +101,13 @@ struct RecPool {
     return c;
   }
   uint8_t* get(size_t n) {
+    if (n < kMin) {
+      uint8_t* b = static_cast<uint8_t*>(malloc(n + 16));
+      if (!b) return nullptr;
+      const size_t none = 0;
+      memcpy(b, &none, sizeof none);
+      return b + 16;
+    }
     {
       std::lock_guard<std::mutex> lk(mu);
       for (size_t i = 0; i < idle.size(); ++i)
@@ -108,7 +117,7 @@ struct RecPool {
           return b + 16;
         }
     }
-    const size_t cap = n + n / 4 + 4096;  // room for somewhat larger batches later
+    const size_t cap = n + n / 8;  // room for somewhat larger batches later
     uint8_t* b = static_cast<uint8_t*>(malloc(cap + 16));
     if (!b) return nullptr;
     memcpy(b, &cap, sizeof cap);
@@ -117,6 +126,10 @@ struct RecPool {
   void put(uint8_t* p) {
     if (!p) return;
     uint8_t* b = p - 16;
+    if (cap_of(b) == 0) {
+      free(b);
+      return;
+    }
     {
       std::lock_guard<std::mutex> lk(mu);
       if (idle.size() < kKeep) {
